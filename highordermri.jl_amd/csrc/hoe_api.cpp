@@ -1,0 +1,700 @@
+// C ABI of libhoe: validation, model preparation and the entry points
+// declared in include/hoe.h.
+//
+// Model preparation follows HighOrderMRI.jl:
+//   - order selection prep_kspha (src/EncodingOperator/prep_kspha.jl:1-56),
+//   - concomitant columns prep_concomitant (prep_kspha.jl:78-98),
+//   - the 16 unnormalised solid harmonics basisfunc_spha
+//     (src/SphericalHarmonics/basisfunction.jl:47-78), here evaluated in FP64
+//     from the FP32 grid coordinates,
+//   - column-major mask order and csm/fieldmap masking
+//     (src/EncodingOperator/Explicit/array_operator.jl:121-126),
+//   - normal-operator weight rules (explicit_gpu_operator.jl:399-416).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "hoe_internal.h"
+
+using hoe::Handle;
+
+namespace {
+
+thread_local std::string g_err;
+
+const int kCurrentWeights = HOE_CURRENT_WEIGHTS;
+
+struct Fail {
+    int code;
+};
+
+int set_err(Handle *h, int code, const std::string &msg) {
+    if (h) h->err = msg;
+    g_err = msg;
+    return code;
+}
+
+#define HOE_TRY_HIP(h, expr)                                                                      \
+    do {                                                                                          \
+        hipError_t _e = (expr);                                                                   \
+        if (_e != hipSuccess)                                                                     \
+            return set_err((h), _e == hipErrorOutOfMemory ? HOE_ENOMEM : HOE_EHIP,                \
+                           std::string("HIP error: ") + hipGetErrorString(_e) + " at " #expr);    \
+    } while (0)
+
+hipError_t dalloc(hoe::DeviceBuf &b, size_t bytes) {
+    if (bytes == 0) bytes = 256;
+    b.bytes = bytes;
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess) {
+        b.p = nullptr;
+        b.bytes = 0;
+        return e;
+    }
+    return hipMemset(b.p, 0, bytes);
+}
+
+void dfree(hoe::DeviceBuf &b) {
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+}
+
+inline int64_t round_up(int64_t a, int64_t m) { return (a + m - 1) / m * m; }
+
+// Real solid harmonics 2..16 in the order of basisfunc_spha (basisfunction.jl:47-78),
+// FP64 from FP32 coordinates. out[0..14] = b_2..b_16.
+void solid_harmonics(double x, double y, double z, double *out) {
+    const double r2 = x * x + y * y + z * z;
+    out[0] = x;                                   // 2
+    out[1] = y;                                   // 3
+    out[2] = z;                                   // 4
+    out[3] = x * y;                               // 5
+    out[4] = z * y;                               // 6
+    out[5] = 3.0 * z * z - r2;                    // 7
+    out[6] = x * z;                               // 8
+    out[7] = x * x - y * y;                       // 9
+    out[8] = 3.0 * y * x * x - y * y * y;         // 10
+    out[9] = x * z * y;                           // 11
+    out[10] = (5.0 * z * z - r2) * y;             // 12
+    out[11] = 5.0 * z * z * z - 3.0 * z * r2;     // 13
+    out[12] = (5.0 * z * z - r2) * x;             // 14
+    out[13] = x * x * z - y * y * z;              // 15
+    out[14] = x * x * x - 3.0 * x * y * y;        // 16
+}
+
+// Uniform Catmull-Rom weights at fractional offset f in [0,1) for taps
+// -1, 0, 1, 2, and their derivatives with respect to f.
+void catmull_rom(double f, double w[4], double dw[4]) {
+    const double f2 = f * f, f3 = f2 * f;
+    w[0] = 0.5 * (-f3 + 2.0 * f2 - f);
+    w[1] = 0.5 * (3.0 * f3 - 5.0 * f2 + 2.0);
+    w[2] = 0.5 * (-3.0 * f3 + 4.0 * f2 + f);
+    w[3] = 0.5 * (f3 - f2);
+    dw[0] = 0.5 * (-3.0 * f2 + 4.0 * f - 1.0);
+    dw[1] = 0.5 * (9.0 * f2 - 10.0 * f);
+    dw[2] = 0.5 * (-9.0 * f2 + 8.0 * f + 1.0);
+    dw[3] = 0.5 * (3.0 * f2 - 2.0 * f);
+}
+
+// Effective coefficient records [k_1..k_L, t] per sample (FP64) from the
+// traces, delayed by tau (shift u = tau/dwell samples).
+void build_coef_records(const Handle *h, std::vector<double> &rec) {
+    const int64_t ns = h->n_sam;
+    const int L = h->L;
+    rec.assign((size_t)h->ns_pad * h->sk, 0.0);
+    const bool shift = h->tau != 0.0;
+    double u = 0.0, f = 0.0;
+    int64_t m = 0;
+    double w[4], dw[4];
+    if (shift) {
+        u = h->tau / h->dwell;
+        m = (int64_t)std::floor(u);
+        f = u - (double)m;
+        catmull_rom(f, w, dw);
+    }
+    for (int64_t s = 0; s < ns; ++s) {
+        double *r = &rec[(size_t)s * h->sk];
+        for (int l = 0; l < L; ++l) {
+            if (!shift) {
+                r[l] = (double)h->traces[(size_t)s * L + l];
+            } else {
+                double acc = 0.0;
+                for (int t = 0; t < 4; ++t) {
+                    int64_t j = s + m - 1 + t;
+                    j = std::min<int64_t>(std::max<int64_t>(j, 0), ns - 1);
+                    acc += w[t] * (double)h->traces[(size_t)j * L + l];
+                }
+                r[l] = acc;
+            }
+        }
+        r[L] = (double)h->times[s];
+    }
+}
+
+int upload_coef(Handle *h) {
+    std::vector<double> rec;
+    build_coef_records(h, rec);
+    HOE_TRY_HIP(h, hipMemcpy(h->coef.p, rec.data(), rec.size() * sizeof(double), hipMemcpyHostToDevice));
+    return HOE_OK;
+}
+
+int check_open(Handle *h) {
+    if (!h) return set_err(nullptr, HOE_EINVAL, "null handle");
+    if (h->closed) return set_err(h, HOE_ECLOSED, "operator is closed");
+    hipError_t e = hipSetDevice(h->device);
+    if (e != hipSuccess) return set_err(h, HOE_EHIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
+    return HOE_OK;
+}
+
+// Upload normal weights D (already squared) into dvec [ns_pad][ncs].
+int set_weights(Handle *h, const float *D, int64_t n_d) {
+    const int64_t ns = h->n_sam, nc = h->n_cha;
+    if (D == nullptr && n_d == 0) {
+        if (h->weights_unit) return HOE_OK;
+    } else {
+        if (D == nullptr) return set_err(h, HOE_EINVAL, "normal weights pointer is NULL");
+        if (n_d != ns && n_d != ns * nc) return set_err(h, HOE_EDIM, "normal weights");
+        for (int64_t i = 0; i < n_d; ++i) {
+            if (!std::isfinite(D[i])) return set_err(h, HOE_EINVAL, "normal weights must be finite");
+            if (D[i] < 0.0f) return set_err(h, HOE_EINVAL, "normal weights must be nonnegative");
+        }
+    }
+    std::vector<float> w((size_t)h->ns_pad * h->ncs, 0.0f);
+    for (int64_t c = 0; c < nc; ++c)
+        for (int64_t s = 0; s < ns; ++s) {
+            float v = 1.0f;
+            if (D) v = (n_d == ns) ? D[s] : D[c * ns + s];
+            w[(size_t)s * h->ncs + c] = v;
+        }
+    HOE_TRY_HIP(h, hipMemcpy(h->dvec.p, w.data(), w.size() * sizeof(float), hipMemcpyHostToDevice));
+    h->weights_unit = (D == nullptr);
+    return HOE_OK;
+}
+
+double ev_ms(hipEvent_t a, hipEvent_t b) {
+    float ms = -1.0f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) return -1.0;
+    return (double)ms;
+}
+
+void release(Handle *h) {
+    if (!h) return;
+    (void)hipSetDevice(h->device);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    hoe::DeviceBuf *bufs[] = {&h->mask_idx, &h->basis, &h->coef, &h->dcoef, &h->csm,   &h->xm,     &h->zbuf,
+                              &h->zfrag,    &h->ybuf,  &h->yfrag, &h->part_f, &h->part_a, &h->part_g, &h->dvec,
+                              &h->io_x,     &h->io_x2, &h->io_y,  &h->io_y2,  &h->scal,   &h->cg_vec, &h->red,
+                              &h->gk,       &h->cg_log};
+    for (auto *b : bufs) dfree(*b);
+    for (auto &e : h->ev)
+        if (e) {
+            (void)hipEventDestroy(e);
+            e = nullptr;
+        }
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    h->stream = nullptr;
+    h->closed = true;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *hoe_version(void) { return "libhoe 0.1.0 (gfx950)"; }
+
+const char *hoe_last_error(const hoe_handle *h) {
+    if (h) return reinterpret_cast<const Handle *>(h)->err.c_str();
+    return g_err.c_str();
+}
+
+int hoe_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int hoe_create(const hoe_model_desc *d, hoe_handle **out) {
+    g_err.clear();
+    if (!out) return set_err(nullptr, HOE_EINVAL, "out pointer is NULL");
+    *out = nullptr;
+    if (!d) return set_err(nullptr, HOE_EINVAL, "model description is NULL");
+    if (d->n_x < 1 || d->n_y < 1 || d->n_z < 1) return set_err(nullptr, HOE_EINVAL, "grid dimensions must be positive");
+    if (!d->x || !d->y || !d->z) return set_err(nullptr, HOE_EINVAL, "grid coordinates are NULL");
+    if (d->n_term != 9 && d->n_term != 16) return set_err(nullptr, HOE_EINVAL, "kspha requires 9 or 16 terms");
+    if (d->n_sam < 1 || d->n_cha < 1) return set_err(nullptr, HOE_EINVAL, "samples and coils must be positive");
+    if (!d->kspha || !d->times) return set_err(nullptr, HOE_EINVAL, "kspha and times are required");
+    if (!d->csm && d->n_cha != 1) return set_err(nullptr, HOE_EDIM, "csm and grid");
+    if (d->backend < HOE_BACKEND_DIRECT || d->backend > HOE_BACKEND_MIXED_BF16)
+        return set_err(nullptr, HOE_EINVAL, "unknown explicit backend");
+    if (!(d->dwell >= 0.0) || !std::isfinite(d->dwell)) return set_err(nullptr, HOE_EINVAL, "dwell must be >= 0");
+    const int n_term = d->n_term;
+    // order flags (prep_kspha.jl:8-33)
+    bool t[4] = {true, true, true, n_term == 16};
+    if (d->recon_terms) {
+        const size_t expect = n_term == 9 ? 3 : 4;
+        const size_t len = std::strlen(d->recon_terms);
+        if (len != expect)
+            return set_err(nullptr, HOE_EINVAL,
+                           "recon_terms must contain " + std::to_string(expect) + " binary digits for nTerm=" +
+                               std::to_string(n_term));
+        for (size_t i = 0; i < len; ++i) {
+            const char c = d->recon_terms[i];
+            if (c != '0' && c != '1') return set_err(nullptr, HOE_EINVAL, "recon_terms must contain only '0' and '1'");
+            t[i] = (c == '1');
+        }
+        if (n_term == 9) t[3] = false;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_err(nullptr, HOE_ENODEV, "no HIP device");
+    if (d->device < 0 || d->device >= ndev) return set_err(nullptr, HOE_EINVAL, "invalid visible GPU ID");
+
+    const int64_t n_grid = (int64_t)d->n_x * d->n_y * d->n_z;
+    std::vector<int32_t> idx;
+    idx.reserve((size_t)n_grid);
+    for (int64_t i = 0; i < n_grid; ++i)
+        if (!d->mask || d->mask[i]) idx.push_back((int32_t)i);
+    if (idx.empty()) return set_err(nullptr, HOE_EINVAL, "mask must contain voxels");
+    if (n_grid > INT32_MAX) return set_err(nullptr, HOE_EINVAL, "grid too large");
+
+    Handle *h = new (std::nothrow) Handle();
+    if (!h) return set_err(nullptr, HOE_ENOMEM, "host allocation failed");
+    h->n_grid = n_grid;
+    h->n_vox = (int64_t)idx.size();
+    h->n_sam = d->n_sam;
+    h->n_cha = d->n_cha;
+    h->n_term_in = n_term;
+    h->L = n_term + (d->concomitant ? 4 : 0);
+    h->nv_pad = round_up(h->n_vox, hoe::kRowPad);
+    h->ns_pad = round_up(h->n_sam, hoe::kRowPad);
+    // coil stride: one 32-coil MFMA group, or whole pairs of groups (the mixed kernels keep two
+    // groups per wave when there are more than 32 coils)
+    h->ncs = (int32_t)(h->n_cha <= 32 ? 32 : round_up(h->n_cha, 64));
+    h->sb = (int32_t)round_up(h->L, 8);
+    h->sk = (int32_t)round_up(h->L + 1, 8);
+    h->backend = d->backend;
+    h->device = d->device;
+    h->dwell = d->dwell;
+    h->mask_idx_h = idx;
+
+    // ---- prepared coefficient traces (term fastest), prep_kspha + prep_concomitant ----
+    const int L = h->L;
+    const int64_t ns = h->n_sam;
+    h->traces.assign((size_t)L * ns, 0.0f);
+    for (int64_t s = 0; s < ns; ++s) {
+        float *r = &h->traces[(size_t)s * L];
+        for (int l = 0; l < n_term; ++l) r[l] = d->kspha[(size_t)s * n_term + l];
+        if (!t[0]) r[0] = 0.0f;
+        if (!t[1] && d->k_nominal)
+            for (int l = 0; l < 3; ++l) r[1 + l] = d->k_nominal[(size_t)s * 3 + l];
+        if (!t[2])
+            for (int l = 4; l < 9; ++l) r[l] = 0.0f;
+        if (!t[3] && n_term == 16)
+            for (int l = 9; l < 16; ++l) r[l] = 0.0f;
+        if (d->concomitant)
+            for (int l = 0; l < 4; ++l) r[n_term + l] = d->concomitant[(size_t)s * 4 + l];
+    }
+    h->times.assign(d->times, d->times + ns);
+
+    // ---- voxel records [b_2..b_L, fieldmap] in FP64 ----
+    const int64_t nv = h->n_vox;
+    std::vector<double> brec((size_t)h->nv_pad * h->sb, 0.0);
+    float csm_max = 0.0f;
+    std::vector<float2> csmv((size_t)h->nv_pad * h->ncs, make_float2(0.0f, 0.0f));
+    for (int64_t v = 0; v < nv; ++v) {
+        const int64_t g = idx[(size_t)v];
+        const double x = d->x[g], y = d->y[g], z = d->z[g];
+        double sh[15];
+        solid_harmonics(x, y, z, sh);
+        double *r = &brec[(size_t)v * h->sb];
+        for (int l = 1; l < n_term; ++l) r[l - 1] = sh[l - 1];
+        if (d->concomitant) {
+            r[n_term - 1 + 0] = z * z;
+            r[n_term - 1 + 1] = x * x + y * y;
+            r[n_term - 1 + 2] = x * z;
+            r[n_term - 1 + 3] = y * z;
+        }
+        r[L - 1] = d->fieldmap ? (double)d->fieldmap[g] : 0.0;
+        for (int64_t c = 0; c < h->n_cha; ++c) {
+            float2 val = make_float2(1.0f, 0.0f);
+            if (d->csm) val = make_float2(d->csm[2 * (c * n_grid + g)], d->csm[2 * (c * n_grid + g) + 1]);
+            csmv[(size_t)v * h->ncs + c] = val;
+            csm_max = std::max(csm_max, std::sqrt(val.x * val.x + val.y * val.y));
+        }
+    }
+    h->csm_max = csm_max;
+
+    // ---- work decomposition: ~4 waves per SIMD per contraction launch ----
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, h->device) == hipSuccess && prop.multiProcessorCount > 0)
+        h->cu_count = prop.multiProcessorCount;
+    const int64_t target_wg = (int64_t)h->cu_count * 4;   // 256-thread workgroups
+    {
+        const int64_t tiles = h->ns_pad / 256;
+        int64_t splits = std::max<int64_t>(1, (target_wg + tiles - 1) / tiles);
+        splits = std::min<int64_t>(splits, h->nv_pad / 64);
+        h->vox_per_split = (int)round_up((h->nv_pad + splits - 1) / splits, 64);
+        h->splits_f = (int)((h->nv_pad + h->vox_per_split - 1) / h->vox_per_split);
+    }
+    {
+        const int64_t tiles = h->nv_pad / 256;
+        int64_t splits = std::max<int64_t>(1, (target_wg + tiles - 1) / tiles);
+        splits = std::min<int64_t>(splits, h->ns_pad / 64);
+        h->sam_per_split = (int)round_up((h->ns_pad + splits - 1) / splits, 64);
+        h->splits_a = (int)((h->ns_pad + h->sam_per_split - 1) / h->sam_per_split);
+    }
+    {
+        const int64_t tiles = h->ns_pad / 256;
+        int64_t splits = std::max<int64_t>(1, (target_wg + tiles - 1) / tiles);
+        splits = std::min<int64_t>(splits, h->nv_pad / 64);
+        h->vox_per_split_g = (int)round_up((h->nv_pad + splits - 1) / splits, 64);
+        h->splits_g = (int)((h->nv_pad + h->vox_per_split_g - 1) / h->vox_per_split_g);
+    }
+
+    // ---- device allocation ----
+    auto fail = [&](int code) {
+        std::string msg = g_err;
+        release(h);
+        delete h;
+        g_err = msg;
+        return code;
+    };
+    hipError_t e = hipSetDevice(h->device);
+    if (e != hipSuccess) return fail(set_err(nullptr, HOE_EHIP, "hipSetDevice failed"));
+    if ((e = hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking)) != hipSuccess)
+        return fail(set_err(nullptr, HOE_EHIP, "hipStreamCreate failed"));
+    for (auto &ev : h->ev)
+        if ((e = hipEventCreate(&ev)) != hipSuccess) return fail(set_err(nullptr, HOE_EHIP, "hipEventCreate failed"));
+    const int cbn = (h->n_cha <= 8) ? 8 : (h->n_cha <= 16 ? 16 : 32);
+    const int64_t n_batches_g = (h->n_cha + cbn - 1) / cbn;
+    const int n_gb = h->ncs / 32;
+    const int nbw = n_gb >= 2 ? 2 : 1;
+    const int64_t n_gb_pad = ((n_gb + nbw - 1) / nbw) * nbw;
+    struct {
+        hoe::DeviceBuf *b;
+        size_t bytes;
+    } plan[] = {
+        {&h->mask_idx, sizeof(int32_t) * (size_t)nv},
+        {&h->basis, sizeof(double) * brec.size()},
+        {&h->coef, sizeof(double) * (size_t)h->ns_pad * h->sk},
+        {&h->csm, sizeof(float2) * csmv.size()},
+        {&h->xm, sizeof(float2) * (size_t)h->nv_pad},
+        {&h->zbuf, sizeof(float2) * (size_t)h->nv_pad * h->ncs},
+        {&h->zfrag, (size_t)n_gb_pad * (size_t)(h->nv_pad / 8) * 64 * 64},
+        {&h->ybuf, sizeof(float2) * (size_t)h->ns_pad * h->ncs},
+        {&h->yfrag, (size_t)n_gb_pad * (size_t)(h->ns_pad / 8) * 64 * 64},
+        {&h->part_f, sizeof(float2) * (size_t)h->splits_f * h->ns_pad * h->ncs},
+        {&h->part_a, sizeof(float2) * (size_t)h->splits_a * h->nv_pad * h->ncs},
+        {&h->part_g, sizeof(double) * (size_t)h->splits_g * n_batches_g * h->ns_pad * L},
+        {&h->dvec, sizeof(float) * (size_t)h->ns_pad * h->ncs},
+        {&h->io_x, sizeof(float2) * (size_t)n_grid},
+        {&h->io_x2, sizeof(float2) * (size_t)n_grid},
+        {&h->io_y, sizeof(float2) * (size_t)(ns * h->n_cha)},
+        {&h->io_y2, sizeof(float2) * (size_t)(ns * h->n_cha)},
+        {&h->scal, 4096},
+        {&h->cg_vec, sizeof(float2) * (size_t)h->nv_pad * 4},
+        {&h->red, sizeof(double) * 4096},
+        {&h->gk, sizeof(double) * (size_t)L * ns},
+    };
+    for (auto &p : plan) {
+        e = dalloc(*p.b, p.bytes);
+        if (e != hipSuccess)
+            return fail(set_err(nullptr, e == hipErrorOutOfMemory ? HOE_ENOMEM : HOE_EHIP,
+                                std::string("device allocation failed: ") + hipGetErrorString(e)));
+    }
+    e = hipMemcpy(h->mask_idx.p, idx.data(), sizeof(int32_t) * nv, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(h->basis.p, brec.data(), sizeof(double) * brec.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(h->csm.p, csmv.data(), sizeof(float2) * csmv.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return fail(set_err(nullptr, HOE_EHIP, std::string("upload failed: ") + hipGetErrorString(e)));
+    int rc = upload_coef(h);
+    if (rc != HOE_OK) return fail(set_err(nullptr, rc, h->err));
+    h->weights_unit = false;
+    rc = set_weights(h, nullptr, 0);
+    if (rc != HOE_OK) return fail(set_err(nullptr, rc, h->err));
+    *out = reinterpret_cast<hoe_handle *>(h);
+    return HOE_OK;
+}
+
+int hoe_get_info(const hoe_handle *hh, hoe_info_t *info) {
+    const Handle *h = reinterpret_cast<const Handle *>(hh);
+    if (!h || !info) return set_err(nullptr, HOE_EINVAL, "null argument");
+    info->n_grid = h->n_grid;
+    info->n_vox = h->n_vox;
+    info->n_sam = h->n_sam;
+    info->n_cha = h->n_cha;
+    info->n_term = h->L;
+    info->backend = h->backend;
+    info->device = h->device;
+    info->closed = h->closed ? 1 : 0;
+    return HOE_OK;
+}
+
+int hoe_set_backend(hoe_handle *hh, int32_t backend) {
+    Handle *h = reinterpret_cast<Handle *>(hh);
+    int rc = check_open(h);
+    if (rc) return rc;
+    if (backend < HOE_BACKEND_DIRECT || backend > HOE_BACKEND_MIXED_BF16)
+        return set_err(h, HOE_EINVAL, "unknown explicit backend");
+    h->backend = backend;
+    return HOE_OK;
+}
+
+// ---- vector staging helpers ----
+namespace {
+
+struct Staging {
+    Handle *h;
+    int flags;
+    bool dev() const { return (flags & HOE_DEVICE_PTRS) != 0; }
+};
+
+int stage_in(Handle *h, int flags, const void *src, hoe::DeviceBuf &buf, size_t bytes, const void **dptr) {
+    if (!src) return set_err(h, HOE_EINVAL, "vector pointer is NULL");
+    if (flags & HOE_DEVICE_PTRS) {
+        *dptr = src;
+        return HOE_OK;
+    }
+    HOE_TRY_HIP(h, hipMemcpyAsync(buf.p, src, bytes, hipMemcpyHostToDevice, h->stream));
+    *dptr = buf.p;
+    return HOE_OK;
+}
+
+int finish(Handle *h, int flags, void *dst_host, const void *dsrc, size_t bytes) {
+    if (!(flags & HOE_DEVICE_PTRS) && dst_host)
+        HOE_TRY_HIP(h, hipMemcpyAsync(dst_host, dsrc, bytes, hipMemcpyDeviceToHost, h->stream));
+    (void)hipEventRecord(h->ev[5], h->stream);
+    if (!(flags & HOE_ASYNC) || !(flags & HOE_DEVICE_PTRS)) HOE_TRY_HIP(h, hipStreamSynchronize(h->stream));
+    HOE_TRY_HIP(h, hipGetLastError());
+    return HOE_OK;
+}
+
+void record_times(Handle *h, bool fwd, bool adj) {
+    h->last_ms[0] = fwd ? ev_ms(h->ev[0], h->ev[1]) : -1.0;
+    h->last_ms[1] = adj ? ev_ms(h->ev[2], h->ev[3]) : -1.0;
+    h->last_ms[2] = ev_ms(h->ev[4], h->ev[5]);
+}
+
+}  // namespace
+
+int hoe_forward(hoe_handle *hh, const void *x, void *y, int flags) {
+    Handle *h = reinterpret_cast<Handle *>(hh);
+    int rc = check_open(h);
+    if (rc) return rc;
+    if (!y) return set_err(h, HOE_EINVAL, "output pointer is NULL");
+    (void)hipEventRecord(h->ev[4], h->stream);
+    const void *dx;
+    if ((rc = stage_in(h, flags, x, h->io_x, sizeof(float2) * h->n_grid, &dx))) return rc;
+    void *dy = (flags & HOE_DEVICE_PTRS) ? y : h->io_y.p;
+    HOE_TRY_HIP(h, hoe::run_forward_contract(h, dx, true));
+    HOE_TRY_HIP(h, hoe::launch_reduce_fwd_user(h, dy, (float)(1.0 / std::sqrt((double)h->n_vox))));
+    if ((rc = finish(h, flags, y, dy, sizeof(float2) * h->n_sam * h->n_cha))) return rc;
+    if (!(flags & HOE_ASYNC)) record_times(h, true, false);
+    return HOE_OK;
+}
+
+int hoe_adjoint(hoe_handle *hh, const void *y, void *x, int flags) {
+    Handle *h = reinterpret_cast<Handle *>(hh);
+    int rc = check_open(h);
+    if (rc) return rc;
+    if (!x) return set_err(h, HOE_EINVAL, "output pointer is NULL");
+    (void)hipEventRecord(h->ev[4], h->stream);
+    const void *dy;
+    if ((rc = stage_in(h, flags, y, h->io_y, sizeof(float2) * h->n_sam * h->n_cha, &dy))) return rc;
+    void *dx = (flags & HOE_DEVICE_PTRS) ? x : h->io_x.p;
+    HOE_TRY_HIP(h, hoe::launch_load_ybuf(h, dy, false));
+    HOE_TRY_HIP(h, hoe::run_adjoint_contract(h, (float)(1.0 / std::sqrt((double)h->n_vox)), h->xm.p, true));
+    HOE_TRY_HIP(h, hoe::launch_scatter(h, h->xm.p, dx, 1.0f));
+    if ((rc = finish(h, flags, x, dx, sizeof(float2) * h->n_grid))) return rc;
+    if (!(flags & HOE_ASYNC)) record_times(h, false, true);
+    return HOE_OK;
+}
+
+int hoe_set_weights(hoe_handle *hh, const float *D, int64_t n_d) {
+    Handle *h = reinterpret_cast<Handle *>(hh);
+    int rc = check_open(h);
+    if (rc) return rc;
+    return set_weights(h, D, n_d);
+}
+
+int hoe_normal(hoe_handle *hh, const float *D, int64_t n_d, const void *x, void *out, int flags) {
+    Handle *h = reinterpret_cast<Handle *>(hh);
+    int rc = check_open(h);
+    if (rc) return rc;
+    if (!out) return set_err(h, HOE_EINVAL, "output pointer is NULL");
+    if (!(D == nullptr && n_d == kCurrentWeights))
+        if ((rc = set_weights(h, D, n_d))) return rc;
+    (void)hipEventRecord(h->ev[4], h->stream);
+    const void *dx;
+    if ((rc = stage_in(h, flags, x, h->io_x, sizeof(float2) * h->n_grid, &dx))) return rc;
+    void *dout = (flags & HOE_DEVICE_PTRS) ? out : h->io_x2.p;
+    HOE_TRY_HIP(h, hoe::launch_gather(h, dx, h->xm.p));
+    float2 *tmp = reinterpret_cast<float2 *>(h->cg_vec.p);   // masked output
+    HOE_TRY_HIP(h, hoe::run_normal_masked(h, h->xm.p, tmp, (float)(1.0 / (double)h->n_vox), 0.0, true));
+    HOE_TRY_HIP(h, hoe::launch_scatter(h, tmp, dout, 1.0f));
+    if ((rc = finish(h, flags, out, dout, sizeof(float2) * h->n_grid))) return rc;
+    if (!(flags & HOE_ASYNC)) record_times(h, true, true);
+    return HOE_OK;
+}
+
+int hoe_set_coefficients(hoe_handle *hh, const float *kspha, double tau) {
+    Handle *h = reinterpret_cast<Handle *>(hh);
+    int rc = check_open(h);
+    if (rc) return rc;
+    if (!std::isfinite(tau)) return set_err(h, HOE_EINVAL, "tau must be finite");
+    if (tau != 0.0 && !(h->dwell > 0.0)) return set_err(h, HOE_EINVAL, "a delay requires dwell > 0");
+    if (kspha) std::memcpy(h->traces.data(), kspha, sizeof(float) * h->traces.size());
+    h->tau = tau;
+    HOE_TRY_HIP(h, hipStreamSynchronize(h->stream));
+    return upload_coef(h);
+}
+
+int hoe_grad(hoe_handle *hh, const void *x, const void *y, float *g_h, double *g_tau, double *loss, int flags) {
+    Handle *h = reinterpret_cast<Handle *>(hh);
+    int rc = check_open(h);
+    if (rc) return rc;
+    (void)hipEventRecord(h->ev[4], h->stream);
+    const void *dx, *dy;
+    if ((rc = stage_in(h, flags, x, h->io_x, sizeof(float2) * h->n_grid, &dx))) return rc;
+    if ((rc = stage_in(h, flags, y, h->io_y, sizeof(float2) * h->n_sam * h->n_cha, &dy))) return rc;
+    // residual r = E x - y into ybuf (coil fastest)
+    HOE_TRY_HIP(h, hoe::run_forward_contract(h, dx, false));
+    HOE_TRY_HIP(h, hoe::launch_reduce_fwd_ybuf(h, (float)(1.0 / std::sqrt((double)h->n_vox)), false, dy));
+    double *loss_dev = reinterpret_cast<double *>(h->scal.p) + 16;
+    HOE_TRY_HIP(h, hoe::launch_norm2(h, h->ybuf.p, h->ns_pad * h->ncs, loss_dev));
+    HOE_TRY_HIP(h, hoe::launch_build_zbuf(h, h->xm.p));
+    HOE_TRY_HIP(h, hoe::run_grad_contract(h, true));
+    (void)hipEventRecord(h->ev[5], h->stream);
+    const int L = h->L;
+    const int64_t ns = h->n_sam;
+    std::vector<double> gk((size_t)L * ns);
+    double lossv = 0.0;
+    HOE_TRY_HIP(h, hipMemcpyAsync(gk.data(), h->gk.p, sizeof(double) * gk.size(), hipMemcpyDeviceToHost, h->stream));
+    HOE_TRY_HIP(h, hipMemcpyAsync(&lossv, loss_dev, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HOE_TRY_HIP(h, hipStreamSynchronize(h->stream));
+    h->last_ms[0] = ev_ms(h->ev[0], h->ev[1]);
+    h->last_ms[1] = -1.0;
+    h->last_ms[2] = ev_ms(h->ev[4], h->ev[5]);
+    if (loss) *loss = lossv;
+    // chain rule through the delay interpolation (host, FP64)
+    const bool shift = h->tau != 0.0;
+    double w[4] = {0, 1, 0, 0}, dw[4] = {0, 0, 0, 0};
+    int64_t m = 0;
+    if (shift) {
+        const double u = h->tau / h->dwell;
+        m = (int64_t)std::floor(u);
+        catmull_rom(u - (double)m, w, dw);
+    }
+    std::vector<double> gh;
+    if (g_h) gh.assign((size_t)L * ns, 0.0);
+    double gt = 0.0;
+    for (int64_t s = 0; s < ns; ++s)
+        for (int l = 0; l < L; ++l) {
+            const double g = gk[(size_t)l * ns + s];
+            if (!shift) {
+                if (g_h) gh[(size_t)s * L + l] += g;
+                continue;
+            }
+            double dk = 0.0;
+            for (int t = 0; t < 4; ++t) {
+                int64_t j = s + m - 1 + t;
+                j = std::min<int64_t>(std::max<int64_t>(j, 0), ns - 1);
+                if (g_h) gh[(size_t)j * L + l] += w[t] * g;
+                dk += dw[t] * (double)h->traces[(size_t)j * L + l];
+            }
+            gt += g * dk / h->dwell;
+        }
+    if (!shift && g_tau) {
+        // d k / d tau at tau = 0 from the same interpolant: u = 0 -> f = 0
+        double w0[4], dw0[4];
+        catmull_rom(0.0, w0, dw0);
+        if (h->dwell > 0.0) {
+            for (int64_t s = 0; s < ns; ++s)
+                for (int l = 0; l < L; ++l) {
+                    double dk = 0.0;
+                    for (int t = 0; t < 4; ++t) {
+                        int64_t j = s - 1 + t;
+                        j = std::min<int64_t>(std::max<int64_t>(j, 0), ns - 1);
+                        dk += dw0[t] * (double)h->traces[(size_t)j * L + l];
+                    }
+                    gt += gk[(size_t)l * ns + s] * dk / h->dwell;
+                }
+        } else {
+            gt = NAN;
+        }
+    }
+    if (g_h)
+        for (size_t i = 0; i < gh.size(); ++i) g_h[i] = (float)gh[i];
+    if (g_tau) *g_tau = gt;
+    return HOE_OK;
+}
+
+int hoe_cg(hoe_handle *hh, const void *y, const float *D, int64_t n_d, int32_t iters, double lambda, double rel_tol,
+           void *x_out, double *resid_log, int flags) {
+    Handle *h = reinterpret_cast<Handle *>(hh);
+    int rc = check_open(h);
+    if (rc) return rc;
+    if (!x_out) return set_err(h, HOE_EINVAL, "output pointer is NULL");
+    if (iters < 0) return set_err(h, HOE_EINVAL, "iterations must be nonnegative");
+    if (!std::isfinite(lambda) || lambda < 0.0) return set_err(h, HOE_EINVAL, "lambda must be finite and >= 0");
+    if (!(D == nullptr && n_d == kCurrentWeights))
+        if ((rc = set_weights(h, D, n_d))) return rc;
+    const size_t log_bytes = sizeof(double) * ((size_t)iters + 1);
+    if (h->cg_log.bytes < log_bytes) {
+        dfree(h->cg_log);
+        HOE_TRY_HIP(h, dalloc(h->cg_log, log_bytes));
+    }
+    (void)hipEventRecord(h->ev[4], h->stream);
+    const void *dy;
+    if ((rc = stage_in(h, flags, y, h->io_y, sizeof(float2) * h->n_sam * h->n_cha, &dy))) return rc;
+    void *dx = (flags & HOE_DEVICE_PTRS) ? x_out : h->io_x.p;
+    // rhs = E^H D y (masked) into xm
+    HOE_TRY_HIP(h, hoe::launch_load_ybuf(h, dy, true));
+    HOE_TRY_HIP(h, hoe::run_adjoint_contract(h, (float)(1.0 / std::sqrt((double)h->n_vox)), h->xm.p, false));
+    float2 *xs = reinterpret_cast<float2 *>(h->cg_vec.p) + 3 * h->nv_pad;
+    HOE_TRY_HIP(h, hoe::cg_run(h, h->xm.p, iters, lambda, rel_tol, xs, (double *)h->cg_log.p));
+    HOE_TRY_HIP(h, hoe::launch_scatter(h, xs, dx, 1.0f));
+    if (resid_log)
+        HOE_TRY_HIP(h, hipMemcpyAsync(resid_log, h->cg_log.p, log_bytes, hipMemcpyDeviceToHost, h->stream));
+    if ((rc = finish(h, flags, x_out, dx, sizeof(float2) * h->n_grid))) return rc;
+    if (resid_log && (flags & HOE_ASYNC) && (flags & HOE_DEVICE_PTRS)) HOE_TRY_HIP(h, hipStreamSynchronize(h->stream));
+    h->last_ms[0] = h->last_ms[1] = -1.0;
+    if (!(flags & HOE_ASYNC)) h->last_ms[2] = ev_ms(h->ev[4], h->ev[5]);
+    return HOE_OK;
+}
+
+int hoe_close(hoe_handle *hh) {
+    Handle *h = reinterpret_cast<Handle *>(hh);
+    if (!h) return set_err(nullptr, HOE_EINVAL, "null handle");
+    if (h->closed) return HOE_OK;
+    release(h);
+    return HOE_OK;
+}
+
+void hoe_destroy(hoe_handle *hh) {
+    Handle *h = reinterpret_cast<Handle *>(hh);
+    if (!h) return;
+    if (!h->closed) release(h);
+    delete h;
+}
+
+int hoe_sync(hoe_handle *hh) {
+    Handle *h = reinterpret_cast<Handle *>(hh);
+    int rc = check_open(h);
+    if (rc) return rc;
+    HOE_TRY_HIP(h, hipStreamSynchronize(h->stream));
+    return HOE_OK;
+}
+
+double hoe_last_kernel_ms(const hoe_handle *hh, int which) {
+    const Handle *h = reinterpret_cast<const Handle *>(hh);
+    if (!h || which < 0 || which > 2) return -1.0;
+    return h->last_ms[which];
+}
+
+}  // extern "C"

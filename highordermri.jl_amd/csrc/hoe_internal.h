@@ -1,0 +1,114 @@
+// Internal state of a libhoe handle and the launchers shared by the C ABI
+// (hoe_api.cpp) and the kernels (hoe_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/hoe.h"
+
+namespace hoe {
+
+// Work decomposition constants.
+constexpr int kWG = 256;        // threads per workgroup (4 wave64)
+constexpr int kRowPad = 256;    // samples and voxels are padded to whole workgroups
+constexpr int kKStep = 8;       // voxels (forward) or samples (adjoint) per MFMA K step
+constexpr int kMaxTerms = 20;   // 16 solid harmonics + 4 concomitant columns
+
+struct DeviceBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+struct Handle {
+    // ---- model sizes ----
+    int64_t n_grid = 0, n_vox = 0, n_sam = 0, n_cha = 0;
+    int64_t nv_pad = 0, ns_pad = 0;   // padded to kRowPad
+    int32_t L = 0;                    // effective terms (b_1 = 1 included)
+    int32_t n_term_in = 0;            // 9 or 16
+    int32_t ncs = 0;                  // coil stride of [row][coil] buffers: 32*ceil(n_cha/32)
+    int32_t sb = 0;                   // doubles per voxel record: [b_2..b_L, fieldmap]
+    int32_t sk = 0;                   // doubles per sample record: [k_1..k_L, t]
+    int32_t backend = HOE_BACKEND_DIRECT;
+    int32_t device = 0;
+    int32_t cu_count = 256;
+    bool closed = false;
+    bool weights_unit = false;        // dvec holds unit weights
+    float csm_max = 0.0f;             // max |Re|,|Im| bound of the masked coil maps: max |csm|
+    double dwell = 0.0, tau = 0.0;
+    std::string err;
+
+    // ---- host copies needed to rebuild coefficients ----
+    std::vector<float> traces;        // L x n_sam, term fastest (prepared coefficients)
+    std::vector<float> times;         // n_sam
+    std::vector<int32_t> mask_idx_h;  // n_vox
+    std::vector<double> basis_h;      // n_vox x L: [1, b_2..b_L] used for gradient scaling on host
+
+    // ---- device model ----
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[6] = {};
+    DeviceBuf mask_idx;   // int32 [n_vox]
+    DeviceBuf basis;      // double [nv_pad][sb]
+    DeviceBuf coef;       // double [ns_pad][sk]
+    DeviceBuf dcoef;      // double [ns_pad][sk] d k / d tau (for the delay gradient)
+    DeviceBuf csm;        // float2 [nv_pad][ncs]
+
+    // ---- workspaces ----
+    DeviceBuf xm;         // float2 [nv_pad]        masked image
+    DeviceBuf zbuf;       // float2 [nv_pad][ncs]   csm .* x (direct path)
+    DeviceBuf zfrag;      // MFMA B fragments of csm .* x (mixed path)
+    DeviceBuf ybuf;       // float2 [ns_pad][ncs]   weighted signal, coil fastest
+    DeviceBuf yfrag;      // MFMA B fragments of the weighted signal (mixed path)
+    DeviceBuf part_f;     // float2 [splits_f][ns_pad][ncs]
+    DeviceBuf part_a;     // float2 [splits_a][nv_pad][ncs]
+    DeviceBuf part_g;     // double [splits_g][ns_pad][L] gradient partials
+    DeviceBuf dvec;       // float  [ns_pad][ncs] normal weights (coil fastest)
+    DeviceBuf io_x;       // float2 [n_grid]  staging for host vectors
+    DeviceBuf io_x2;      // float2 [n_grid]
+    DeviceBuf io_y;       // float2 [n_sam*n_cha]
+    DeviceBuf io_y2;      // float2 [n_sam*n_cha]
+    DeviceBuf scal;       // small scalars: scales, CG dot products
+    DeviceBuf cg_vec;     // float2 [4][nv_pad] CG vectors x, r, p, q
+    DeviceBuf red;        // double [4096] block partial sums
+    DeviceBuf gk;         // double [L][n_sam] dL/dk reduced
+    DeviceBuf cg_log;     // double [iters+1] CG relative residuals
+
+    int splits_f = 1, splits_a = 1, splits_g = 1;
+    int vox_per_split = 0, sam_per_split = 0, vox_per_split_g = 0;
+
+    // timing of the last call (ms)
+    double last_ms[3] = {-1, -1, -1};
+};
+
+// ---- launchers (hoe_kernels.hip) ----
+// All run on h->stream and return hipSuccess or the first HIP error.
+hipError_t launch_gather(Handle *h, const void *x_full, void *xm);
+hipError_t launch_scatter(Handle *h, const void *xm, void *x_full, float scale);
+// Forward: x_full (device, n_grid complex) -> part_f.  Uses the current backend.
+hipError_t run_forward_contract(Handle *h, const void *x_full, bool time_it);
+// Reduce part_f into the user signal layout (complex [c][s]) with scale.
+hipError_t launch_reduce_fwd_user(Handle *h, void *y_out, float scale);
+// Reduce part_f into ybuf (coil fastest) with scale and optional weights dvec
+// (weights == true) and optional subtraction of data (y_data != nullptr, user layout).
+hipError_t launch_reduce_fwd_ybuf(Handle *h, float scale, bool weights, const void *y_data);
+// Load a user signal (complex [c][s]) into ybuf (coil fastest), optionally weighted.
+hipError_t launch_load_ybuf(Handle *h, const void *y_user, bool weights);
+// Adjoint contraction from ybuf -> part_a, then coil reduction into xm (masked) with scale.
+hipError_t run_adjoint_contract(Handle *h, float scale, void *xm_out, bool time_it);
+// Normal operator on masked vectors: out_m = scale * E^H D E x_m (+ lambda x_m).
+hipError_t run_normal_masked(Handle *h, const void *xm_in, void *xm_out, float scale, double lambda,
+                             bool time_it);
+// Gradient partial kernel: uses zbuf (csm .* x) and ybuf (residual, coil fastest) -> gk.
+hipError_t run_grad_contract(Handle *h, bool time_it);
+// zbuf = csm .* xm
+hipError_t launch_build_zbuf(Handle *h, const void *xm);
+// CG helper kernels.
+hipError_t cg_run(Handle *h, const void *rhs_m, int iters, double lambda, double rel_tol,
+                  void *x_m, double *resid_log_dev);
+// squared norm of a complex vector (n entries) into a device double.
+hipError_t launch_norm2(Handle *h, const void *v, int64_t n, double *out_dev);
+
+}  // namespace hoe

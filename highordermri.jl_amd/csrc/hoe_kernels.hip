@@ -1,0 +1,1030 @@
+// gfx950 kernels of libhoe: the Explicit higher-order encoding operator.
+//
+// Phase model (HighOrderMRI.jl docs/src/theory/encoding-model.md:77-123,
+// src/EncodingOperator/Explicit/array_operator.jl:269-271):
+//   phi(s,v) = A_s[0] + sum_{i=1..L} A_s[i] * B_v[i-1]
+// with the sample record A_s = [k_1(s), ..., k_L(s), t_s] and the voxel record
+// B_v = [b_2(v), ..., b_L(v), fieldmap(v)] (b_1 = 1 is folded into A_s[0]).
+// phi is accumulated in FP64 (every product of the FP32 inputs is exact in
+// FP64), reduced modulo one cycle in FP64, and its sine and cosine are
+// evaluated in FP32 by v_sin_f32 / v_cos_f32, which take revolutions.
+//
+// Work decomposition. One lane owns one "stationary" row (a sample for the
+// forward, a voxel for the adjoint) and keeps its record in VGPRs; the other
+// index is "streamed": its record has the same address in every lane, so it
+// is fetched with scalar loads and broadcast through SGPRs. The streamed range
+// is split across workgroups (split-K); partial sums go to a [split][row][coil]
+// buffer and a second kernel reduces them in a fixed order (deterministic).
+// Workgroup ids are remapped so that all workgroups of one split land on one
+// XCD (blocks are dealt to the 8 XCDs round-robin), keeping that split's
+// streamed data resident in that XCD's L2.
+#include "hoe_internal.h"
+
+#include <algorithm>
+#include <cmath>
+
+namespace hoe {
+
+typedef _Float16 h8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 b8_t __attribute__((ext_vector_type(8)));
+typedef float f16x_t __attribute__((ext_vector_type(16)));
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+
+// Map blockIdx.x to (tile, split): blocks g = xcd*chunk + j, split-major, so
+// that consecutive tiles of one split share an XCD.
+__device__ __forceinline__ bool map_block(int n_tiles, int n_splits, int &tile, int &split) {
+    const int total = n_tiles * n_splits;
+    const int chunk = (total + 7) >> 3;
+    const int id = blockIdx.x;
+    const int g = (id & 7) * chunk + (id >> 3);
+    if (g >= total) return false;
+    split = g / n_tiles;
+    tile = g - split * n_tiles;
+    return true;
+}
+
+static inline int grid_blocks(int n_tiles, int n_splits) {
+    const int total = n_tiles * n_splits;
+    return 8 * ((total + 7) / 8);
+}
+
+// Fractional part of the phase in cycles, in [-0.5, 0.5], then sin/cos(2*pi*r).
+__device__ __forceinline__ void sincos_cycles(double ph, float &sn, float &cs) {
+    const float r = (float)(ph - rint(ph));
+    sn = __builtin_amdgcn_sinf(r);
+    cs = __builtin_amdgcn_cosf(r);
+}
+
+// Power-of-two scale that maps |values| <= maxabs below 2^12 (FP16 hi/lo
+// split without overflow and without losing the low part to underflow).
+__device__ __forceinline__ float pow2_scale(float maxabs) {
+    if (!(maxabs > 0.0f) || !(maxabs < 3.0e38f)) return 1.0f;
+    int e;
+    (void)frexpf(maxabs, &e);      // maxabs < 2^e
+    return ldexpf(1.0f, 12 - e);
+}
+
+template <typename T16>
+__device__ __forceinline__ unsigned pack2(T16 a, T16 b) {
+    unsigned short ua = __builtin_bit_cast(unsigned short, a);
+    unsigned short ub = __builtin_bit_cast(unsigned short, b);
+    return (unsigned)ua | ((unsigned)ub << 16);
+}
+
+// hi = T16(c), lo = T16(c - hi): c == hi + lo to ~2^-22 relative (FP16) or
+// ~2^-16 (BF16), the compensated split of Markidis et al. (2018) and
+// Ootomo & Yokota (2022) for matrix-core products.
+template <typename T16>
+__device__ __forceinline__ void split2(float c, float s, unsigned &hi, unsigned &lo) {
+    const T16 ch = (T16)c, sh = (T16)s;
+    const float cl = c - (float)ch, sl = s - (float)sh;
+    hi = pack2<T16>(ch, sh);
+    lo = pack2<T16>((T16)cl, (T16)sl);
+}
+
+template <typename T16>
+struct mfma_t;
+template <>
+struct mfma_t<_Float16> {
+    typedef h8_t vec;
+    static __device__ __forceinline__ f16x_t run(vec a, vec b, f16x_t c) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+    }
+};
+template <>
+struct mfma_t<__bf16> {
+    typedef b8_t vec;
+    static __device__ __forceinline__ f16x_t run(vec a, vec b, f16x_t c) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// direct FP32 kernels (VALU contraction)
+// ---------------------------------------------------------------------------
+//
+// FWD:  part[split][s][c] = sum_{v in split} e(s,v) * z[v][c]
+// !FWD: part[split][v][c] = sum_{s in split} conj(e(s,v)) * y[s][c]
+// Each lane keeps CB complex accumulators; the streamed record and the CB
+// complex operands of the streamed row arrive through scalar loads.
+template <bool FWD, int L, int CB>
+__global__ __launch_bounds__(256) void k_direct(
+    const double *__restrict__ coef, int sk, const double *__restrict__ basis, int sb,
+    const float2 *__restrict__ src, float2 *__restrict__ part, int rows_pad, int ncs,
+    int n_tiles, int n_splits, int per_split, int n_stream, int n_batches) {
+    int tile, sg;
+    if (!map_block(n_tiles, n_splits * n_batches, tile, sg)) return;
+    const int batch = sg % n_batches;
+    const int split = sg / n_batches;
+    const int cb0 = batch * CB;
+    const int row = tile * 256 + threadIdx.x;
+
+    constexpr int NST = FWD ? L + 1 : L;
+    double st[NST];
+    {
+        const double *p = FWD ? coef + (size_t)row * sk : basis + (size_t)row * sb;
+#pragma unroll
+        for (int i = 0; i < NST; ++i) st[i] = p[i];
+    }
+    float ar[CB], ai[CB];
+#pragma unroll
+    for (int c = 0; c < CB; ++c) ar[c] = ai[c] = 0.0f;
+
+    const int q0 = split * per_split;
+    const int q1 = min(q0 + per_split, n_stream);
+    for (int q = q0; q < q1; ++q) {
+        const double *sm = FWD ? basis + (size_t)q * sb : coef + (size_t)q * sk;
+        double ph;
+        if (FWD) {
+            ph = st[0];
+#pragma unroll
+            for (int i = 1; i <= L; ++i) ph = fma(st[i], sm[i - 1], ph);
+        } else {
+            ph = sm[0];
+#pragma unroll
+            for (int i = 1; i <= L; ++i) ph = fma(sm[i], st[i - 1], ph);
+        }
+        float sn, cs;
+        sincos_cycles(ph, sn, cs);
+        if (!FWD) sn = -sn;   // conj(e)
+        const float2 *w = src + (size_t)q * ncs + cb0;
+#pragma unroll
+        for (int c = 0; c < CB; ++c) {
+            const float2 u = w[c];
+            ar[c] = fmaf(cs, u.x, ar[c]);
+            ar[c] = fmaf(-sn, u.y, ar[c]);
+            ai[c] = fmaf(cs, u.y, ai[c]);
+            ai[c] = fmaf(sn, u.x, ai[c]);
+        }
+    }
+    float2 *o = part + ((size_t)split * rows_pad + row) * ncs + cb0;
+#pragma unroll
+    for (int c = 0; c < CB; ++c) o[c] = make_float2(ar[c], ai[c]);
+}
+
+// ---------------------------------------------------------------------------
+// mixed-precision phase-reuse kernels (MFMA contraction)
+// ---------------------------------------------------------------------------
+//
+// A wave owns 64 stationary rows = two 32-row MFMA tiles. Per K step it
+// evaluates exp(i*phi) for 8 streamed indices in all 64 lanes (lane = row),
+// splits cos/sin into T16 hi+lo, and uses v_permlane32_swap to move the
+// values into the v_mfma_f32_32x32x16 A-operand layout (lane l holds row
+// l%32 and K entries 8*(l/32)..+7 = 4 complex (cos,sin) pairs). The B operand
+// (coil data, precomputed per application in exactly the B-register layout,
+// FP16/BF16 hi+lo, scaled by a power of two) is read with 16-byte loads.
+// Three products per K step: Ahi*Bhi + Alo*Bhi + Ahi*Blo, FP32 accumulate.
+// Output columns: tile 0 = real part, tile 1 = imaginary part of 32 coils.
+template <typename T16, bool FWD, int L, int NB>
+__global__ __launch_bounds__(256) void k_mixed(
+    const double *__restrict__ coef, int sk, const double *__restrict__ basis, int sb,
+    const uint4 *__restrict__ frag, int n_ks, float2 *__restrict__ part, int rows_pad, int ncs,
+    int n_tiles, int n_splits, int ks_per_split, int n_groups) {
+    typedef typename mfma_t<T16>::vec vec_t;
+    int tile, sg;
+    if (!map_block(n_tiles, n_splits * n_groups, tile, sg)) return;
+    const int grp = sg % n_groups;
+    const int split = sg / n_groups;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int row = tile * 256 + wave * 64 + lane;
+
+    constexpr int NST = FWD ? L + 1 : L;
+    double st[NST];
+    {
+        const double *p = FWD ? coef + (size_t)row * sk : basis + (size_t)row * sb;
+#pragma unroll
+        for (int i = 0; i < NST; ++i) st[i] = p[i];
+    }
+
+    f16x_t acc[2][NB][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            acc[a][b][0] = f16x_t{};
+            acc[a][b][1] = f16x_t{};
+        }
+
+    const int ks0 = split * ks_per_split;
+    const int ks1 = min(ks0 + ks_per_split, n_ks);
+    for (int ks = ks0; ks < ks1; ++ks) {
+        uint4 bfr[NB][4];
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const uint4 *p = frag + ((((size_t)(grp * NB + b) * n_ks + ks) * 64 + lane) << 2);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) bfr[b][q] = p[q];
+        }
+        unsigned hp[8], lp[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int q = ks * 8 + j;
+            const double *sm = FWD ? basis + (size_t)q * sb : coef + (size_t)q * sk;
+            double ph;
+            if (FWD) {
+                ph = st[0];
+#pragma unroll
+                for (int i = 1; i <= L; ++i) ph = fma(st[i], sm[i - 1], ph);
+            } else {
+                ph = sm[0];
+#pragma unroll
+                for (int i = 1; i <= L; ++i) ph = fma(sm[i], st[i - 1], ph);
+            }
+            float sn, cs;
+            sincos_cycles(ph, sn, cs);
+            split2<T16>(cs, sn, hp[j], lp[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            auto a = __builtin_amdgcn_permlane32_swap(hp[j], hp[j + 4], false, false);
+            hp[j] = a[0];
+            hp[j + 4] = a[1];
+            auto b = __builtin_amdgcn_permlane32_swap(lp[j], lp[j + 4], false, false);
+            lp[j] = b[0];
+            lp[j + 4] = b[1];
+        }
+        vec_t ahi[2], alo[2];
+        {
+            uint4 t0 = make_uint4(hp[0], hp[1], hp[2], hp[3]);
+            uint4 t1 = make_uint4(hp[4], hp[5], hp[6], hp[7]);
+            uint4 u0 = make_uint4(lp[0], lp[1], lp[2], lp[3]);
+            uint4 u1 = make_uint4(lp[4], lp[5], lp[6], lp[7]);
+            ahi[0] = __builtin_bit_cast(vec_t, t0);
+            ahi[1] = __builtin_bit_cast(vec_t, t1);
+            alo[0] = __builtin_bit_cast(vec_t, u0);
+            alo[1] = __builtin_bit_cast(vec_t, u1);
+        }
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const vec_t bhr = __builtin_bit_cast(vec_t, bfr[b][0]);
+            const vec_t bhi = __builtin_bit_cast(vec_t, bfr[b][1]);
+            const vec_t blr = __builtin_bit_cast(vec_t, bfr[b][2]);
+            const vec_t bli = __builtin_bit_cast(vec_t, bfr[b][3]);
+#pragma unroll
+            for (int rt = 0; rt < 2; ++rt) {
+                acc[rt][b][0] = mfma_t<T16>::run(ahi[rt], bhr, acc[rt][b][0]);
+                acc[rt][b][1] = mfma_t<T16>::run(ahi[rt], bhi, acc[rt][b][1]);
+                acc[rt][b][0] = mfma_t<T16>::run(alo[rt], bhr, acc[rt][b][0]);
+                acc[rt][b][1] = mfma_t<T16>::run(alo[rt], bhi, acc[rt][b][1]);
+                acc[rt][b][0] = mfma_t<T16>::run(ahi[rt], blr, acc[rt][b][0]);
+                acc[rt][b][1] = mfma_t<T16>::run(ahi[rt], bli, acc[rt][b][1]);
+            }
+        }
+    }
+    // D layout of v_mfma_f32_32x32x*: register r of lane l holds
+    // row 8*(r/4) + 4*(l/32) + r%4, column l%32.
+    const int col = lane & 31;
+    const int hi32 = lane >> 5;
+#pragma unroll
+    for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const int coil = (grp * NB + b) * 32 + col;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int rr = tile * 256 + wave * 64 + rt * 32 + 8 * (r >> 2) + 4 * hi32 + (r & 3);
+                part[((size_t)split * rows_pad + rr) * ncs + coil] =
+                    make_float2(acc[rt][b][0][r], acc[rt][b][1][r]);
+            }
+        }
+}
+
+// ---------------------------------------------------------------------------
+// gradient kernel: G[split][s][l] = sum_{v in split} b_l(v) * Im(e(s,v) * u(s,v)),
+// u(s,v) = sum_c conj(r[s][c]) * z[v][c]   (FP32 products, FP64 accumulation)
+// ---------------------------------------------------------------------------
+template <int L, int CB>
+__global__ __launch_bounds__(256) void k_grad(
+    const double *__restrict__ coef, int sk, const double *__restrict__ basis, int sb,
+    const float2 *__restrict__ z, const float2 *__restrict__ res, double *__restrict__ part,
+    int ns_pad, int ncs, int n_tiles, int n_splits, int per_split, int nv, int n_batches) {
+    int tile, sg;
+    if (!map_block(n_tiles, n_splits * n_batches, tile, sg)) return;
+    const int batch = sg % n_batches;
+    const int split = sg / n_batches;
+    const int cb0 = batch * CB;
+    const int s = tile * 256 + threadIdx.x;
+    double st[L + 1];
+#pragma unroll
+    for (int i = 0; i <= L; ++i) st[i] = coef[(size_t)s * sk + i];
+    float rr[CB], ri[CB];
+#pragma unroll
+    for (int c = 0; c < CB; ++c) {
+        const float2 t = res[(size_t)s * ncs + cb0 + c];
+        rr[c] = t.x;
+        ri[c] = t.y;
+    }
+    double g[L];
+#pragma unroll
+    for (int l = 0; l < L; ++l) g[l] = 0.0;
+    const int v0 = split * per_split;
+    const int v1 = min(v0 + per_split, nv);
+    for (int v = v0; v < v1; ++v) {
+        const double *sm = basis + (size_t)v * sb;
+        double ph = st[0];
+#pragma unroll
+        for (int i = 1; i <= L; ++i) ph = fma(st[i], sm[i - 1], ph);
+        float sn, cs;
+        sincos_cycles(ph, sn, cs);
+        const float2 *w = z + (size_t)v * ncs + cb0;
+        float ur = 0.0f, ui = 0.0f;
+#pragma unroll
+        for (int c = 0; c < CB; ++c) {
+            const float2 u = w[c];
+            ur = fmaf(rr[c], u.x, ur);
+            ur = fmaf(ri[c], u.y, ur);
+            ui = fmaf(rr[c], u.y, ui);
+            ui = fmaf(-ri[c], u.x, ui);
+        }
+        const double m = (double)fmaf(cs, ui, sn * ur);
+        g[0] += m;
+#pragma unroll
+        for (int l = 1; l < L; ++l) g[l] = fma(m, sm[l - 1], g[l]);
+    }
+    double *o = part + ((size_t)sg * ns_pad + s) * L;
+#pragma unroll
+    for (int l = 0; l < L; ++l) o[l] = g[l];
+}
+
+// ---------------------------------------------------------------------------
+// data movement and reductions
+// ---------------------------------------------------------------------------
+
+__global__ void k_zero_u32(unsigned *p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+__device__ __forceinline__ void wave_absmax_to(unsigned *dst, float m) {
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0 && m > 0.0f) atomicMax(dst, __float_as_uint(m));
+}
+
+// xm[v] = x_full[mask_idx[v]] (zero for padded voxels); optional |.|max.
+__global__ void k_gather(const float2 *__restrict__ x, const int *__restrict__ idx, float2 *__restrict__ xm,
+                         int nv, int nv_pad, unsigned *maxslot) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    float m = 0.0f;
+    if (v < nv_pad) {
+        float2 val = make_float2(0.0f, 0.0f);
+        if (v < nv) val = x[idx[v]];
+        xm[v] = val;
+        m = fmaxf(fabsf(val.x), fabsf(val.y));
+    }
+    if (maxslot) wave_absmax_to(maxslot, m);
+}
+
+// |.|max of a masked vector (float2) into a slot.
+__global__ void k_absmax2(const float2 *__restrict__ a, int n, unsigned *maxslot) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    float m = 0.0f;
+    if (v < n) {
+        const float2 t = a[v];
+        m = fmaxf(fabsf(t.x), fabsf(t.y));
+    }
+    wave_absmax_to(maxslot, m);
+}
+
+__global__ void k_scatter(const float2 *__restrict__ xm, const int *__restrict__ idx, float2 *__restrict__ x,
+                          int nv, float scale) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < nv) {
+        const float2 t = xm[v];
+        x[idx[v]] = make_float2(t.x * scale, t.y * scale);
+    }
+}
+
+__global__ void k_fill0(float2 *p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = make_float2(0.0f, 0.0f);
+}
+
+// z[v][c] = csm[v][c] * xm[v] (coil stride ncs; zero for padded coils/voxels).
+__global__ void k_build_z(const float2 *__restrict__ csm, const float2 *__restrict__ xm, float2 *__restrict__ z,
+                          int nv_pad, int ncs) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)nv_pad * ncs) return;
+    const int v = (int)(i / ncs);
+    const float2 a = csm[i], b = xm[v];
+    z[i] = make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// MFMA B fragments of z = csm .* xm for the forward (K = voxels):
+// frag[gb][ks][lane] = {hi_re, hi_im, lo_re, lo_im}, each 8 x T16:
+//   lane l: coil 32*gb + l%32, voxels 8*ks + 4*(l/32) + i/2 (i = 0..7);
+//   re tile: (Zr, -Zi) pairs, im tile: (Zi, Zr) pairs.
+template <typename T16>
+__global__ void k_build_zfrag(const float2 *__restrict__ csm, const float2 *__restrict__ xm, uint4 *__restrict__ frag,
+                              int n_ks, int ncs, int n_gb, const unsigned *__restrict__ maxslot, float csm_max) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)n_gb * n_ks * 64) return;
+    const int lane = (int)(t & 63);
+    const int64_t rest = t >> 6;
+    const int ks = (int)(rest % n_ks);
+    const int gb = (int)(rest / n_ks);
+    const float scale = pow2_scale(__uint_as_float(*maxslot) * csm_max);
+    const int c = gb * 32 + (lane & 31);
+    unsigned hr[4], hi[4], lr[4], li[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int v = ks * 8 + 4 * (lane >> 5) + j;
+        const float2 a = csm[(size_t)v * ncs + c], b = xm[v];
+        const float zr = (a.x * b.x - a.y * b.y) * scale;
+        const float zi = (a.x * b.y + a.y * b.x) * scale;
+        unsigned h0, l0, h1, l1;
+        split2<T16>(zr, -zi, h0, l0);   // re tile
+        split2<T16>(zi, zr, h1, l1);    // im tile
+        hr[j] = h0; lr[j] = l0; hi[j] = h1; li[j] = l1;
+    }
+    uint4 *o = frag + (t << 2);
+    o[0] = make_uint4(hr[0], hr[1], hr[2], hr[3]);
+    o[1] = make_uint4(hi[0], hi[1], hi[2], hi[3]);
+    o[2] = make_uint4(lr[0], lr[1], lr[2], lr[3]);
+    o[3] = make_uint4(li[0], li[1], li[2], li[3]);
+}
+
+// MFMA B fragments of the weighted signal y[s][c] for the adjoint (K = samples):
+//   re tile: (Yr, Yi) pairs, im tile: (Yi, -Yr) pairs.
+template <typename T16>
+__global__ void k_build_yfrag(const float2 *__restrict__ y, uint4 *__restrict__ frag, int n_ks, int ncs, int n_gb,
+                              const unsigned *__restrict__ maxslot) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)n_gb * n_ks * 64) return;
+    const int lane = (int)(t & 63);
+    const int64_t rest = t >> 6;
+    const int ks = (int)(rest % n_ks);
+    const int gb = (int)(rest / n_ks);
+    const float scale = pow2_scale(__uint_as_float(*maxslot));
+    const int c = gb * 32 + (lane & 31);
+    unsigned hr[4], hi[4], lr[4], li[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int s = ks * 8 + 4 * (lane >> 5) + j;
+        const float2 a = y[(size_t)s * ncs + c];
+        const float yr = a.x * scale, yi = a.y * scale;
+        unsigned h0, l0, h1, l1;
+        split2<T16>(yr, yi, h0, l0);
+        split2<T16>(yi, -yr, h1, l1);
+        hr[j] = h0; lr[j] = l0; hi[j] = h1; li[j] = l1;
+    }
+    uint4 *o = frag + (t << 2);
+    o[0] = make_uint4(hr[0], hr[1], hr[2], hr[3]);
+    o[1] = make_uint4(hi[0], hi[1], hi[2], hi[3]);
+    o[2] = make_uint4(lr[0], lr[1], lr[2], lr[3]);
+    o[3] = make_uint4(li[0], li[1], li[2], li[3]);
+}
+
+// Forward reduction over splits. Tile of 32 samples x 32 coils per block.
+// mode 0: user layout y_out[c*ns + s] = scale * sum
+// mode 1: ybuf[s][c] = w[s][c] * (scale * sum - y_data[c*ns + s]) (w, y_data optional),
+//         zero for padded rows/coils, optional |.|max into maxslot.
+// The effective scale is scale / pow2_scale(max from scale_slot) when scale_slot
+// is given (undoing the fragment scaling of the mixed path).
+__global__ __launch_bounds__(256) void k_reduce_fwd(
+    const float2 *__restrict__ part, int n_splits, int ns_pad, int ncs, int ns, int nc, float scale,
+    const unsigned *__restrict__ scale_slot, float scale_mul, int mode, float2 *__restrict__ out,
+    const float *__restrict__ w, const float2 *__restrict__ y_data, unsigned *maxslot) {
+    __shared__ float2 tileb[32][33];
+    const int s0 = blockIdx.x * 32, c0 = blockIdx.y * 32;
+    const int tc = threadIdx.x & 31, tr = threadIdx.x >> 5;   // 8 row groups
+    float sc = scale;
+    if (scale_slot) sc = scale / pow2_scale(__uint_as_float(*scale_slot) * scale_mul);
+    if (mode == 1 && y_data) {
+        // stage the data tile transposed: y_data[c*ns + s]
+        for (int k = tr; k < 32; k += 8) {
+            const int c = c0 + k, s = s0 + tc;
+            tileb[k][tc] = (c < nc && s < ns) ? y_data[(size_t)c * ns + s] : make_float2(0.0f, 0.0f);
+        }
+        __syncthreads();
+    }
+    float m = 0.0f;
+    for (int k = tr; k < 32; k += 8) {
+        const int s = s0 + k, c = c0 + tc;
+        float2 acc = make_float2(0.0f, 0.0f);
+        for (int p = 0; p < n_splits; ++p) {
+            const float2 t = part[((size_t)p * ns_pad + s) * ncs + c];
+            acc.x += t.x;
+            acc.y += t.y;
+        }
+        acc.x *= sc;
+        acc.y *= sc;
+        if (mode == 0) {
+            tileb[k][tc] = acc;
+        } else {
+            const bool valid = (s < ns) && (c < nc);
+            if (y_data) {
+                const float2 d = tileb[tc][k];
+                acc.x -= d.x;
+                acc.y -= d.y;
+            }
+            if (w) {
+                const float ww = w[(size_t)s * ncs + c];
+                acc.x *= ww;
+                acc.y *= ww;
+            }
+            if (!valid) acc = make_float2(0.0f, 0.0f);
+            out[(size_t)s * ncs + c] = acc;
+            m = fmaxf(m, fmaxf(fabsf(acc.x), fabsf(acc.y)));
+        }
+    }
+    if (mode == 0) {
+        __syncthreads();
+        for (int k = tr; k < 32; k += 8) {
+            const int c = c0 + k, s = s0 + tc;
+            if (c < nc && s < ns) out[(size_t)c * ns + s] = tileb[tc][k];
+        }
+    } else if (maxslot) {
+        wave_absmax_to(maxslot, m);
+    }
+}
+
+// ybuf[s][c] = w[s][c] * y_user[c*ns + s] (zero padded), optional |.|max.
+__global__ __launch_bounds__(256) void k_load_ybuf(const float2 *__restrict__ y, const float *__restrict__ w,
+                                                   float2 *__restrict__ out, int ns, int nc, int ns_pad, int ncs,
+                                                   unsigned *maxslot) {
+    __shared__ float2 tileb[32][33];
+    const int s0 = blockIdx.x * 32, c0 = blockIdx.y * 32;
+    const int tc = threadIdx.x & 31, tr = threadIdx.x >> 5;
+    for (int k = tr; k < 32; k += 8) {
+        const int c = c0 + k, s = s0 + tc;
+        tileb[k][tc] = (c < nc && s < ns) ? y[(size_t)c * ns + s] : make_float2(0.0f, 0.0f);
+    }
+    __syncthreads();
+    float m = 0.0f;
+    for (int k = tr; k < 32; k += 8) {
+        const int s = s0 + k, c = c0 + tc;
+        float2 v = tileb[tc][k];
+        if (w) {
+            const float ww = w[(size_t)s * ncs + c];
+            v.x *= ww;
+            v.y *= ww;
+        }
+        out[(size_t)s * ncs + c] = v;
+        m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
+    }
+    if (maxslot) wave_absmax_to(maxslot, m);
+}
+
+// Adjoint reduction: xm_out[v] = sc * sum_c conj(csm[v][c]) * sum_split part[split][v][c]
+//                               (+ lambda * x_in[v]).
+// Half-wave per voxel: lanes over coils, shuffle reduction over 32 lanes.
+__global__ __launch_bounds__(256) void k_reduce_adj(
+    const float2 *__restrict__ part, int n_splits, int nv_pad, int ncs, const float2 *__restrict__ csm, int nv,
+    float scale, const unsigned *__restrict__ scale_slot, float2 *__restrict__ xm_out, float lambda,
+    const float2 *__restrict__ x_in) {
+    const int hw = (blockIdx.x * blockDim.x + threadIdx.x) >> 5;   // half-wave id = voxel
+    const int cl = threadIdx.x & 31;
+    float sc = scale;
+    if (scale_slot) sc = scale / pow2_scale(__uint_as_float(*scale_slot));
+    const int v = hw;
+    float ar = 0.0f, ai = 0.0f;
+    if (v < nv_pad) {
+        for (int c = cl; c < ncs; c += 32) {
+            float2 acc = make_float2(0.0f, 0.0f);
+            for (int p = 0; p < n_splits; ++p) {
+                const float2 t = part[((size_t)p * nv_pad + v) * ncs + c];
+                acc.x += t.x;
+                acc.y += t.y;
+            }
+            const float2 s = csm[(size_t)v * ncs + c];
+            ar += s.x * acc.x + s.y * acc.y;   // conj(s) * acc
+            ai += s.x * acc.y - s.y * acc.x;
+        }
+    }
+    for (int o = 16; o > 0; o >>= 1) {
+        ar += __shfl_xor(ar, o, 32);
+        ai += __shfl_xor(ai, o, 32);
+    }
+    if (cl == 0 && v < nv_pad) {
+        float2 r = make_float2(ar * sc, ai * sc);
+        if (v >= nv) r = make_float2(0.0f, 0.0f);
+        if (x_in && v < nv) {
+            const float2 t = x_in[v];
+            r.x += lambda * t.x;
+            r.y += lambda * t.y;
+        }
+        xm_out[v] = r;
+    }
+}
+
+// gk[l][s] = scale * sum_p part[p][s][l]  (FP64)
+__global__ void k_reduce_grad(const double *__restrict__ part, int n_parts, int ns_pad, int L, int ns,
+                              double scale, double *__restrict__ gk) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)ns * L) return;
+    const int s = (int)(i / L), l = (int)(i % L);
+    double acc = 0.0;
+    for (int p = 0; p < n_parts; ++p) acc += part[((size_t)p * ns_pad + s) * L + l];
+    gk[(size_t)l * ns + s] = acc * scale;
+}
+
+// ---- FP64 reductions for CG and losses ----
+
+// block partial sums of Re<a,b> (b == nullptr: |a|^2)
+__global__ __launch_bounds__(256) void k_dot_partial(const float2 *__restrict__ a, const float2 *__restrict__ b,
+                                                     int64_t n, double *__restrict__ red) {
+    __shared__ double sh[4];
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const float2 x = a[i];
+        const float2 y = b ? b[i] : x;
+        acc += (double)x.x * (double)y.x + (double)x.y * (double)y.y;
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) red[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+// CG scalar logic, one block. op: 0 = store sum into out[0];
+// 1 = pAp: cg[2] = sum, cg[3] = alpha = rr/pAp (0 if done or pAp <= 0);
+// 2 = rr_new: cg[6] = sum; beta = rr_new/rr; log[it] = sqrt(rr_new/rr0); done test; rr = rr_new;
+// 3 = init: rr = rr0 = sum; log[0] = 1.
+__global__ __launch_bounds__(256) void k_cg_scalar(const double *__restrict__ red, int nred, double *cg, int op,
+                                                   double *log, int it, double rel_tol) {
+    __shared__ double sh[256];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < nred; i += 256) acc += red[i];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    const double sum = sh[0];
+    if (op == 0) {
+        cg[0] = sum;
+    } else if (op == 3) {
+        cg[0] = sum;   // rr
+        cg[1] = sum;   // rr0
+        cg[5] = (sum > 0.0) ? 0.0 : 1.0;
+        if (log) log[0] = (sum > 0.0) ? 1.0 : 0.0;
+    } else if (op == 1) {
+        cg[2] = sum;
+        cg[3] = (cg[5] != 0.0 || !(sum > 0.0)) ? 0.0 : cg[0] / sum;
+        if (!(sum > 0.0)) cg[5] = 1.0;
+    } else if (op == 2) {
+        const bool done = cg[5] != 0.0;
+        if (!done) {
+            cg[6] = sum;
+            cg[4] = (cg[0] > 0.0) ? sum / cg[0] : 0.0;
+            const double rel = (cg[1] > 0.0) ? sqrt(sum / cg[1]) : 0.0;
+            if (log) log[it] = rel;
+            if (rel <= rel_tol) cg[5] = 1.0;
+            cg[0] = sum;
+        } else {
+            cg[4] = 0.0;
+            if (log) log[it] = 0.0;
+        }
+    }
+}
+
+// x += alpha p; r -= alpha q; partial |r|^2 (skipped when done)
+__global__ __launch_bounds__(256) void k_cg_update(float2 *__restrict__ x, float2 *__restrict__ r,
+                                                   const float2 *__restrict__ p, const float2 *__restrict__ q,
+                                                   int64_t n, const double *__restrict__ cg, double *__restrict__ red) {
+    __shared__ double sh[4];
+    const bool done = cg[5] != 0.0;
+    const float alpha = (float)cg[3];
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        float2 rv = r[i];
+        if (!done) {
+            const float2 pv = p[i], qv = q[i];
+            float2 xv = x[i];
+            xv.x = fmaf(alpha, pv.x, xv.x);
+            xv.y = fmaf(alpha, pv.y, xv.y);
+            rv.x = fmaf(-alpha, qv.x, rv.x);
+            rv.y = fmaf(-alpha, qv.y, rv.y);
+            x[i] = xv;
+            r[i] = rv;
+        }
+        acc += (double)rv.x * rv.x + (double)rv.y * rv.y;
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) red[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+// p = r + beta p (skipped when done)
+__global__ void k_cg_pupdate(float2 *__restrict__ p, const float2 *__restrict__ r, int64_t n,
+                             const double *__restrict__ cg) {
+    if (cg[5] != 0.0) return;
+    const float beta = (float)cg[4];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const float2 rv = r[i], pv = p[i];
+        p[i] = make_float2(fmaf(beta, pv.x, rv.x), fmaf(beta, pv.y, rv.y));
+    }
+}
+
+__global__ void k_copy2(float2 *__restrict__ d, const float2 *__restrict__ s, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) d[i] = s[i];
+}
+
+// ---------------------------------------------------------------------------
+// host-side launchers
+// ---------------------------------------------------------------------------
+
+static inline unsigned *slot(Handle *h, int i) { return reinterpret_cast<unsigned *>(h->scal.p) + 64 + i; }
+static inline double *cgs(Handle *h) { return reinterpret_cast<double *>(h->scal.p); }
+
+static inline int blocks_for(int64_t n, int b = 256) { return (int)((n + b - 1) / b); }
+
+hipError_t launch_gather(Handle *h, const void *x_full, void *xm) {
+    k_gather<<<blocks_for(h->nv_pad), 256, 0, h->stream>>>((const float2 *)x_full, (const int *)h->mask_idx.p,
+                                                           (float2 *)xm, (int)h->n_vox, (int)h->nv_pad, nullptr);
+    return hipGetLastError();
+}
+
+hipError_t launch_scatter(Handle *h, const void *xm, void *x_full, float scale) {
+    k_fill0<<<blocks_for(h->n_grid), 256, 0, h->stream>>>((float2 *)x_full, h->n_grid);
+    k_scatter<<<blocks_for(h->n_vox), 256, 0, h->stream>>>((const float2 *)xm, (const int *)h->mask_idx.p,
+                                                           (float2 *)x_full, (int)h->n_vox, scale);
+    return hipGetLastError();
+}
+
+static void zero_slots(Handle *h) { k_zero_u32<<<1, 64, 0, h->stream>>>(slot(h, 0), 64); }
+
+// ---- kernel dispatch tables over (L, coil batch) ----
+
+template <bool FWD, int L>
+static void launch_direct_L(Handle *h, int cb, dim3 grid, const float2 *src, float2 *part, int rows_pad,
+                            int n_tiles, int n_splits, int per_split, int n_stream, int n_batches) {
+    const double *coef = (const double *)h->coef.p, *basis = (const double *)h->basis.p;
+    if (cb == 8)
+        k_direct<FWD, L, 8><<<grid, 256, 0, h->stream>>>(coef, h->sk, basis, h->sb, src, part, rows_pad, h->ncs,
+                                                         n_tiles, n_splits, per_split, n_stream, n_batches);
+    else if (cb == 16)
+        k_direct<FWD, L, 16><<<grid, 256, 0, h->stream>>>(coef, h->sk, basis, h->sb, src, part, rows_pad, h->ncs,
+                                                          n_tiles, n_splits, per_split, n_stream, n_batches);
+    else
+        k_direct<FWD, L, 32><<<grid, 256, 0, h->stream>>>(coef, h->sk, basis, h->sb, src, part, rows_pad, h->ncs,
+                                                          n_tiles, n_splits, per_split, n_stream, n_batches);
+}
+
+template <bool FWD>
+static void launch_direct(Handle *h, const float2 *src, float2 *part, int rows_pad, int n_tiles, int n_splits,
+                          int per_split, int n_stream) {
+    const int cb = h->n_cha <= 8 ? 8 : (h->n_cha <= 16 ? 16 : 32);
+    const int n_batches = (int)((h->n_cha + cb - 1) / cb);
+    dim3 grid(grid_blocks(n_tiles, n_splits * n_batches));
+    switch (h->L) {
+        case 9: launch_direct_L<FWD, 9>(h, cb, grid, src, part, rows_pad, n_tiles, n_splits, per_split, n_stream, n_batches); break;
+        case 13: launch_direct_L<FWD, 13>(h, cb, grid, src, part, rows_pad, n_tiles, n_splits, per_split, n_stream, n_batches); break;
+        case 16: launch_direct_L<FWD, 16>(h, cb, grid, src, part, rows_pad, n_tiles, n_splits, per_split, n_stream, n_batches); break;
+        default: launch_direct_L<FWD, 20>(h, cb, grid, src, part, rows_pad, n_tiles, n_splits, per_split, n_stream, n_batches); break;
+    }
+}
+
+template <typename T16, bool FWD, int L>
+static void launch_mixed_L(Handle *h, int nb, dim3 grid, const uint4 *frag, int n_ks, float2 *part, int rows_pad,
+                           int n_tiles, int n_splits, int ks_per_split, int n_groups) {
+    const double *coef = (const double *)h->coef.p, *basis = (const double *)h->basis.p;
+    if (nb == 1)
+        k_mixed<T16, FWD, L, 1><<<grid, 256, 0, h->stream>>>(coef, h->sk, basis, h->sb, frag, n_ks, part, rows_pad,
+                                                             h->ncs, n_tiles, n_splits, ks_per_split, n_groups);
+    else
+        k_mixed<T16, FWD, L, 2><<<grid, 256, 0, h->stream>>>(coef, h->sk, basis, h->sb, frag, n_ks, part, rows_pad,
+                                                             h->ncs, n_tiles, n_splits, ks_per_split, n_groups);
+}
+
+template <typename T16, bool FWD>
+static void launch_mixed(Handle *h, const uint4 *frag, int n_ks, float2 *part, int rows_pad, int n_tiles,
+                         int n_splits, int ks_per_split) {
+    const int n_gb = h->ncs / 32;
+    const int nb = n_gb >= 2 ? 2 : 1;
+    const int n_groups = (n_gb + nb - 1) / nb;
+    dim3 grid(grid_blocks(n_tiles, n_splits * n_groups));
+    switch (h->L) {
+        case 9: launch_mixed_L<T16, FWD, 9>(h, nb, grid, frag, n_ks, part, rows_pad, n_tiles, n_splits, ks_per_split, n_groups); break;
+        case 13: launch_mixed_L<T16, FWD, 13>(h, nb, grid, frag, n_ks, part, rows_pad, n_tiles, n_splits, ks_per_split, n_groups); break;
+        case 16: launch_mixed_L<T16, FWD, 16>(h, nb, grid, frag, n_ks, part, rows_pad, n_tiles, n_splits, ks_per_split, n_groups); break;
+        default: launch_mixed_L<T16, FWD, 20>(h, nb, grid, frag, n_ks, part, rows_pad, n_tiles, n_splits, ks_per_split, n_groups); break;
+    }
+}
+
+// Number of coil-group fragments (32 coils each) in a padded layout where the
+// group count is rounded up to the per-wave batch count.
+static inline int frag_groups(Handle *h) {
+    const int n_gb = h->ncs / 32;
+    const int nb = n_gb >= 2 ? 2 : 1;
+    return ((n_gb + nb - 1) / nb) * nb;
+}
+
+// Builds the forward operand from masked xm (device) and runs the forward
+// contraction into part_f.
+static hipError_t forward_from_xm(Handle *h, const float2 *xm, bool time_it) {
+    const int n_tiles = (int)(h->ns_pad / 256);
+    if (h->backend == HOE_BACKEND_DIRECT) {
+        const int64_t nz = h->nv_pad * h->ncs;
+        k_build_z<<<blocks_for(nz), 256, 0, h->stream>>>((const float2 *)h->csm.p, xm, (float2 *)h->zbuf.p,
+                                                          (int)h->nv_pad, h->ncs);
+        if (time_it) (void)hipEventRecord(h->ev[0], h->stream);
+        launch_direct<true>(h, (const float2 *)h->zbuf.p, (float2 *)h->part_f.p, (int)h->ns_pad, n_tiles,
+                            h->splits_f, h->vox_per_split, (int)h->n_vox);
+        if (time_it) (void)hipEventRecord(h->ev[1], h->stream);
+    } else {
+        const int n_ks = (int)(h->nv_pad / 8);
+        const int n_gb = frag_groups(h);
+        zero_slots(h);
+        k_absmax2<<<blocks_for(h->nv_pad), 256, 0, h->stream>>>(xm, (int)h->nv_pad, slot(h, 0));
+        const int64_t nt = (int64_t)n_gb * n_ks * 64;
+        if (h->backend == HOE_BACKEND_MIXED_FP16)
+            k_build_zfrag<_Float16><<<blocks_for(nt), 256, 0, h->stream>>>(
+                (const float2 *)h->csm.p, xm, (uint4 *)h->zfrag.p, n_ks, h->ncs, n_gb, slot(h, 0), h->csm_max);
+        else
+            k_build_zfrag<__bf16><<<blocks_for(nt), 256, 0, h->stream>>>(
+                (const float2 *)h->csm.p, xm, (uint4 *)h->zfrag.p, n_ks, h->ncs, n_gb, slot(h, 0), h->csm_max);
+        if (time_it) (void)hipEventRecord(h->ev[0], h->stream);
+        const int ks_per_split = h->vox_per_split / 8;
+        if (h->backend == HOE_BACKEND_MIXED_FP16)
+            launch_mixed<_Float16, true>(h, (const uint4 *)h->zfrag.p, n_ks, (float2 *)h->part_f.p, (int)h->ns_pad,
+                                         n_tiles, h->splits_f, ks_per_split);
+        else
+            launch_mixed<__bf16, true>(h, (const uint4 *)h->zfrag.p, n_ks, (float2 *)h->part_f.p, (int)h->ns_pad,
+                                       n_tiles, h->splits_f, ks_per_split);
+        if (time_it) (void)hipEventRecord(h->ev[1], h->stream);
+    }
+    return hipGetLastError();
+}
+
+hipError_t run_forward_contract(Handle *h, const void *x_full, bool time_it) {
+    hipError_t e = launch_gather(h, x_full, h->xm.p);
+    if (e != hipSuccess) return e;
+    return forward_from_xm(h, (const float2 *)h->xm.p, time_it);
+}
+
+static inline bool mixed(Handle *h) { return h->backend != HOE_BACKEND_DIRECT; }
+
+hipError_t launch_reduce_fwd_user(Handle *h, void *y_out, float scale) {
+    dim3 grid((unsigned)(h->ns_pad / 32), (unsigned)(h->ncs / 32));
+    k_reduce_fwd<<<grid, 256, 0, h->stream>>>((const float2 *)h->part_f.p, h->splits_f, (int)h->ns_pad, h->ncs,
+                                              (int)h->n_sam, (int)h->n_cha, scale, mixed(h) ? slot(h, 0) : nullptr,
+                                              h->csm_max, 0, (float2 *)y_out, nullptr, nullptr, nullptr);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_fwd_ybuf(Handle *h, float scale, bool weights, const void *y_data) {
+    dim3 grid((unsigned)(h->ns_pad / 32), (unsigned)(h->ncs / 32));
+    k_reduce_fwd<<<grid, 256, 0, h->stream>>>((const float2 *)h->part_f.p, h->splits_f, (int)h->ns_pad, h->ncs,
+                                              (int)h->n_sam, (int)h->n_cha, scale, mixed(h) ? slot(h, 0) : nullptr,
+                                              h->csm_max, 1, (float2 *)h->ybuf.p,
+                                              weights ? (const float *)h->dvec.p : nullptr,
+                                              (const float2 *)y_data, slot(h, 1));
+    return hipGetLastError();
+}
+
+hipError_t launch_load_ybuf(Handle *h, const void *y_user, bool weights) {
+    zero_slots(h);
+    dim3 grid((unsigned)(h->ns_pad / 32), (unsigned)(h->ncs / 32));
+    k_load_ybuf<<<grid, 256, 0, h->stream>>>((const float2 *)y_user, weights ? (const float *)h->dvec.p : nullptr,
+                                             (float2 *)h->ybuf.p, (int)h->n_sam, (int)h->n_cha, (int)h->ns_pad,
+                                             h->ncs, slot(h, 1));
+    return hipGetLastError();
+}
+
+// Adjoint contraction from ybuf (slot 1 must hold |ybuf|max for the mixed path).
+static hipError_t adjoint_contract_only(Handle *h, bool time_it) {
+    const int n_tiles = (int)(h->nv_pad / 256);
+    if (h->backend == HOE_BACKEND_DIRECT) {
+        if (time_it) (void)hipEventRecord(h->ev[2], h->stream);
+        launch_direct<false>(h, (const float2 *)h->ybuf.p, (float2 *)h->part_a.p, (int)h->nv_pad, n_tiles,
+                             h->splits_a, h->sam_per_split, (int)h->n_sam);
+        if (time_it) (void)hipEventRecord(h->ev[3], h->stream);
+    } else {
+        const int n_ks = (int)(h->ns_pad / 8);
+        const int n_gb = frag_groups(h);
+        const int64_t nt = (int64_t)n_gb * n_ks * 64;
+        if (h->backend == HOE_BACKEND_MIXED_FP16)
+            k_build_yfrag<_Float16><<<blocks_for(nt), 256, 0, h->stream>>>((const float2 *)h->ybuf.p,
+                                                                            (uint4 *)h->yfrag.p, n_ks, h->ncs, n_gb,
+                                                                            slot(h, 1));
+        else
+            k_build_yfrag<__bf16><<<blocks_for(nt), 256, 0, h->stream>>>((const float2 *)h->ybuf.p,
+                                                                          (uint4 *)h->yfrag.p, n_ks, h->ncs, n_gb,
+                                                                          slot(h, 1));
+        if (time_it) (void)hipEventRecord(h->ev[2], h->stream);
+        const int ks_per_split = h->sam_per_split / 8;
+        if (h->backend == HOE_BACKEND_MIXED_FP16)
+            launch_mixed<_Float16, false>(h, (const uint4 *)h->yfrag.p, n_ks, (float2 *)h->part_a.p,
+                                          (int)h->nv_pad, n_tiles, h->splits_a, ks_per_split);
+        else
+            launch_mixed<__bf16, false>(h, (const uint4 *)h->yfrag.p, n_ks, (float2 *)h->part_a.p, (int)h->nv_pad,
+                                        n_tiles, h->splits_a, ks_per_split);
+        if (time_it) (void)hipEventRecord(h->ev[3], h->stream);
+    }
+    return hipGetLastError();
+}
+
+static hipError_t adjoint_reduce(Handle *h, float scale, void *xm_out, float lambda, const float2 *x_in) {
+    const int64_t threads = h->nv_pad * 32;
+    k_reduce_adj<<<blocks_for(threads), 256, 0, h->stream>>>((const float2 *)h->part_a.p, h->splits_a,
+                                                             (int)h->nv_pad, h->ncs, (const float2 *)h->csm.p,
+                                                             (int)h->n_vox, scale, mixed(h) ? slot(h, 1) : nullptr,
+                                                             (float2 *)xm_out, lambda, x_in);
+    return hipGetLastError();
+}
+
+hipError_t run_adjoint_contract(Handle *h, float scale, void *xm_out, bool time_it) {
+    hipError_t e = adjoint_contract_only(h, time_it);
+    if (e != hipSuccess) return e;
+    return adjoint_reduce(h, scale, xm_out, 0.0f, nullptr);
+}
+
+hipError_t run_normal_masked(Handle *h, const void *xm_in, void *xm_out, float scale, double lambda,
+                             bool time_it) {
+    // slot 0 / slot 1 hold the fragment scales of the mixed path
+    zero_slots(h);
+    hipError_t e = forward_from_xm(h, (const float2 *)xm_in, time_it);
+    if (e != hipSuccess) return e;
+    e = launch_reduce_fwd_ybuf(h, 1.0f, true, nullptr);
+    if (e != hipSuccess) return e;
+    e = adjoint_contract_only(h, time_it);
+    if (e != hipSuccess) return e;
+    return adjoint_reduce(h, scale, xm_out, (float)lambda, lambda != 0.0 ? (const float2 *)xm_in : nullptr);
+}
+
+hipError_t run_grad_contract(Handle *h, bool time_it) {
+    // zbuf must hold csm .* xm, ybuf the residual (coil fastest)
+    const int cb = h->n_cha <= 8 ? 8 : (h->n_cha <= 16 ? 16 : 32);
+    const int n_batches = (int)((h->n_cha + cb - 1) / cb);
+    const int n_tiles = (int)(h->ns_pad / 256);
+    const int n_parts = h->splits_g * n_batches;
+    dim3 grid(grid_blocks(n_tiles, n_parts));
+    const double *coef = (const double *)h->coef.p, *basis = (const double *)h->basis.p;
+    const float2 *z = (const float2 *)h->zbuf.p, *res = (const float2 *)h->ybuf.p;
+    double *part = (double *)h->part_g.p;
+    if (time_it) (void)hipEventRecord(h->ev[0], h->stream);
+#define HOE_GRAD(LL, CC)                                                                                           \
+    k_grad<LL, CC><<<grid, 256, 0, h->stream>>>(coef, h->sk, basis, h->sb, z, res, part, (int)h->ns_pad, h->ncs,  \
+                                                n_tiles, h->splits_g, h->vox_per_split_g, (int)h->n_vox, n_batches)
+#define HOE_GRAD_L(LL)                  \
+    if (cb == 8) HOE_GRAD(LL, 8);       \
+    else if (cb == 16) HOE_GRAD(LL, 16); \
+    else HOE_GRAD(LL, 32);
+    switch (h->L) {
+        case 9: HOE_GRAD_L(9) break;
+        case 13: HOE_GRAD_L(13) break;
+        case 16: HOE_GRAD_L(16) break;
+        default: HOE_GRAD_L(20) break;
+    }
+#undef HOE_GRAD_L
+#undef HOE_GRAD
+    if (time_it) (void)hipEventRecord(h->ev[1], h->stream);
+    const double scale = -4.0 * M_PI / std::sqrt((double)h->n_vox);
+    k_reduce_grad<<<blocks_for(h->n_sam * h->L), 256, 0, h->stream>>>(part, n_parts, (int)h->ns_pad, h->L,
+                                                                      (int)h->n_sam, scale, (double *)h->gk.p);
+    return hipGetLastError();
+}
+
+hipError_t launch_build_zbuf(Handle *h, const void *xm) {
+    const int64_t nz = h->nv_pad * h->ncs;
+    k_build_z<<<blocks_for(nz), 256, 0, h->stream>>>((const float2 *)h->csm.p, (const float2 *)xm,
+                                                      (float2 *)h->zbuf.p, (int)h->nv_pad, h->ncs);
+    return hipGetLastError();
+}
+
+hipError_t launch_norm2(Handle *h, const void *v, int64_t n, double *out_dev) {
+    const int nb = 512;
+    k_dot_partial<<<nb, 256, 0, h->stream>>>((const float2 *)v, nullptr, n, (double *)h->red.p);
+    k_cg_scalar<<<1, 256, 0, h->stream>>>((const double *)h->red.p, nb, out_dev, 0, nullptr, 0, 0.0);
+    return hipGetLastError();
+}
+
+hipError_t cg_run(Handle *h, const void *rhs_m, int iters, double lambda, double rel_tol, void *x_m,
+                  double *log_dev) {
+    const int64_t n = h->nv_pad;
+    float2 *x = (float2 *)x_m;
+    float2 *r = (float2 *)h->cg_vec.p;
+    float2 *p = r + n;
+    float2 *q = p + n;
+    double *cg = cgs(h);
+    double *red = (double *)h->red.p;
+    const int nb = 512;
+    const float nscale = (float)(1.0 / (double)h->n_vox);
+    k_fill0<<<blocks_for(n), 256, 0, h->stream>>>(x, n);
+    k_copy2<<<blocks_for(n), 256, 0, h->stream>>>(r, (const float2 *)rhs_m, n);
+    k_copy2<<<blocks_for(n), 256, 0, h->stream>>>(p, (const float2 *)rhs_m, n);
+    k_dot_partial<<<nb, 256, 0, h->stream>>>(r, nullptr, n, red);
+    k_cg_scalar<<<1, 256, 0, h->stream>>>(red, nb, cg, 3, log_dev, 0, rel_tol);
+    for (int it = 1; it <= iters; ++it) {
+        hipError_t e = run_normal_masked(h, p, q, nscale, lambda, false);
+        if (e != hipSuccess) return e;
+        k_dot_partial<<<nb, 256, 0, h->stream>>>(p, q, n, red);
+        k_cg_scalar<<<1, 256, 0, h->stream>>>(red, nb, cg, 1, log_dev, it, rel_tol);
+        k_cg_update<<<nb, 256, 0, h->stream>>>(x, r, p, q, n, cg, red);
+        k_cg_scalar<<<1, 256, 0, h->stream>>>(red, nb, cg, 2, log_dev, it, rel_tol);
+        k_cg_pupdate<<<blocks_for(n), 256, 0, h->stream>>>(p, r, n, cg);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace hoe

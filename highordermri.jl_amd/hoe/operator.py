@@ -1,0 +1,493 @@
+"""Host-side mirror of HighOrderMRI.jl's Explicit operator API over libhoe.
+
+Reference interface (HighOrderMRI.jl):
+  - ``HighOrderExplicitOp(grid, kspha, times; implementation, fieldmap, csm,
+    mask, concomitant, recon_terms, k_nominal, gpus, backend, ...)``
+    (src/EncodingOperator/HighOrderExplicitOp.jl:44-51,
+    Explicit/kernel_operator.jl:199-315, explicit_gpu_operator.jl:299-392);
+  - ``op * x`` (prod!), ``adjoint(op) * y`` (ctprod!), ``normalOperator(op, D)``
+    and ``explicit_gpu_normal(op; weights)`` (explicit_gpu_operator.jl:399-433),
+    ``close(op)`` (:435-446);
+  - ``recon_HOOp(op, Data, weight, recParams)`` (src/Recon/recon_HOOp.jl:52-88).
+
+Vectors may be numpy arrays (host; copied to and from the GPU) or torch
+tensors on the handle's GPU (used in place, no copies). Every application runs
+the gfx950 kernels of libhoe; there is no CPU execution path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._lib import ArgumentError, DimensionMismatch, check
+from .grid import Grid
+
+try:  # torch is only needed for device-resident vectors
+    import torch  # noqa: F401
+except Exception:  # pragma: no cover
+    torch = None
+
+
+def _is_torch(a) -> bool:
+    return torch is not None and isinstance(a, torch.Tensor)
+
+
+def _f32(a, name):
+    a = np.ascontiguousarray(np.asarray(a, dtype=np.float32))
+    return a
+
+
+def _c64_host(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.complex64).reshape(-1))
+
+
+class _Vec:
+    """A vector argument: host numpy (copied) or device torch (in place)."""
+
+    def __init__(self, a, n: int, name: str):
+        self.dev = _is_torch(a) and a.is_cuda
+        if self.dev:
+            if a.dtype != torch.complex64:
+                raise ArgumentError(f"{name} must be complex64 (ComplexF32)")
+            t = a.reshape(-1)
+            if not t.is_contiguous():
+                t = t.contiguous()
+            self.obj = t
+            self.ptr = t.data_ptr()
+        else:
+            if _is_torch(a):
+                a = a.detach().numpy()
+            h = _c64_host(a)
+            self.obj = h
+            self.ptr = h.ctypes.data
+        size = self.obj.numel() if self.dev else self.obj.size
+        if size != n:
+            raise DimensionMismatch(f"{name} has length {size}, expected {n}")
+
+    def like(self, n: int):
+        if self.dev:
+            return torch.empty(n, dtype=torch.complex64, device=self.obj.device)
+        return np.empty(n, dtype=np.complex64)
+
+
+def _ptr(o):
+    return o.data_ptr() if _is_torch(o) else o.ctypes.data
+
+
+class HighOrderExplicitOp:
+    """Explicit higher-order encoding operator of size ``(nSam*nCha, prod(matrixSize))``.
+
+    Parameters mirror the reference constructor. ``backend`` selects the
+    libhoe execution path: ``"direct"`` (FP32 VALU contraction), ``"mixed"``
+    / ``"mixed_fp16"`` or ``"mixed_bf16"`` (MFMA phase-reuse). ``dwell`` is
+    the coefficient sampling interval used by ``set_coefficients(..., tau)``.
+    """
+
+    def __init__(self, grid: Grid, kspha, times, *, implementation: str = "kernel", fieldmap=None,
+                 csm=None, mask=None, concomitant=None, recon_terms: str | None = None,
+                 k_nominal=None, kspha_dt=None, gpus=None, backend: str = "direct",
+                 dwell: float = 0.0, verbose: bool = False):
+        if implementation not in ("kernel", "array"):
+            raise ArgumentError("implementation must be :array or :kernel")
+        if implementation == "array":
+            raise ArgumentError("implementation=:array (the reference CPU array path) is not provided; "
+                                "use implementation=:kernel")
+        if kspha_dt is not None:
+            raise ArgumentError("kspha_dt requires backend=:legacy")
+        if backend not in _lib.BACKENDS:
+            raise ArgumentError(f"unknown explicit backend {backend}")
+        gpus = [0] if gpus is None else [int(g) for g in gpus]
+        if len(gpus) == 0:
+            raise ArgumentError("gpus must contain at least one device id")
+        if len(set(gpus)) != len(gpus):
+            raise ArgumentError(f"gpus contains duplicate device ids: {gpus}")
+        if len(gpus) > 1:
+            raise ArgumentError("one GPU per operator: sample-sharded multi-GPU execution is not built")
+        lib = _lib.load()
+        kspha = _f32(kspha, "kspha")
+        if kspha.ndim != 2:
+            raise DimensionMismatch("kspha must be (nTerm, nSam)")
+        n_term, n_sam = kspha.shape
+        if n_term not in (9, 16):
+            raise ArgumentError("kspha requires 9 or 16 terms")
+        times = _f32(times, "times").reshape(-1)
+        if times.size != n_sam:
+            raise DimensionMismatch("times and kspha")
+        nX, nY, nZ = grid.matrixSize
+        n_grid = nX * nY * nZ
+        if mask is None:
+            mask_u8 = np.ones(n_grid, dtype=np.uint8)
+        else:
+            m = np.asarray(mask)
+            if m.size != n_grid:
+                raise DimensionMismatch("mask and grid")
+            mask_u8 = np.ascontiguousarray(m.reshape(-1, order="F") if m.ndim > 1 else m,
+                                           dtype=np.uint8)
+        fm = None
+        if fieldmap is not None:
+            f = np.asarray(fieldmap, dtype=np.float32)
+            if f.size != n_grid:
+                raise DimensionMismatch("fieldmap and mask")
+            fm = np.ascontiguousarray(f.reshape(-1, order="F") if f.ndim > 1 else f)
+        if csm is None:
+            n_cha, cs = 1, None
+        else:
+            c = np.asarray(csm, dtype=np.complex64)
+            if c.ndim == 1:
+                c = c.reshape(-1, 1)
+            n_cha = c.shape[-1]
+            if c.size != n_grid * n_cha:
+                raise DimensionMismatch("csm and grid")
+            cs = np.ascontiguousarray(c.reshape(n_grid, n_cha, order="F") if c.ndim > 2 else c)
+            cs = np.ascontiguousarray(cs.T)   # [coil][grid]: grid fastest per coil
+        conc = None
+        if concomitant is not None:
+            ca = np.asarray(concomitant)
+            if np.iscomplexobj(ca) or not np.issubdtype(ca.dtype, np.number):
+                raise ArgumentError("concomitant must be a real array or nothing")
+            if ca.shape != (4, n_sam):
+                raise DimensionMismatch(f"concomitant must have size (4, {n_sam}), got {ca.shape}")
+            conc = _f32(ca, "concomitant")
+        kn = None
+        if k_nominal is not None:
+            kn = _f32(k_nominal, "k_nominal")
+            if kn.shape[0] != 3:
+                raise AssertionError("k_nominal must have 3 terms (row) for kx, ky, kz")
+            if kn.shape != (3, n_sam):
+                raise DimensionMismatch("k_nominal must be (3, nSam)")
+        # column-major host arrays: Julia (nTerm, nSam) is term-fastest = numpy (nSam, nTerm) C order
+        self._keep = [grid.x.astype(np.float32), grid.y.astype(np.float32), grid.z.astype(np.float32),
+                      mask_u8, np.ascontiguousarray(kspha.T), times]
+        d = _lib.ModelDesc()
+        d.n_x, d.n_y, d.n_z = nX, nY, nZ
+        d.x, d.y, d.z = (self._keep[0].ctypes.data, self._keep[1].ctypes.data, self._keep[2].ctypes.data)
+        d.mask = mask_u8.ctypes.data
+        d.n_term, d.n_sam = n_term, n_sam
+        d.kspha = self._keep[4].ctypes.data
+        d.times = times.ctypes.data
+        if fm is not None:
+            self._keep.append(fm)
+            d.fieldmap = fm.ctypes.data
+        d.n_cha = n_cha
+        if cs is not None:
+            self._keep.append(cs)
+            d.csm = cs.ctypes.data
+        if conc is not None:
+            ct = np.ascontiguousarray(conc.T)
+            self._keep.append(ct)
+            d.concomitant = ct.ctypes.data
+        if recon_terms is not None:
+            d.recon_terms = recon_terms.encode()
+        if kn is not None:
+            kt = np.ascontiguousarray(kn.T)
+            self._keep.append(kt)
+            d.k_nominal = kt.ctypes.data
+        d.dwell = float(dwell)
+        d.backend = _lib.BACKENDS[backend]
+        d.device = gpus[0]
+        h = C.c_void_p()
+        check(lib.hoe_create(C.byref(d), C.byref(h)))
+        self._keep = None
+        self._h = h
+        self._lib = lib
+        self.grid = grid
+        self.nSam, self.nCha, self.nGrid = n_sam, n_cha, n_grid
+        self.nTerm = n_term + (4 if concomitant is not None else 0)
+        info = _lib.Info()
+        check(lib.hoe_get_info(h, C.byref(info)), h)
+        self.nVox = int(info.n_vox)
+        self.nrow, self.ncol = n_sam * n_cha, n_grid
+        self.device = gpus[0]
+        self.backend = backend
+        self.symmetric = self.hermitian = False
+        self._weights_owner = None
+
+    # ---- LinearOperator protocol ----
+    @property
+    def shape(self):
+        return (self.nrow, self.ncol)
+
+    def size(self, dim=None):
+        return self.shape if dim is None else self.shape[dim - 1]
+
+    def storage_type(self):
+        return "device"
+
+    def _handle(self):
+        if self._h is None:
+            raise ArgumentError("operator is closed")
+        return self._h
+
+    def set_backend(self, backend: str):
+        if backend not in _lib.BACKENDS:
+            raise ArgumentError(f"unknown explicit backend {backend}")
+        check(self._lib.hoe_set_backend(self._handle(), _lib.BACKENDS[backend]), self._h)
+        self.backend = backend
+
+    def prod(self, x, out=None):
+        """``op * x``: x of length prod(matrixSize) -> signal of length nSam*nCha."""
+        h = self._handle()
+        xv = _Vec(x, self.ncol, "x")
+        y = out if out is not None else xv.like(self.nrow)
+        flags = _lib.HOE_DEVICE_PTRS if xv.dev else 0
+        if xv.dev:
+            torch.cuda.current_stream(xv.obj.device).synchronize()
+        check(self._lib.hoe_forward(h, xv.ptr, _ptr(y), flags), h)
+        return y
+
+    def ctprod(self, y, out=None):
+        """``adjoint(op) * y``."""
+        h = self._handle()
+        yv = _Vec(y, self.nrow, "y")
+        x = out if out is not None else yv.like(self.ncol)
+        flags = _lib.HOE_DEVICE_PTRS if yv.dev else 0
+        if yv.dev:
+            torch.cuda.current_stream(yv.obj.device).synchronize()
+        check(self._lib.hoe_adjoint(h, yv.ptr, _ptr(x), flags), h)
+        return x
+
+    def __mul__(self, x):
+        return self.prod(x)
+
+    def __matmul__(self, x):
+        return self.prod(x)
+
+    @property
+    def H(self):
+        return AdjointOp(self)
+
+    def adjoint(self):
+        return AdjointOp(self)
+
+    def set_coefficients(self, kspha=None, tau: float = 0.0):
+        """Replace the coefficient traces ``(nTerm_eff, nSam)`` and the delay tau (s)."""
+        h = self._handle()
+        ptr = None
+        if kspha is not None:
+            k = _f32(kspha, "kspha")
+            if k.shape != (self.nTerm, self.nSam):
+                raise DimensionMismatch(f"kspha must be ({self.nTerm}, {self.nSam})")
+            kt = np.ascontiguousarray(k.T)
+            ptr = kt.ctypes.data
+        check(self._lib.hoe_set_coefficients(h, ptr, float(tau)), h)
+
+    def grad(self, x, y):
+        """Gradient of ``||E(h,tau) x - y||^2``: returns (g_h (nTerm, nSam), g_tau, loss)."""
+        h = self._handle()
+        xv = _Vec(x, self.ncol, "x")
+        yv = _Vec(y, self.nrow, "y")
+        if xv.dev != yv.dev:
+            raise ArgumentError("x and y must both be host or both be device arrays")
+        flags = _lib.HOE_DEVICE_PTRS if xv.dev else 0
+        if xv.dev:
+            torch.cuda.current_stream(xv.obj.device).synchronize()
+        g = np.empty((self.nSam, self.nTerm), dtype=np.float32)
+        gt, loss = C.c_double(), C.c_double()
+        check(self._lib.hoe_grad(h, xv.ptr, yv.ptr, g.ctypes.data, C.byref(gt), C.byref(loss), flags), h)
+        return np.ascontiguousarray(g.T), gt.value, loss.value
+
+    def last_kernel_ms(self, which: int = 2) -> float:
+        return float(self._lib.hoe_last_kernel_ms(self._handle(), which))
+
+    def close(self):
+        if self._h is not None:
+            self._lib.hoe_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class AdjointOp:
+    def __init__(self, op: HighOrderExplicitOp):
+        self.op = op
+
+    @property
+    def shape(self):
+        return (self.op.ncol, self.op.nrow)
+
+    def __mul__(self, y):
+        return self.op.ctprod(y)
+
+    def __matmul__(self, y):
+        return self.op.ctprod(y)
+
+    def prod(self, y, out=None):
+        return self.op.ctprod(y, out)
+
+    @property
+    def H(self):
+        return self.op
+
+
+@dataclass
+class WeightingOp:
+    """Diagonal weighting ``W`` of raw complex sample weights repeated over coils
+    (the LinearOperatorCollection WeightingOp used at recon_HOOp.jl:77)."""
+    weights: object
+    rep: int = 1
+
+
+@dataclass
+class ComposedOp:
+    """``W ∘ op``."""
+    W: WeightingOp
+    op: HighOrderExplicitOp
+
+
+def compose(W: WeightingOp, op: HighOrderExplicitOp) -> ComposedOp:
+    return ComposedOp(W, op)
+
+
+def _check_diag(D, nrow, nsam):
+    D = np.asarray(D).reshape(-1)
+    if D.size == 0:
+        raise DimensionMismatch("normal weights")
+    if not np.all(np.isfinite(D)):
+        raise ArgumentError("normal weights must be finite")
+    Dr = np.real(D).astype(np.float64)
+    if np.iscomplexobj(D):
+        scale = max(1.0, float(np.max(np.abs(D))))
+        eps = np.finfo(np.float32).eps
+        if float(np.max(np.abs(np.imag(D)))) > 10 * eps * scale:
+            raise ArgumentError("normal weights must already be real nonnegative WᴴW")
+    if np.min(Dr) < 0:
+        raise ArgumentError("normal weights must be nonnegative")
+    if D.size not in (nsam, nrow):
+        raise DimensionMismatch("normal weights")
+    return np.ascontiguousarray(Dr.astype(np.float32))
+
+
+class NormalOp:
+    """``Aᴴ D A`` sharing the parent's handle (not reentrant)."""
+
+    _gen = 0
+
+    def __init__(self, op: HighOrderExplicitOp, D=None):
+        op._handle()
+        self.op = op
+        self.D = None if D is None else _check_diag(D, op.nrow, op.nSam)
+        NormalOp._gen += 1
+        self._id = NormalOp._gen
+        self.symmetric, self.hermitian = False, True
+
+    @property
+    def shape(self):
+        return (self.op.ncol, self.op.ncol)
+
+    def _sync_weights(self, h):
+        lib = self.op._lib
+        if self.op._weights_owner != self._id:
+            if self.D is None:
+                check(lib.hoe_set_weights(h, None, 0), h)
+            else:
+                check(lib.hoe_set_weights(h, self.D.ctypes.data, self.D.size), h)
+            self.op._weights_owner = self._id
+
+    def prod(self, x, out=None):
+        h = self.op._handle()
+        self._sync_weights(h)
+        xv = _Vec(x, self.op.ncol, "x")
+        o = out if out is not None else xv.like(self.op.ncol)
+        flags = _lib.HOE_DEVICE_PTRS if xv.dev else 0
+        if xv.dev:
+            torch.cuda.current_stream(xv.obj.device).synchronize()
+        check(self.op._lib.hoe_normal(h, None, _lib.HOE_CURRENT_WEIGHTS, xv.ptr, _ptr(o), flags), h)
+        return o
+
+    def __mul__(self, x):
+        return self.prod(x)
+
+    def __matmul__(self, x):
+        return self.prod(x)
+
+
+def normalOperator(op, weights=None):
+    """``normalOperator(op, D)``: D is the already-squared real diagonal (length
+    nSam or nSam*nCha); ``normalOperator(W ∘ op)`` takes ``|W.weights|²``
+    (explicit_gpu_operator.jl:399-416)."""
+    if isinstance(op, ComposedOp):
+        w = np.asarray(op.W.weights).reshape(-1)
+        D = (np.abs(w.astype(np.complex128)) ** 2)
+        if D.size * op.W.rep == op.op.nrow:
+            D = np.tile(D, op.W.rep)
+        return NormalOp(op.op, D)
+    if isinstance(weights, WeightingOp):
+        weights = weights.weights
+    return NormalOp(op, weights)
+
+
+def explicit_gpu_normal(op: HighOrderExplicitOp, weights=None):
+    """Normal ``Aᴴ D A`` with ``D = abs2.(weights)`` from raw weights (explicit_gpu_operator.jl:418-433)."""
+    if weights is None:
+        return NormalOp(op, None)
+    w = np.asarray(weights).reshape(-1).astype(np.complex128)
+    return NormalOp(op, np.abs(w) ** 2)
+
+
+@dataclass
+class L2Regularization:
+    """``L2Regularization(λ)``: adds λI to the normal equations."""
+    λ: float = 0.0
+
+
+class CGNR:
+    """Solver tag for recParams[:solver] (RegularizedLeastSquares.CGNR)."""
+
+
+def recon_HOOp(HOOp: HighOrderExplicitOp, Data, weight, recParams: dict, *, return_log: bool = False):
+    """Weighted CGNR reconstruction (src/Recon/recon_HOOp.jl:52-88).
+
+    Solves ``(Aᴴ WᴴW A + λI) x = Aᴴ WᴴW y`` with x₀ = 0 on the GPU (FP64 inner
+    products). ``Data`` is ``(nSam, nCha)``; ``weight`` holds one raw complex
+    weight per sample (not squared). ``recParams`` must contain ``solver``,
+    ``reg`` and ``reconSize``; ``iterations`` (default 10) and ``relTol``
+    (default 0: run every iteration) are optional. Returns the image reshaped
+    to ``reconSize`` (column-major) as a numpy array.
+    """
+    for key in ("solver", "reg", "reconSize"):
+        if key not in recParams:
+            raise ArgumentError(f"recParams must contain :{key}")
+    solver = recParams["solver"]
+    if solver is not CGNR and not isinstance(solver, CGNR):
+        raise ArgumentError("only the CGNR solver is provided")
+    reg = recParams["reg"]
+    lam = float(reg.λ if isinstance(reg, L2Regularization) else reg)
+    iters = int(recParams.get("iterations", 10))
+    rel_tol = float(recParams.get("relTol", 0.0))
+    op = HOOp
+    h = op._handle()
+    data = Data
+    if _is_torch(data) and data.is_cuda:
+        yv = _Vec(data.t().contiguous().reshape(-1) if data.dim() == 2 else data, op.nrow, "Data")
+    else:
+        d = np.asarray(data)
+        yv = _Vec(d.reshape(-1, order="F") if d.ndim == 2 else d, op.nrow, "Data")
+    w = weight.detach().cpu().numpy() if _is_torch(weight) else np.asarray(weight)
+    w = w.reshape(-1)
+    if w.size != op.nSam:
+        raise DimensionMismatch("weight must have one entry per sample")
+    D = np.ascontiguousarray((np.abs(w.astype(np.complex128)) ** 2).astype(np.float32))
+    op._weights_owner = None
+    x = yv.like(op.ncol)
+    log = np.zeros(iters + 1, dtype=np.float64)
+    flags = _lib.HOE_DEVICE_PTRS if yv.dev else 0
+    if yv.dev:
+        torch.cuda.current_stream(yv.obj.device).synchronize()
+    check(op._lib.hoe_cg(h, yv.ptr, D.ctypes.data, D.size, iters, lam, rel_tol, _ptr(x), log.ctypes.data,
+                         flags), h)
+    img = x.detach().cpu().numpy() if _is_torch(x) else x
+    img = np.asarray(img).reshape(recParams["reconSize"], order="F")
+    return (img, log) if return_log else img

@@ -1,0 +1,365 @@
+"""CPU oracle of HighOrderMRI.jl's Explicit encoding operator.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py as the checker; never by the product.
+
+Each function cites the reference code it restates:
+  - prep_kspha / prep_concomitant: src/EncodingOperator/prep_kspha.jl:1-56, 78-98
+  - masking and ordering: src/EncodingOperator/Explicit/array_operator.jl:109-138
+  - FP64 operator: test/test_explicit_gpu.jl:8-35 (explicit_reference_basis/_matrix)
+  - CPU Array path (cpu_ref_*): array_operator.jl:241-348
+  - deterministic fixture: test/test_HighOrderExplicit.jl:3-85
+  - normal weights: explicit_gpu_operator.jl:399-416; reconstruction: recon_HOOp.jl:73-81
+The heavy loops live in hoe_oracle.c (liboracle.so, built by oracle/Makefile).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        L = C.CDLL(str(LIB))
+        vp, i64, ci = C.c_void_p, C.c_int64, C.c_int
+        L.oracle_basis_f64.argtypes = [vp, vp, vp, i64, vp]
+        L.oracle_basis_f64d.argtypes = [vp, vp, vp, i64, vp]
+        L.cpu_ref_basis_f32.argtypes = [vp, vp, vp, i64, vp]
+        for n in ("oracle_forward_f64", "oracle_adjoint_f64"):
+            getattr(L, n).argtypes = [i64, i64, i64, ci, vp, vp, vp, vp, vp, vp, vp]
+        L.oracle_grad_f64.argtypes = [i64, i64, i64, ci, vp, vp, vp, vp, vp, vp, vp, vp]
+        for n in ("cpu_ref_forward", "cpu_ref_adjoint"):
+            getattr(L, n).argtypes = [i64, i64, i64, ci, vp, vp, vp, vp, vp, vp, vp, i64]
+        L.oracle_num_threads.restype = ci
+        L.oracle_set_threads.argtypes = [ci]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data
+
+
+# ---------------------------------------------------------------- model prep
+
+def prep_kspha(kspha, k_nominal, n_term, recon_terms=None):
+    """prep_kspha.jl:1-56 (returns a modified copy)."""
+    if recon_terms is None:
+        recon_terms = "111" if n_term == 9 else "1111"
+    expect = 3 if n_term == 9 else 4 if n_term == 16 else 0
+    if expect == 0:
+        raise ValueError(f"nTerm must be 9 or 16, got {n_term}")
+    if len(recon_terms) != expect or any(c not in "01" for c in recon_terms):
+        raise ValueError("bad recon_terms")
+    t = [c == "1" for c in recon_terms] + ([False] if n_term == 9 else [])
+    k = np.array(kspha, copy=True)
+    if not t[0]:
+        k[0, :] = 0
+    if not t[1]:
+        k[1:4, :] = k_nominal
+    if not t[2]:
+        k[4:9, :] = 0
+    if not t[3] and n_term == 16:
+        k[9:16, :] = 0
+    return k
+
+
+def basis_f64(x, y, z):
+    """FP64 basis [n][16] from FP32 coordinates (test_explicit_gpu.jl:8-13)."""
+    x, y, z = (np.ascontiguousarray(a, dtype=np.float32) for a in (x, y, z))
+    out = np.empty((x.size, 16), dtype=np.float64)
+    lib().oracle_basis_f64(_p(x), _p(y), _p(z), x.size, _p(out))
+    return out
+
+
+def basis_f64d(x, y, z):
+    """FP64 basis [n][16] from FP64 coordinates."""
+    x, y, z = (np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z))
+    out = np.empty((x.size, 16), dtype=np.float64)
+    lib().oracle_basis_f64d(_p(x), _p(y), _p(z), x.size, _p(out))
+    return out
+
+
+def basis_f32(x, y, z):
+    """FP32 basis [n][16] in Julia Float32 operation order (basisfunction.jl:47-78)."""
+    x, y, z = (np.ascontiguousarray(a, dtype=np.float32) for a in (x, y, z))
+    out = np.empty((x.size, 16), dtype=np.float32)
+    lib().cpu_ref_basis_f32(_p(x), _p(y), _p(z), x.size, _p(out))
+    return out
+
+
+@dataclass
+class Model:
+    """Masked operator inputs; arrays follow the reference's column-major conventions."""
+    nv: int
+    ns: int
+    nc: int
+    L: int
+    mask_idx: np.ndarray        # [nv] flat grid indices (column-major order)
+    n_grid: int
+    bf64: np.ndarray            # [nv][L]
+    bf32: np.ndarray            # [nv][L]
+    k64: np.ndarray             # [ns][L]
+    k32: np.ndarray             # [ns][L]
+    times: np.ndarray           # [ns] float32
+    fieldmap: np.ndarray        # [nv] float32
+    csm: np.ndarray             # [nc][nv] complex64
+
+
+def make_model(grid, kspha, times, fieldmap=None, csm=None, mask=None, concomitant=None, recon_terms=None,
+               k_nominal=None):
+    """array_operator.jl:102-139 (masking, prep_kspha, basisfunc_spha, prep_concomitant)."""
+    kspha = np.asarray(kspha, dtype=np.float32)
+    n_term, ns = kspha.shape
+    n_grid = grid.nX * grid.nY * grid.nZ
+    m = np.ones(n_grid, dtype=bool) if mask is None else np.asarray(mask).reshape(-1, order="F").astype(bool)
+    idx = np.nonzero(m)[0]
+    kn = kspha[1:4, :] if k_nominal is None else np.asarray(k_nominal, dtype=np.float32)
+    k = prep_kspha(kspha, kn, n_term, recon_terms)
+    x, y, z = grid.x[idx], grid.y[idx], grid.z[idx]
+    b64 = basis_f64(x, y, z)[:, :n_term]
+    b32 = basis_f32(x, y, z)[:, :n_term]
+    if concomitant is not None:
+        conc = np.asarray(concomitant, dtype=np.float32)
+        X, Y, Z = (a.astype(np.float64) for a in (x, y, z))
+        c64 = np.stack([Z * Z, X * X + Y * Y, X * Z, Y * Z], axis=1)
+        xf, yf, zf = x.astype(np.float32), y.astype(np.float32), z.astype(np.float32)
+        c32 = np.stack([zf * zf, xf * xf + yf * yf, xf * zf, yf * zf], axis=1).astype(np.float32)
+        b64 = np.concatenate([b64, c64], axis=1)
+        b32 = np.concatenate([b32, c32], axis=1)
+        k = np.concatenate([k, conc], axis=0)
+    fm = np.zeros(n_grid, dtype=np.float32) if fieldmap is None else \
+        np.asarray(fieldmap, dtype=np.float32).reshape(-1, order="F")
+    if csm is None:
+        cs = np.ones((1, idx.size), dtype=np.complex64)
+    else:
+        c = np.asarray(csm, dtype=np.complex64)
+        c = c.reshape(n_grid, -1, order="F") if c.ndim != 2 else c
+        cs = np.ascontiguousarray(c[idx, :].T)
+    L = k.shape[0]
+    return Model(nv=idx.size, ns=ns, nc=cs.shape[0], L=L, mask_idx=idx, n_grid=n_grid,
+                 bf64=np.ascontiguousarray(b64), bf32=np.ascontiguousarray(b32, dtype=np.float32),
+                 k64=np.ascontiguousarray(k.T.astype(np.float64)),
+                 k32=np.ascontiguousarray(k.T.astype(np.float32)),
+                 times=np.ascontiguousarray(np.asarray(times, dtype=np.float32).reshape(-1)),
+                 fieldmap=np.ascontiguousarray(fm[idx]), csm=cs)
+
+
+def with_coefficients(m: Model, k_rows) -> Model:
+    """Same model with effective coefficients replaced by ``k_rows`` ([L][ns])."""
+    k = np.asarray(k_rows, dtype=np.float64)
+    return Model(**{**m.__dict__, "k64": np.ascontiguousarray(k.T),
+                    "k32": np.ascontiguousarray(k.T.astype(np.float32))})
+
+
+def gather(m: Model, x_full):
+    return np.asarray(x_full).reshape(-1)[m.mask_idx]
+
+
+def scatter(m: Model, xm, dtype=np.complex128):
+    out = np.zeros(m.n_grid, dtype=dtype)
+    out[m.mask_idx] = xm
+    return out
+
+
+# ---------------------------------------------------------------- FP64 oracle
+
+def forward_f64(m: Model, x_full):
+    """E x in FP64 -> [nc*ns] (samples fastest)."""
+    xm = np.ascontiguousarray(gather(m, x_full).astype(np.complex128))
+    cs = np.ascontiguousarray(m.csm.astype(np.complex128))
+    y = np.empty(m.nc * m.ns, dtype=np.complex128)
+    lib().oracle_forward_f64(m.nv, m.ns, m.nc, m.L, _p(m.bf64), _p(m.k64), _p(m.times.astype(np.float64)),
+                             _p(m.fieldmap.astype(np.float64)), _p(cs), _p(xm), _p(y))
+    return y
+
+
+def adjoint_f64(m: Model, y):
+    ys = np.ascontiguousarray(np.asarray(y).reshape(-1).astype(np.complex128))
+    cs = np.ascontiguousarray(m.csm.astype(np.complex128))
+    xm = np.empty(m.nv, dtype=np.complex128)
+    lib().oracle_adjoint_f64(m.nv, m.ns, m.nc, m.L, _p(m.bf64), _p(m.k64), _p(m.times.astype(np.float64)),
+                             _p(m.fieldmap.astype(np.float64)), _p(cs), _p(ys), _p(xm))
+    return scatter(m, xm)
+
+
+def expand_weights(m: Model, D):
+    if D is None:
+        return np.ones(m.ns * m.nc)
+    D = np.asarray(D, dtype=np.float64).reshape(-1)
+    return np.tile(D, m.nc) if D.size == m.ns else D
+
+
+def normal_f64(m: Model, x_full, D=None):
+    """Aᴴ D A x (D already squared; explicit_gpu_operator.jl:399-416)."""
+    return adjoint_f64(m, expand_weights(m, D) * forward_f64(m, x_full))
+
+
+def grad_f64(m: Model, x_full, y):
+    """dL/dk_l(s) for L = ||E x - y||^2, FP64; returns (g [L][ns], loss, residual)."""
+    r = forward_f64(m, x_full) - np.asarray(y).reshape(-1)
+    xm = np.ascontiguousarray(gather(m, x_full).astype(np.complex128))
+    cs = np.ascontiguousarray(m.csm.astype(np.complex128))
+    rr = np.ascontiguousarray(r.astype(np.complex128))
+    g = np.empty((m.L, m.ns), dtype=np.float64)
+    lib().oracle_grad_f64(m.nv, m.ns, m.nc, m.L, _p(m.bf64), _p(m.k64), _p(m.times.astype(np.float64)),
+                          _p(m.fieldmap.astype(np.float64)), _p(cs), _p(xm), _p(rr), _p(g))
+    return g, float(np.vdot(r, r).real), r
+
+
+def dense_matrix_f64(m: Model):
+    """Dense FP64 encoding matrix (nc*ns, n_grid) (test_explicit_gpu.jl:15-35); tiny problems only."""
+    ph = m.k64 @ m.bf64.T + np.outer(m.times.astype(np.float64), m.fieldmap.astype(np.float64))
+    P = np.exp(2j * np.pi * ph) / np.sqrt(m.nv)
+    A = np.zeros((m.nc * m.ns, m.n_grid), dtype=np.complex128)
+    for c in range(m.nc):
+        A[c * m.ns:(c + 1) * m.ns, m.mask_idx] = P * m.csm[c].astype(np.complex128)[None, :]
+    return A
+
+
+# ---------------------------------------------------------------- reference CPU path
+
+def cpu_ref_forward(m: Model, x_full, nblock=50):
+    xm = np.ascontiguousarray(gather(m, x_full).astype(np.complex64))
+    y = np.empty(m.nc * m.ns, dtype=np.complex64)
+    lib().cpu_ref_forward(m.nv, m.ns, m.nc, m.L, _p(m.bf32), _p(m.k32), _p(m.times), _p(m.fieldmap),
+                          _p(np.ascontiguousarray(m.csm)), _p(xm), _p(y), nblock)
+    return y
+
+
+def cpu_ref_adjoint(m: Model, y, nblock=50):
+    ys = np.ascontiguousarray(np.asarray(y).reshape(-1).astype(np.complex64))
+    xm = np.empty(m.nv, dtype=np.complex64)
+    lib().cpu_ref_adjoint(m.nv, m.ns, m.nc, m.L, _p(m.bf32), _p(m.k32), _p(m.times), _p(m.fieldmap),
+                          _p(np.ascontiguousarray(m.csm)), _p(ys), _p(xm), nblock)
+    return scatter(m, xm, np.complex64)
+
+
+def cpu_ref_normal(m: Model, x_full, D=None, nblock=50):
+    y = cpu_ref_forward(m, x_full, nblock)
+    return cpu_ref_adjoint(m, (expand_weights(m, D) * y).astype(np.complex64), nblock)
+
+
+# ---------------------------------------------------------------- solvers
+
+def cgnr(apply_normal, rhs, iters, lam=0.0, rel_tol=0.0):
+    """Textbook CG on (N + λI) x = rhs, x0 = 0, FP64 (the CGNR iteration of
+    RegularizedLeastSquares invoked at recon_HOOp.jl:80-81)."""
+    x = np.zeros_like(rhs, dtype=np.complex128)
+    r = rhs.astype(np.complex128).copy()
+    p = r.copy()
+    rr = np.vdot(r, r).real
+    rr0 = rr
+    log = [1.0 if rr0 > 0 else 0.0]
+    for _ in range(iters):
+        if rr0 == 0 or (rel_tol > 0 and np.sqrt(rr / rr0) <= rel_tol):
+            log.append(0.0)
+            continue
+        q = apply_normal(p) + lam * p
+        pap = np.vdot(p, q).real
+        if pap <= 0:
+            break
+        a = rr / pap
+        x += a * p
+        r -= a * q
+        rn = np.vdot(r, r).real
+        log.append(np.sqrt(rn / rr0))
+        p = r + (rn / rr) * p
+        rr = rn
+    return x, np.array(log)
+
+
+# ---------------------------------------------------------------- delay model
+
+def catmull_rom_weights(f):
+    f2, f3 = f * f, f * f * f
+    w = np.array([0.5 * (-f3 + 2 * f2 - f), 0.5 * (3 * f3 - 5 * f2 + 2), 0.5 * (-3 * f3 + 4 * f2 + f),
+                  0.5 * (f3 - f2)])
+    dw = np.array([0.5 * (-3 * f2 + 4 * f - 1), 0.5 * (9 * f2 - 10 * f), 0.5 * (-9 * f2 + 8 * f + 1),
+                   0.5 * (3 * f2 - 2 * f)])
+    return w, dw
+
+
+def delayed(traces, tau, dwell):
+    """Traces [L][ns] resampled at s + tau/dwell (uniform Catmull-Rom, clamped ends)."""
+    tr = np.asarray(traces, dtype=np.float64)
+    if tau == 0:
+        return tr.copy()
+    u = tau / dwell
+    m = int(np.floor(u))
+    w, _ = catmull_rom_weights(u - m)
+    ns = tr.shape[1]
+    out = np.zeros_like(tr)
+    s = np.arange(ns)
+    for t in range(4):
+        j = np.clip(s + m - 1 + t, 0, ns - 1)
+        out += w[t] * tr[:, j]
+    return out
+
+
+# ---------------------------------------------------------------- reference fixtures
+
+def _jrange(a, b, n, T=np.float32):
+    return (np.float64(T(a)) + (np.float64(T(b)) - np.float64(T(a))) * np.arange(n) / (n - 1)).astype(T)
+
+
+def explicit_highorder_test_problem(n_coil=3, grid_cls=None):
+    """The deterministic fixture of test/test_HighOrderExplicit.jl:3-59 (ranges, sin, cos only)."""
+    import sys
+    if grid_cls is None:
+        sys.path.insert(0, str(HERE.parent / "highordermri.jl_amd"))
+        from hoe.grid import Grid as grid_cls  # the Grid of the host API (pure index arithmetic)
+    f32 = np.float32
+    n_x, n_y, n_z = 4, 3, 2
+    ns, nt = 7, 16
+    grid = grid_cls(n_x, n_y, n_z, f32(0.7), f32(0.9), f32(1.2))
+    times = _jrange(0.0, 1.3e-3, ns)
+    k = np.zeros((nt, ns), dtype=f32)
+    k[0] = _jrange(-0.01, 0.015, ns)
+    k[1] = _jrange(-0.08, 0.07, ns)
+    k[2] = _jrange(0.06, -0.05, ns)
+    k[3] = _jrange(-0.03, 0.04, ns)
+    ang = _jrange(0.0, np.pi, ns)
+    for term in range(5, 10):
+        k[term - 1] = f32(4e-4 * (term - 4)) * np.sin(ang).astype(f32)
+    for term in range(10, 17):
+        k[term - 1] = f32(2e-5 * (term - 9)) * np.cos(ang).astype(f32)
+    n = n_x * n_y * n_z
+    fieldmap = _jrange(-25.0, 20.0, n)
+    nxv = (grid.x / np.max(np.abs(grid.x))).astype(f32)
+    nyv = (grid.y / np.max(np.abs(grid.y))).astype(f32)
+    nzv = (grid.z / np.max(np.abs(grid.z))).astype(f32)
+    csm = np.empty((n, n_coil), dtype=np.complex64)
+    for coil in range(1, n_coil + 1):
+        angle = f32(2 * np.pi * (coil - 1) / n_coil)
+        mag = f32(1) + f32(0.08) * (np.cos(angle).astype(f32) * nxv + np.sin(angle).astype(f32) * nyv)
+        ph = f32(0.12 * coil) * (nyv - nzv)
+        csm[:, coil - 1] = mag * np.exp(1j * ph.astype(np.float64)).astype(np.complex64)
+    mask = np.ones(n, dtype=bool)
+    mask[0] = False
+    mask[n - 1] = False
+    image = (_jrange(0.1, 1.0, n) + 1j * _jrange(-0.2, 0.15, n)).astype(np.complex64)
+    return dict(grid=grid, kspha=k, times=times, fieldmap=fieldmap, csm=csm, mask=mask, image=image)
+
+
+def relerr(a, b):
+    a = np.asarray(a).reshape(-1).astype(np.complex128)
+    b = np.asarray(b).reshape(-1).astype(np.complex128)
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0))

@@ -1,0 +1,188 @@
+"""GPU parity of the libhoe forward / adjoint / normal against the oracle.
+
+Mirrors the reference's operator tests: test/test_HighOrderExplicit.jl:87-128
+(deterministic fixture, rtol 3e-6 for the CPU path), test/test_explicit_gpu.jl
+(independent FP64 matrix, 3e-5 for Float32 GPU operators, the four shapes,
+adjoint identity, weighted normal, linearity, zero input, <x,Nx> >= 0,
+invalid weights, close), test_explicit_interface.jl:24-43 (recon_terms with
+k_nominal) and test_Concomitant.jl:88-137 (concomitant terms).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+import oracle as O  # noqa: E402
+import hoe  # noqa: E402
+from hoe import synth  # noqa: E402
+
+BACKENDS = ["direct", "mixed_fp16", "mixed_bf16"]
+TOL32 = 3e-5          # Float32 GPU operators vs FP64 matrix (test_explicit_gpu.jl:51)
+# Parity bars per backend: (vs FP64 matrix, vs the CPU reference path). FP16 hi/lo
+# products carry ~22 significant bits; BF16 hi/lo ~16 bits, so its bar is the
+# 1e-4 mixed-precision target.
+TOLS = {"direct": (3e-5, 1e-5), "mixed_fp16": (3e-5, 1e-5), "mixed_bf16": (1e-4, 1e-4)}
+
+
+def _op(p, backend, **kw):
+    return hoe.HighOrderExplicitOp(p["grid"], p["kspha"], p["times"], fieldmap=p.get("fieldmap"),
+                                   csm=p.get("csm"), mask=p.get("mask"), backend=backend, **kw)
+
+
+def _model(p, **kw):
+    return O.make_model(p["grid"], p["kspha"], p["times"], p.get("fieldmap"), p.get("csm"), p.get("mask"),
+                        **kw)
+
+
+def _random_case(shape, nt, nc, seed):
+    rng = np.random.default_rng(seed)
+    grid = hoe.Grid(*shape, np.float32(0.07), np.float32(0.08), np.float32(0.09))
+    ns = 11
+    n = int(np.prod(shape))
+    times = np.linspace(0.001, 0.009, ns).astype(np.float32)
+    kspha = rng.normal(size=(nt, ns)).astype(np.float32)
+    fieldmap = np.linspace(-37, 49, n).astype(np.float32)
+    csm = (rng.normal(size=(n, nc)) + 1j * rng.normal(size=(n, nc))).astype(np.complex64)
+    mask = np.ones(n, dtype=bool)
+    mask[0] = False
+    mask[n - 2] = False
+    x = (rng.normal(size=n) + 1j * rng.normal(size=n)).astype(np.complex64)
+    y = (rng.normal(size=ns * nc) + 1j * rng.normal(size=ns * nc)).astype(np.complex64)
+    w = (rng.normal(size=ns) + 1j * rng.normal(size=ns) + 0.7).astype(np.complex64)
+    return dict(grid=grid, kspha=kspha, times=times, fieldmap=fieldmap, csm=csm, mask=mask, x=x, y=y, w=w)
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_fixture_matches_dense(backend, gpu_lib):
+    TOL32 = TOLS[backend][0]
+    p = O.explicit_highorder_test_problem()
+    m = _model(p)
+    A = O.dense_matrix_f64(m)
+    n = m.ns * m.nc
+    probe = (np.linspace(-0.4, 0.5, n) + 1j * np.linspace(0.3, -0.2, n)).astype(np.complex64)
+    with _op(p, backend) as op:
+        assert op.shape == A.shape
+        fwd = op * p["image"]
+        adj = op.H * probe
+        assert O.relerr(fwd, A @ p["image"]) < TOL32
+        assert O.relerr(adj, A.conj().T @ probe) < TOL32
+        assert abs(np.vdot(probe, fwd) - np.vdot(adj, p["image"])) / (np.linalg.norm(fwd) * np.linalg.norm(probe)) < TOL32
+        assert np.all(adj[~p["mask"]] == 0)
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+@pytest.mark.parametrize("shape,nt,nc", [((5, 4, 1), 9, 3), ((4, 3, 2), 16, 33), ((4, 5, 1), 16, 5),
+                                         ((3, 4, 2), 9, 33)])
+def test_independent_fp64_reference(backend, shape, nt, nc, gpu_lib):
+    TOL32 = TOLS[backend][0]
+    c = _random_case(shape, nt, nc, 1234 + nt + nc)
+    m = _model(c)
+    A = O.dense_matrix_f64(m)
+    D = np.tile(np.abs(c["w"].astype(np.complex128)) ** 2, nc)
+    with _op(c, backend) as op:
+        fwd = op * c["x"]
+        bwd = op.H * c["y"]
+        assert O.relerr(fwd, A @ c["x"]) < TOL32
+        assert O.relerr(bwd, A.conj().T @ c["y"]) < TOL32
+        assert abs(np.vdot(c["y"], fwd) - np.vdot(bwd, c["x"])) / max(np.linalg.norm(fwd) * np.linalg.norm(c["y"]), 1) < TOL32
+        assert np.all(bwd[~c["mask"]] == 0)
+        expected = A.conj().T @ (D * (A @ c["x"]))
+        N = hoe.normalOperator(hoe.compose(hoe.WeightingOp(c["w"], nc), op))
+        result = N * c["x"]
+        assert O.relerr(result, expected) < TOL32
+        assert O.relerr(hoe.explicit_gpu_normal(op, c["w"]) * c["x"], expected) < TOL32
+        assert O.relerr(hoe.normalOperator(op, D.astype(np.float32)) * c["x"], expected) < TOL32
+        assert np.real(np.vdot(c["x"], result)) >= 0
+        assert O.relerr(N * (2 * c["x"]), 2 * result) < 2 * TOL32
+        assert np.linalg.norm(N * np.zeros_like(c["x"])) == 0
+        assert O.relerr(N * c["x"], result) < TOL32
+        with pytest.raises(hoe.ArgumentError):
+            hoe.normalOperator(op, np.full(m.ns, 1 + 1j, dtype=np.complex64))
+        with pytest.raises(hoe.ArgumentError):
+            hoe.normalOperator(op, -np.ones(m.ns, dtype=np.float32))
+    op.close()
+    op.close()
+    with pytest.raises(hoe.ArgumentError):
+        hoe.normalOperator(op)
+    with pytest.raises(hoe.ArgumentError):
+        op * c["x"]
+
+
+def test_known_answer_ones(gpu_lib):
+    # test_explicit_gpu.jl:109-113: zero coefficients and times, 2x2x1 grid, ones(4) -> fill(2, 3)
+    grid = hoe.Grid(2, 2, 1, np.float32(0.01), np.float32(0.01), np.float32(0.01))
+    for backend in BACKENDS:
+        with hoe.HighOrderExplicitOp(grid, np.zeros((9, 3), np.float32), np.zeros(3, np.float32),
+                                     backend=backend) as op:
+            np.testing.assert_allclose(op * np.ones(4, np.complex64), np.full(3, 2.0), rtol=1e-6)
+            # unit-weight normal of ones(4): (1/4) * 3 samples * 4 voxels = 3 (test_explicit_multi_gpu.jl:77)
+            np.testing.assert_allclose(hoe.normalOperator(op) * np.ones(4, np.complex64), np.full(4, 3.0),
+                                       rtol=1e-6)
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_recon_terms_and_concomitant(backend, gpu_lib):
+    TOL32 = TOLS[backend][0]
+    rng = np.random.default_rng(731)
+    grid = hoe.Grid(4, 3, 2, np.float32(0.04), np.float32(0.03), np.float32(0.02))
+    kspha = rng.normal(size=(16, 13)).astype(np.float32)
+    k_nominal = rng.normal(size=(3, 13)).astype(np.float32)
+    times = np.linspace(0.001, 0.013, 13).astype(np.float32)
+    csm = (rng.normal(size=(24, 3)) + 1j * rng.normal(size=(24, 3))).astype(np.complex64)
+    fieldmap = (rng.normal(size=24) * 25).astype(np.float32)
+    mask = np.ones(24, dtype=bool)
+    mask[0] = mask[12] = False
+    conc = (rng.normal(size=(4, 13)) * 1e-1).astype(np.float32)
+    x = (rng.normal(size=24) + 1j * rng.normal(size=24)).astype(np.complex64)
+    p = dict(grid=grid, kspha=kspha, times=times, fieldmap=fieldmap, csm=csm, mask=mask)
+    m = _model(p, recon_terms="0011", k_nominal=k_nominal, concomitant=conc)
+    A = O.dense_matrix_f64(m)
+    with _op(p, backend, recon_terms="0011", k_nominal=k_nominal, concomitant=conc) as op:
+        assert O.relerr(op * x, A @ x) < TOL32
+    with pytest.raises(hoe.DimensionMismatch):
+        _op(p, backend, concomitant=conc[:3])
+    with pytest.raises(hoe.ArgumentError):
+        _op(p, backend, concomitant=conc.astype(np.complex64))
+    with pytest.raises(hoe.ArgumentError):
+        _op(p, backend, recon_terms="012")
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_config0_matches_cpu_reference(backend, gpu_lib):
+    """Full workload 0 (64x64, 8 coils, 6000-sample spiral) vs the restated CPU
+    Array path (<= 1e-5 relative L2) and the FP64 oracle."""
+    c = synth.make(0)
+    m = _model(c)
+    x = c["image"]
+    y_ref = O.cpu_ref_forward(m, x)
+    y64 = O.forward_f64(m, x)
+    with _op(c, backend) as op:
+        y = op * x
+        e_ref, e64 = O.relerr(y, y_ref), O.relerr(y, y64)
+        print(f"config0 {backend}: fwd vs cpu_ref {e_ref:.2e}, vs fp64 {e64:.2e}, cpu_ref vs fp64 "
+              f"{O.relerr(y_ref, y64):.2e}")
+        tol = TOLS[backend][1]
+        assert e_ref < tol
+        assert e64 < tol
+        xa = op.H * y64.astype(np.complex64)
+        xa_ref = O.cpu_ref_adjoint(m, y64.astype(np.complex64))
+        assert O.relerr(xa, xa_ref) < tol
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_config1_subset_matches_cpu_reference(backend, gpu_lib):
+    """Workload 1 geometry (200x200, 32 coils) on a 1024-sample slice of the
+    trajectory so the CPU restatement finishes in seconds."""
+    c = synth.make(1, n_sam=1024)
+    m = _model(c)
+    x = c["image"]
+    y_ref = O.cpu_ref_forward(m, x)
+    with _op(c, backend) as op:
+        y = op * x
+        tol = TOLS[backend][1]
+        e1 = O.relerr(y, y_ref)
+        xa = op.H * y_ref
+        e2 = O.relerr(xa, O.cpu_ref_adjoint(m, y_ref))
+        print(f"config1 subset {backend}: fwd {e1:.2e} adj {e2:.2e}")
+        assert e1 < tol
+        assert e2 < tol
