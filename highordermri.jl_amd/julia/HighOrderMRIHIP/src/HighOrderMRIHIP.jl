@@ -1,0 +1,203 @@
+"""
+    HighOrderMRIHIP
+
+Julia binding of libhoe (include/hoe.h): an MI355X (gfx950) implementation of
+HighOrderMRI.jl's Explicit encoding operator behind the same operator API.
+
+`HIPExplicitOp <: HighOrderExplicitOp` carries the LinearOperators fields and
+`prod!`/`ctprod!` closures of the reference's resident operators
+(src/EncodingOperator/Explicit/explicit_gpu_operator.jl:89-104), so
+`op * x`, `adjoint(op) * y`, `normalOperator(op, D)`, `normalOperator(W ∘ op)`,
+`recon_HOOp` and RegularizedLeastSquares solvers work unchanged. Vectors may be
+host `Array`s (copied by the library) or device arrays whose `pointer` is a
+HIP device address on the operator's GPU (e.g. AMDGPU.jl `ROCArray`).
+"""
+module HighOrderMRIHIP
+
+using LinearAlgebra
+using LinearOperators
+using LinearOperatorCollection
+using ChainRulesCore
+import HighOrderMRI
+using HighOrderMRI: Grid, HighOrderExplicitOp
+
+export HIPExplicitOp, hip_explicit_loss
+
+const libhoe = get(ENV, "HOE_LIBRARY", joinpath(@__DIR__, "..", "..", "..", "libhoe.so"))
+
+const HOE_DEVICE_PTRS = Cint(1)
+const HOE_CURRENT_WEIGHTS = Int64(-1)
+const BACKENDS = Dict(:direct => Int32(0), :mixed => Int32(1), :mixed_fp16 => Int32(1), :mixed_bf16 => Int32(2))
+
+# Field order and types of hoe_model_desc.
+struct ModelDesc
+    n_x::Int32; n_y::Int32; n_z::Int32
+    x::Ptr{Float32}; y::Ptr{Float32}; z::Ptr{Float32}
+    mask::Ptr{UInt8}
+    n_term::Int32; n_sam::Int32
+    kspha::Ptr{Float32}; times::Ptr{Float32}; fieldmap::Ptr{Float32}
+    n_cha::Int32
+    csm::Ptr{ComplexF32}; concomitant::Ptr{Float32}
+    recon_terms::Ptr{UInt8}; k_nominal::Ptr{Float32}
+    dwell::Float64
+    backend::Int32; device::Int32
+end
+
+function check(rc::Integer, h::Ptr{Cvoid}=C_NULL)
+    rc == 0 && return nothing
+    msg = unsafe_string(ccall((:hoe_last_error, libhoe), Cstring, (Ptr{Cvoid},), h))
+    rc == 1 && throw(ArgumentError(msg))
+    rc == 2 && throw(DimensionMismatch(msg))
+    rc == 3 && throw(AssertionError(msg))
+    rc == 4 && throw(OutOfMemoryError())
+    rc == 6 && throw(ArgumentError(msg))
+    error("libhoe: ", msg)
+end
+
+ondevice(x) = !(x isa Array)
+flags(x) = ondevice(x) ? HOE_DEVICE_PTRS : Cint(0)
+
+mutable struct HIPExplicitOp{F,G} <: HighOrderExplicitOp{ComplexF32}
+    nrow::Int
+    ncol::Int
+    symmetric::Bool
+    hermitian::Bool
+    prod!::F
+    tprod!::Nothing
+    ctprod!::G
+    nprod::Int
+    ntprod::Int
+    nctprod::Int
+    Mv::Vector{ComplexF32}
+    Mtu::Vector{ComplexF32}
+    handle::Ptr{Cvoid}
+    nSam::Int
+    nCha::Int
+    nTerm::Int
+    weights_id::UInt
+end
+
+LinearOperators.storage_type(op::HIPExplicitOp) = typeof(op.Mv)
+
+handle(op::HIPExplicitOp) = op.handle == C_NULL ? throw(ArgumentError("operator is closed")) : op.handle
+
+"""
+    HIPExplicitOp(grid, kspha, times; fieldmap, csm, mask, concomitant,
+                  recon_terms, k_nominal, gpus=[0], backend=:mixed, dwell=0)
+
+Same inputs and conventions as `HighOrderExplicitOp(...; implementation=:kernel)`
+(kernel_operator.jl:199-221): `kspha` is `(9|16, nSam)` in cycles/m^p, `times` in
+seconds, `fieldmap` in Hz, `csm` `(nX,nY,nZ,nCha)`, `mask` Bool, `concomitant`
+`(4, nSam)`. `backend` is `:direct` (FP32 VALU), `:mixed`/`:mixed_fp16` or
+`:mixed_bf16` (MFMA). `dwell` is the coefficient sampling interval used by delays.
+"""
+function HIPExplicitOp(grid::Grid{Float32}, kspha::AbstractMatrix{Float32}, times::AbstractVector{Float32};
+                       fieldmap=zeros(Float32, grid.matrixSize...), csm=ones(ComplexF32, grid.matrixSize..., 1),
+                       mask=trues(grid.matrixSize...), concomitant=nothing, recon_terms=nothing,
+                       k_nominal=nothing, gpus=[0], backend::Symbol=:mixed, dwell::Real=0)
+    length(gpus) == 1 || throw(ArgumentError("HIPExplicitOp runs on one GPU"))
+    haskey(BACKENDS, backend) || throw(ArgumentError("unknown explicit backend $backend"))
+    nTerm, nSam = size(kspha)
+    nCha = size(csm)[end]
+    x, y, z = Vector{Float32}(grid.x), Vector{Float32}(grid.y), Vector{Float32}(grid.z)
+    m = Vector{UInt8}(vec(mask))
+    k = Matrix{Float32}(kspha)
+    t = Vector{Float32}(times)
+    f = Vector{Float32}(vec(fieldmap))
+    c = Vector{ComplexF32}(vec(csm))                       # grid fastest, then coils
+    conc = concomitant === nothing ? nothing : Matrix{Float32}(concomitant)
+    kn = k_nominal === nothing ? nothing : Matrix{Float32}(k_nominal)
+    rt = recon_terms === nothing ? nothing : Vector{UInt8}(codeunits(recon_terms * "\0"))
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    GC.@preserve x y z m k t f c conc kn rt begin
+        d = ModelDesc(grid.nX, grid.nY, grid.nZ, pointer(x), pointer(y), pointer(z), pointer(m),
+                      nTerm, nSam, pointer(k), pointer(t), pointer(f), nCha, pointer(c),
+                      conc === nothing ? C_NULL : pointer(conc), rt === nothing ? C_NULL : pointer(rt),
+                      kn === nothing ? C_NULL : pointer(kn), Float64(dwell), BACKENDS[backend], Int32(first(gpus)))
+        check(ccall((:hoe_create, libhoe), Cint, (Ref{ModelDesc}, Ref{Ptr{Cvoid}}), d, h))
+    end
+    nTermEff = nTerm + (concomitant === nothing ? 0 : 4)
+    op = HIPExplicitOp(nSam * nCha, prod(grid.matrixSize), false, false, nothing, nothing, nothing,
+                       0, 0, 0, ComplexF32[], ComplexF32[], h[], nSam, nCha, nTermEff, UInt(0))
+    op.prod! = (res, xv) -> begin
+        check(ccall((:hoe_forward, libhoe), Cint, (Ptr{Cvoid}, Ptr{ComplexF32}, Ptr{ComplexF32}, Cint),
+                    handle(op), pointer(xv), pointer(res), flags(xv)), op.handle)
+        res
+    end
+    op.ctprod! = (res, yv) -> begin
+        check(ccall((:hoe_adjoint, libhoe), Cint, (Ptr{Cvoid}, Ptr{ComplexF32}, Ptr{ComplexF32}, Cint),
+                    handle(op), pointer(yv), pointer(res), flags(yv)), op.handle)
+        res
+    end
+    finalizer(close, op)
+    return op
+end
+
+function Base.adjoint(op::HIPExplicitOp)
+    return LinearOperator{ComplexF32}(op.ncol, op.nrow, false, false, op.ctprod!, nothing, op.prod!)
+end
+
+function Base.close(op::HIPExplicitOp)
+    op.handle == C_NULL && return nothing
+    ccall((:hoe_destroy, libhoe), Cvoid, (Ptr{Cvoid},), op.handle)
+    op.handle = C_NULL
+    return nothing
+end
+
+# normalOperator(W ∘ op) supplies WᴴW; normalOperator(op, D) takes the squared diagonal
+# (explicit_gpu_operator.jl:399-416).
+function LinearOperatorCollection.normalOperator(op::HIPExplicitOp, weights=nothing; kwargs...)
+    raw = weights isa LinearOperatorCollection.WeightingOp ? weights.weights : weights
+    D = raw === nothing ? nothing : Vector{Float32}(real.(Array(raw)))
+    if D !== nothing
+        all(isfinite, D) || throw(ArgumentError("normal weights must be finite"))
+        minimum(D) >= 0 || throw(ArgumentError("normal weights must be nonnegative"))
+        length(D) in (op.nSam, op.nrow) || throw(DimensionMismatch("normal weights"))
+    end
+    id = objectid(D)
+    normal = (res, xv) -> begin
+        h = handle(op)
+        if op.weights_id != id
+            GC.@preserve D check(ccall((:hoe_set_weights, libhoe), Cint, (Ptr{Cvoid}, Ptr{Float32}, Int64), h,
+                                       D === nothing ? C_NULL : pointer(D), D === nothing ? 0 : length(D)), h)
+            op.weights_id = id
+        end
+        check(ccall((:hoe_normal, libhoe), Cint, (Ptr{Cvoid}, Ptr{Float32}, Int64, Ptr{ComplexF32}, Ptr{ComplexF32}, Cint),
+                    h, C_NULL, HOE_CURRENT_WEIGHTS, pointer(xv), pointer(res), flags(xv)), h)
+        res
+    end
+    return LinearOperator{ComplexF32}(op.ncol, op.ncol, false, true, normal, nothing, normal)
+end
+
+"""
+    hip_explicit_loss(op, h, τ, x, y)
+
+`‖E(h, τ) x − y‖²` with coefficient traces `h` `(nTerm_eff, nSam)` and delay `τ`
+(seconds); differentiable through `ChainRulesCore.rrule` with respect to `h` and `τ`.
+"""
+function hip_explicit_loss(op::HIPExplicitOp, h::AbstractMatrix{Float32}, τ::Real, x, y)
+    loss, _, _ = hip_explicit_grad(op, h, τ, x, y)
+    return loss
+end
+
+function hip_explicit_grad(op::HIPExplicitOp, h::AbstractMatrix{Float32}, τ::Real, x, y)
+    H = Matrix{Float32}(h)
+    size(H) == (op.nTerm, op.nSam) || throw(DimensionMismatch("coefficients"))
+    hd = handle(op)
+    GC.@preserve H check(ccall((:hoe_set_coefficients, libhoe), Cint, (Ptr{Cvoid}, Ptr{Float32}, Float64),
+                               hd, pointer(H), Float64(τ)), hd)
+    g = similar(H)
+    gτ, loss = Ref{Float64}(0), Ref{Float64}(0)
+    check(ccall((:hoe_grad, libhoe), Cint,
+                (Ptr{Cvoid}, Ptr{ComplexF32}, Ptr{ComplexF32}, Ptr{Float32}, Ref{Float64}, Ref{Float64}, Cint),
+                hd, pointer(x), pointer(y), pointer(g), gτ, loss, flags(x)), hd)
+    return loss[], g, gτ[]
+end
+
+function ChainRulesCore.rrule(::typeof(hip_explicit_loss), op::HIPExplicitOp, h, τ, x, y)
+    loss, g, gτ = hip_explicit_grad(op, h, τ, x, y)
+    pullback(Δ) = (NoTangent(), NoTangent(), Δ .* g, Δ * gτ, NoTangent(), NoTangent())
+    return loss, pullback
+end
+
+end # module

@@ -53,7 +53,14 @@ def lib():
 
 
 def _p(a):
+    """Address of a contiguous array. The caller must hold a reference to ``a``
+    for the duration of the C call (never pass a temporary)."""
+    assert a.flags["C_CONTIGUOUS"]
     return a.ctypes.data
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
 
 
 # ---------------------------------------------------------------- model prep
@@ -185,8 +192,8 @@ def forward_f64(m: Model, x_full):
     xm = np.ascontiguousarray(gather(m, x_full).astype(np.complex128))
     cs = np.ascontiguousarray(m.csm.astype(np.complex128))
     y = np.empty(m.nc * m.ns, dtype=np.complex128)
-    lib().oracle_forward_f64(m.nv, m.ns, m.nc, m.L, _p(m.bf64), _p(m.k64), _p(m.times.astype(np.float64)),
-                             _p(m.fieldmap.astype(np.float64)), _p(cs), _p(xm), _p(y))
+    t, fm = _f64(m.times), _f64(m.fieldmap)      # keep the temporaries alive during the call
+    lib().oracle_forward_f64(m.nv, m.ns, m.nc, m.L, _p(m.bf64), _p(m.k64), _p(t), _p(fm), _p(cs), _p(xm), _p(y))
     return y
 
 
@@ -194,8 +201,8 @@ def adjoint_f64(m: Model, y):
     ys = np.ascontiguousarray(np.asarray(y).reshape(-1).astype(np.complex128))
     cs = np.ascontiguousarray(m.csm.astype(np.complex128))
     xm = np.empty(m.nv, dtype=np.complex128)
-    lib().oracle_adjoint_f64(m.nv, m.ns, m.nc, m.L, _p(m.bf64), _p(m.k64), _p(m.times.astype(np.float64)),
-                             _p(m.fieldmap.astype(np.float64)), _p(cs), _p(ys), _p(xm))
+    t, fm = _f64(m.times), _f64(m.fieldmap)
+    lib().oracle_adjoint_f64(m.nv, m.ns, m.nc, m.L, _p(m.bf64), _p(m.k64), _p(t), _p(fm), _p(cs), _p(ys), _p(xm))
     return scatter(m, xm)
 
 
@@ -218,8 +225,9 @@ def grad_f64(m: Model, x_full, y):
     cs = np.ascontiguousarray(m.csm.astype(np.complex128))
     rr = np.ascontiguousarray(r.astype(np.complex128))
     g = np.empty((m.L, m.ns), dtype=np.float64)
-    lib().oracle_grad_f64(m.nv, m.ns, m.nc, m.L, _p(m.bf64), _p(m.k64), _p(m.times.astype(np.float64)),
-                          _p(m.fieldmap.astype(np.float64)), _p(cs), _p(xm), _p(rr), _p(g))
+    t, fm = _f64(m.times), _f64(m.fieldmap)
+    lib().oracle_grad_f64(m.nv, m.ns, m.nc, m.L, _p(m.bf64), _p(m.k64), _p(t), _p(fm), _p(cs), _p(xm), _p(rr),
+                          _p(g))
     return g, float(np.vdot(r, r).real), r
 
 
@@ -238,16 +246,18 @@ def dense_matrix_f64(m: Model):
 def cpu_ref_forward(m: Model, x_full, nblock=50):
     xm = np.ascontiguousarray(gather(m, x_full).astype(np.complex64))
     y = np.empty(m.nc * m.ns, dtype=np.complex64)
+    cs = np.ascontiguousarray(m.csm)
     lib().cpu_ref_forward(m.nv, m.ns, m.nc, m.L, _p(m.bf32), _p(m.k32), _p(m.times), _p(m.fieldmap),
-                          _p(np.ascontiguousarray(m.csm)), _p(xm), _p(y), nblock)
+                          _p(cs), _p(xm), _p(y), nblock)
     return y
 
 
 def cpu_ref_adjoint(m: Model, y, nblock=50):
     ys = np.ascontiguousarray(np.asarray(y).reshape(-1).astype(np.complex64))
     xm = np.empty(m.nv, dtype=np.complex64)
+    cs = np.ascontiguousarray(m.csm)
     lib().cpu_ref_adjoint(m.nv, m.ns, m.nc, m.L, _p(m.bf32), _p(m.k32), _p(m.times), _p(m.fieldmap),
-                          _p(np.ascontiguousarray(m.csm)), _p(ys), _p(xm), nblock)
+                          _p(cs), _p(ys), _p(xm), nblock)
     return scatter(m, xm, np.complex64)
 
 

@@ -1,0 +1,76 @@
+"""GPU CGNR reconstruction (recon_HOOp) against the oracle.
+
+- test_ReconstructionStability.jl:25-76: 3x3 grid, 37 samples, 2 coils,
+  complex random-phase weights, 30 CGNR iterations, lambda = 0, compared
+  with a QR least-squares solve of the weighted dense system (3e-4 F32 bar).
+- Workload 0 (64x64, 8 coils, 6000-sample spiral): 20 iterations against the
+  FP64 textbook CG of the oracle after the same number of iterations.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+import oracle as O  # noqa: E402
+import hoe  # noqa: E402
+from hoe import synth  # noqa: E402
+
+BACKENDS = ["direct", "mixed_fp16", "mixed_bf16"]
+
+
+def _stability_problem():
+    rng = np.random.default_rng(514)
+    ns, nc = 37, 2
+    data = (rng.normal(size=(ns, nc)) + 1j * rng.normal(size=(ns, nc))).astype(np.complex64)
+    weight = ((0.5 + rng.uniform(size=ns)) * np.exp(1j * rng.uniform(size=ns))).astype(np.complex64)
+    grid = hoe.Grid(3, 3, 1, 1.0, 1.0, 1.0)
+    kspha = (rng.normal(size=(9, ns)) * 0.01).astype(np.float32)
+    for j in range(ns):
+        kspha[1, j] = ((j % 3) - 1) / 3
+        kspha[2, j] = (((j // 3) % 3) - 1) / 3
+        kspha[3, j] = 0
+    times = np.linspace(0, 1e-3, ns).astype(np.float32)
+    csm = (np.exp(1j * rng.uniform(size=(9, nc))) / np.sqrt(nc)).astype(np.complex64)
+    return grid, kspha, times, csm, data, weight
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_cgnr_matches_qr(backend, gpu_lib):
+    grid, kspha, times, csm, data, weight = _stability_problem()
+    m = O.make_model(grid, kspha, times, None, csm, None)
+    A = O.dense_matrix_f64(m)
+    wr = np.tile(weight.astype(np.complex128), 2)
+    expected = np.linalg.lstsq(A * wr[:, None], data.reshape(-1, order="F") * wr, rcond=None)[0]
+    params = {"solver": hoe.CGNR, "reg": hoe.L2Regularization(0.0), "reconSize": grid.matrixSize,
+              "iterations": 30, "relTol": 1e-10}
+    with hoe.HighOrderExplicitOp(grid, kspha, times, csm=csm, backend=backend) as op:
+        img, log = hoe.recon_HOOp(op, data, weight, params, return_log=True)
+    assert img.shape == grid.matrixSize
+    assert img.dtype == np.complex64
+    err = O.relerr(img.reshape(-1, order="F"), expected)
+    print(f"CGNR vs QR {backend}: {err:.2e}, residual log {log[:4]} ... {log[-1]:.2e}")
+    assert err < 3e-4
+    assert log[0] == 1.0
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_cg_workload0_matches_fp64_cg(backend, gpu_lib):
+    c = synth.make(0)
+    m = O.make_model(c["grid"], c["kspha"], c["times"], c["fieldmap"], c["csm"], None)
+    y = O.forward_f64(m, c["image"])
+    rng = np.random.default_rng(11)
+    y = (y + 0.01 * np.linalg.norm(y) / np.sqrt(y.size) * (rng.normal(size=y.size) + 1j * rng.normal(size=y.size)))
+    data = y.astype(np.complex64).reshape(m.nc, m.ns).T          # (nSam, nCha)
+    weight = np.ones(m.ns, np.complex64)
+    iters, lam = 20, 1e-3
+    rhs = O.adjoint_f64(m, y.astype(np.complex64).astype(np.complex128))
+    x64, log64 = O.cgnr(lambda v: O.normal_f64(m, v), rhs, iters, lam)
+    params = {"solver": hoe.CGNR, "reg": hoe.L2Regularization(lam), "reconSize": c["grid"].matrixSize,
+              "iterations": iters}
+    with hoe.HighOrderExplicitOp(c["grid"], c["kspha"], c["times"], fieldmap=c["fieldmap"], csm=c["csm"],
+                                 backend=backend) as op:
+        img, log = hoe.recon_HOOp(op, data, weight, params, return_log=True)
+    err = O.relerr(img.reshape(-1, order="F"), x64)
+    print(f"workload0 CG {backend}: image vs FP64 CG {err:.2e}; residual {log[-1]:.3e} vs {log64[-1]:.3e}")
+    assert err < 1e-3
+    assert abs(log[-1] - log64[-1]) <= 0.05 * log64[-1] + 1e-6
