@@ -90,6 +90,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-grad", action="store_true")
+    ap.add_argument("--no-direct", action="store_true")
     ap.add_argument("--cpu-samples", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -118,10 +120,13 @@ def main():
     t_mixed, kf_m, ka_m = time_normal(op, x_dev, out_mixed, args.steps, args.warmup)
     log(f"mixed: {args.steps} E^H E in {t_mixed:.3f} s")
     # ---- plain FP32 direct ----
-    op.set_backend("direct")
     steps_d = max(3, args.steps // 5)
-    t_direct, kf_d, ka_d = time_normal(op, x_dev, out_direct, steps_d, max(1, args.warmup // 2))
-    log(f"direct: {steps_d} E^H E in {t_direct:.3f} s")
+    if args.no_direct:
+        t_direct, kf_d, ka_d = float("nan"), [float("nan")], [float("nan")]
+    else:
+        op.set_backend("direct")
+        t_direct, kf_d, ka_d = time_normal(op, x_dev, out_direct, steps_d, max(1, args.warmup // 2))
+        log(f"direct: {steps_d} E^H E in {t_direct:.3f} s")
     # ---- accuracy: mixed vs direct (full normal), forward vs FP64 on a sample subset ----
     torch.cuda.synchronize()
     nm = out_mixed.cpu().numpy()
@@ -139,23 +144,25 @@ def main():
             fwd_err[be] = O.relerr(so * cfg["image"], y64)
     log(f"errors: normal mixed vs direct {err_mixed_vs_direct:.2e}, fwd vs fp64 {fwd_err}")
     # ---- gradient (workload 3 convention: y from perturbed coefficients and a 3 us delay) ----
-    op.set_backend("direct")
-    kp = synth.perturb(cfg)
-    gop = make_op(dict(cfg, kspha=kp), "direct")
-    gop.set_coefficients(None, 3e-6) if cfg["dwell"] > 0 else None
-    y_obs = gop * x_dev
-    gop.close()
-    torch.cuda.synchronize()
-    gts = []
-    for i in range(4):
-        t0 = time.perf_counter()
-        g, gtau, loss = op.grad(x_dev, y_obs)
+    grad_ms = grad_kernel_ms = float("nan")
+    if not args.no_grad:
+        op.set_backend("direct")
+        kp = synth.perturb(cfg)
+        gop = make_op(dict(cfg, kspha=kp), "direct")
+        gop.set_coefficients(None, 3e-6)
+        y_obs = gop * x_dev
+        gop.close()
         torch.cuda.synchronize()
-        if i > 0:
-            gts.append(time.perf_counter() - t0)
-    grad_ms = 1e3 * float(np.median(gts))
-    grad_kernel_ms = op.last_kernel_ms(0)
-    log(f"grad: {grad_ms:.2f} ms (kernel {grad_kernel_ms:.2f} ms), loss {loss:.4e}, dL/dtau {gtau:.4e}")
+        gts = []
+        for i in range(4):
+            t0 = time.perf_counter()
+            g, gtau, loss = op.grad(x_dev, y_obs)
+            torch.cuda.synchronize()
+            if i > 0:
+                gts.append(time.perf_counter() - t0)
+        grad_ms = 1e3 * float(np.median(gts))
+        grad_kernel_ms = op.last_kernel_ms(0)
+        log(f"grad: {grad_ms:.2f} ms (kernel {grad_kernel_ms:.2f} ms), loss {loss:.4e}, dL/dtau {gtau:.4e}")
     op.close()
 
     value = args.steps / t_mixed

@@ -103,38 +103,39 @@ void catmull_rom(double f, double w[4], double dw[4]) {
     dw[3] = 0.5 * (3.0 * f2 - 2.0 * f);
 }
 
-// Effective coefficient records [k_1..k_L, t] per sample (FP64) from the
-// traces, delayed by tau (shift u = tau/dwell samples).
-void build_coef_records(const Handle *h, std::vector<double> &rec) {
+// Effective coefficient records per sample (FP64) from the traces, delayed by
+// tau (shift u = tau/dwell samples): [k_1, k_{a_1}, ..., k_{a_m}, t, 0...]
+// where a_1..a_m are the active (not identically zero) basis terms.
+double trace_at(const Handle *h, int64_t s, int l, bool shift, int64_t m, const double w[4]) {
     const int64_t ns = h->n_sam;
     const int L = h->L;
+    if (!shift) return (double)h->traces[(size_t)s * L + l];
+    double acc = 0.0;
+    for (int t = 0; t < 4; ++t) {
+        int64_t j = s + m - 1 + t;
+        j = std::min<int64_t>(std::max<int64_t>(j, 0), ns - 1);
+        acc += w[t] * (double)h->traces[(size_t)j * L + l];
+    }
+    return acc;
+}
+
+void build_coef_records(const Handle *h, std::vector<double> &rec) {
+    const int64_t ns = h->n_sam;
     rec.assign((size_t)h->ns_pad * h->sk, 0.0);
     const bool shift = h->tau != 0.0;
-    double u = 0.0, f = 0.0;
     int64_t m = 0;
-    double w[4], dw[4];
+    double w[4] = {0, 1, 0, 0}, dw[4];
     if (shift) {
-        u = h->tau / h->dwell;
+        const double u = h->tau / h->dwell;
         m = (int64_t)std::floor(u);
-        f = u - (double)m;
-        catmull_rom(f, w, dw);
+        catmull_rom(u - (double)m, w, dw);
     }
+    const int na = (int)h->active.size();
     for (int64_t s = 0; s < ns; ++s) {
         double *r = &rec[(size_t)s * h->sk];
-        for (int l = 0; l < L; ++l) {
-            if (!shift) {
-                r[l] = (double)h->traces[(size_t)s * L + l];
-            } else {
-                double acc = 0.0;
-                for (int t = 0; t < 4; ++t) {
-                    int64_t j = s + m - 1 + t;
-                    j = std::min<int64_t>(std::max<int64_t>(j, 0), ns - 1);
-                    acc += w[t] * (double)h->traces[(size_t)j * L + l];
-                }
-                r[l] = acc;
-            }
-        }
-        r[L] = (double)h->times[s];
+        r[0] = trace_at(h, s, 0, shift, m, w);
+        for (int i = 0; i < na; ++i) r[1 + i] = trace_at(h, s, h->active[i], shift, m, w);
+        r[1 + na] = (double)h->times[s];
     }
 }
 
@@ -193,6 +194,15 @@ void release(Handle *h) {
                               &h->io_x,     &h->io_x2, &h->io_y,  &h->io_y2,  &h->scal,   &h->cg_vec, &h->red,
                               &h->gk,       &h->cg_log};
     for (auto *b : bufs) dfree(*b);
+    hoe::Sched *scheds[] = {&h->sched_f[0], &h->sched_f[1], &h->sched_f[2], &h->sched_a[0],
+                            &h->sched_a[1], &h->sched_a[2], &h->sched_g};
+    for (auto *sc : scheds) {
+        dfree(sc->seg);
+        dfree(sc->chunk_off);
+        dfree(sc->tile_off);
+        dfree(sc->tile_list);
+        sc->built = false;
+    }
     for (auto &e : h->ev)
         if (e) {
             (void)hipEventDestroy(e);
@@ -276,8 +286,6 @@ int hoe_create(const hoe_model_desc *d, hoe_handle **out) {
     // coil stride: one 32-coil MFMA group, or whole pairs of groups (the mixed kernels keep two
     // groups per wave when there are more than 32 coils)
     h->ncs = (int32_t)(h->n_cha <= 32 ? 32 : round_up(h->n_cha, 64));
-    h->sb = (int32_t)round_up(h->L, 8);
-    h->sk = (int32_t)round_up(h->L + 1, 8);
     h->backend = d->backend;
     h->device = d->device;
     h->dwell = d->dwell;
@@ -302,25 +310,51 @@ int hoe_create(const hoe_model_desc *d, hoe_handle **out) {
     }
     h->times.assign(d->times, d->times + ns);
 
-    // ---- voxel records [b_2..b_L, fieldmap] in FP64 ----
+    // ---- basis columns in FP64; identically zero columns (z terms on a z = 0
+    // plane, ...) contribute nothing and are dropped from the kernel records ----
     const int64_t nv = h->n_vox;
-    std::vector<double> brec((size_t)h->nv_pad * h->sb, 0.0);
-    float csm_max = 0.0f;
-    std::vector<float2> csmv((size_t)h->nv_pad * h->ncs, make_float2(0.0f, 0.0f));
+    std::vector<double> bfull((size_t)nv * L, 0.0);   // [v][l], column 0 = 1
+    std::vector<char> nonzero(L, 0);
     for (int64_t v = 0; v < nv; ++v) {
         const int64_t g = idx[(size_t)v];
         const double x = d->x[g], y = d->y[g], z = d->z[g];
         double sh[15];
         solid_harmonics(x, y, z, sh);
-        double *r = &brec[(size_t)v * h->sb];
-        for (int l = 1; l < n_term; ++l) r[l - 1] = sh[l - 1];
+        double *r = &bfull[(size_t)v * L];
+        r[0] = 1.0;
+        for (int l = 1; l < n_term; ++l) r[l] = sh[l - 1];
         if (d->concomitant) {
-            r[n_term - 1 + 0] = z * z;
-            r[n_term - 1 + 1] = x * x + y * y;
-            r[n_term - 1 + 2] = x * z;
-            r[n_term - 1 + 3] = y * z;
+            r[n_term + 0] = z * z;
+            r[n_term + 1] = x * x + y * y;
+            r[n_term + 2] = x * z;
+            r[n_term + 3] = y * z;
         }
-        r[L - 1] = d->fieldmap ? (double)d->fieldmap[g] : 0.0;
+        for (int l = 1; l < L; ++l)
+            if (r[l] != 0.0) nonzero[l] = 1;
+    }
+    h->active.clear();
+    for (int l = 1; l < L; ++l)
+        if (nonzero[l]) h->active.push_back(l);
+    const int need = (int)h->active.size() + 1;     // active terms + fieldmap
+    static const int kBuckets[] = {6, 10, 12, 16, 20};
+    h->LK = 20;
+    for (int b : kBuckets)
+        if (need <= b) {
+            h->LK = b;
+            break;
+        }
+    h->sb = (int32_t)round_up(h->LK, 8);
+    h->sk = (int32_t)round_up(h->LK + 1, 8);
+    // voxel records [b_{a_1}..b_{a_m}, fieldmap, 0...]
+    std::vector<double> brec((size_t)h->nv_pad * h->sb, 0.0);
+    float csm_max = 0.0f;
+    std::vector<float2> csmv((size_t)h->nv_pad * h->ncs, make_float2(0.0f, 0.0f));
+    const int na = (int)h->active.size();
+    for (int64_t v = 0; v < nv; ++v) {
+        const int64_t g = idx[(size_t)v];
+        double *r = &brec[(size_t)v * h->sb];
+        for (int i = 0; i < na; ++i) r[i] = bfull[(size_t)v * L + h->active[i]];
+        r[na] = d->fieldmap ? (double)d->fieldmap[g] : 0.0;
         for (int64_t c = 0; c < h->n_cha; ++c) {
             float2 val = make_float2(1.0f, 0.0f);
             if (d->csm) val = make_float2(d->csm[2 * (c * n_grid + g)], d->csm[2 * (c * n_grid + g) + 1]);
@@ -330,32 +364,9 @@ int hoe_create(const hoe_model_desc *d, hoe_handle **out) {
     }
     h->csm_max = csm_max;
 
-    // ---- work decomposition: ~4 waves per SIMD per contraction launch ----
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, h->device) == hipSuccess && prop.multiProcessorCount > 0)
         h->cu_count = prop.multiProcessorCount;
-    const int64_t target_wg = (int64_t)h->cu_count * 4;   // 256-thread workgroups
-    {
-        const int64_t tiles = h->ns_pad / 256;
-        int64_t splits = std::max<int64_t>(1, (target_wg + tiles - 1) / tiles);
-        splits = std::min<int64_t>(splits, h->nv_pad / 64);
-        h->vox_per_split = (int)round_up((h->nv_pad + splits - 1) / splits, 64);
-        h->splits_f = (int)((h->nv_pad + h->vox_per_split - 1) / h->vox_per_split);
-    }
-    {
-        const int64_t tiles = h->nv_pad / 256;
-        int64_t splits = std::max<int64_t>(1, (target_wg + tiles - 1) / tiles);
-        splits = std::min<int64_t>(splits, h->ns_pad / 64);
-        h->sam_per_split = (int)round_up((h->ns_pad + splits - 1) / splits, 64);
-        h->splits_a = (int)((h->ns_pad + h->sam_per_split - 1) / h->sam_per_split);
-    }
-    {
-        const int64_t tiles = h->ns_pad / 256;
-        int64_t splits = std::max<int64_t>(1, (target_wg + tiles - 1) / tiles);
-        splits = std::min<int64_t>(splits, h->nv_pad / 64);
-        h->vox_per_split_g = (int)round_up((h->nv_pad + splits - 1) / splits, 64);
-        h->splits_g = (int)((h->nv_pad + h->vox_per_split_g - 1) / h->vox_per_split_g);
-    }
 
     // ---- device allocation ----
     auto fail = [&](int code) {
@@ -371,8 +382,6 @@ int hoe_create(const hoe_model_desc *d, hoe_handle **out) {
         return fail(set_err(nullptr, HOE_EHIP, "hipStreamCreate failed"));
     for (auto &ev : h->ev)
         if ((e = hipEventCreate(&ev)) != hipSuccess) return fail(set_err(nullptr, HOE_EHIP, "hipEventCreate failed"));
-    const int cbn = (h->n_cha <= 8) ? 8 : (h->n_cha <= 16 ? 16 : 32);
-    const int64_t n_batches_g = (h->n_cha + cbn - 1) / cbn;
     const int n_gb = h->ncs / 32;
     const int nbw = n_gb >= 2 ? 2 : 1;
     const int64_t n_gb_pad = ((n_gb + nbw - 1) / nbw) * nbw;
@@ -389,9 +398,6 @@ int hoe_create(const hoe_model_desc *d, hoe_handle **out) {
         {&h->zfrag, (size_t)n_gb_pad * (size_t)(h->nv_pad / 8) * 64 * 64},
         {&h->ybuf, sizeof(float2) * (size_t)h->ns_pad * h->ncs},
         {&h->yfrag, (size_t)n_gb_pad * (size_t)(h->ns_pad / 8) * 64 * 64},
-        {&h->part_f, sizeof(float2) * (size_t)h->splits_f * h->ns_pad * h->ncs},
-        {&h->part_a, sizeof(float2) * (size_t)h->splits_a * h->nv_pad * h->ncs},
-        {&h->part_g, sizeof(double) * (size_t)h->splits_g * n_batches_g * h->ns_pad * L},
         {&h->dvec, sizeof(float) * (size_t)h->ns_pad * h->ncs},
         {&h->io_x, sizeof(float2) * (size_t)n_grid},
         {&h->io_x2, sizeof(float2) * (size_t)n_grid},
@@ -400,7 +406,7 @@ int hoe_create(const hoe_model_desc *d, hoe_handle **out) {
         {&h->scal, 4096},
         {&h->cg_vec, sizeof(float2) * (size_t)h->nv_pad * 4},
         {&h->red, sizeof(double) * 4096},
-        {&h->gk, sizeof(double) * (size_t)L * ns},
+        {&h->gk, sizeof(double) * (size_t)h->LK * ns},
     };
     for (auto &p : plan) {
         e = dalloc(*p.b, p.bytes);
@@ -412,6 +418,11 @@ int hoe_create(const hoe_model_desc *d, hoe_handle **out) {
     if (e == hipSuccess) e = hipMemcpy(h->basis.p, brec.data(), sizeof(double) * brec.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(h->csm.p, csmv.data(), sizeof(float2) * csmv.size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) return fail(set_err(nullptr, HOE_EHIP, std::string("upload failed: ") + hipGetErrorString(e)));
+    e = hoe::prepare_backend(h, h->backend);
+    if (e == hipSuccess) e = hoe::prepare_grad(h);
+    if (e != hipSuccess)
+        return fail(set_err(nullptr, e == hipErrorOutOfMemory ? HOE_ENOMEM : HOE_EHIP,
+                            std::string("schedule setup failed: ") + hipGetErrorString(e)));
     int rc = upload_coef(h);
     if (rc != HOE_OK) return fail(set_err(nullptr, rc, h->err));
     h->weights_unit = false;
@@ -441,6 +452,8 @@ int hoe_set_backend(hoe_handle *hh, int32_t backend) {
     if (rc) return rc;
     if (backend < HOE_BACKEND_DIRECT || backend > HOE_BACKEND_MIXED_BF16)
         return set_err(h, HOE_EINVAL, "unknown explicit backend");
+    HOE_TRY_HIP(h, hipStreamSynchronize(h->stream));
+    HOE_TRY_HIP(h, hoe::prepare_backend(h, backend));
     h->backend = backend;
     return HOE_OK;
 }
@@ -572,15 +585,22 @@ int hoe_grad(hoe_handle *hh, const void *x, const void *y, float *g_h, double *g
     (void)hipEventRecord(h->ev[5], h->stream);
     const int L = h->L;
     const int64_t ns = h->n_sam;
-    std::vector<double> gk((size_t)L * ns);
+    std::vector<double> gkc((size_t)h->LK * ns);      // kernel rows: term 1, active terms, ...
     double lossv = 0.0;
-    HOE_TRY_HIP(h, hipMemcpyAsync(gk.data(), h->gk.p, sizeof(double) * gk.size(), hipMemcpyDeviceToHost, h->stream));
+    HOE_TRY_HIP(h, hipMemcpyAsync(gkc.data(), h->gk.p, sizeof(double) * gkc.size(), hipMemcpyDeviceToHost,
+                                  h->stream));
     HOE_TRY_HIP(h, hipMemcpyAsync(&lossv, loss_dev, sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HOE_TRY_HIP(h, hipStreamSynchronize(h->stream));
     h->last_ms[0] = ev_ms(h->ev[0], h->ev[1]);
     h->last_ms[1] = -1.0;
     h->last_ms[2] = ev_ms(h->ev[4], h->ev[5]);
     if (loss) *loss = lossv;
+    // dL/dk for every original term (inactive terms have an identically zero basis)
+    std::vector<double> gk((size_t)L * ns, 0.0);
+    std::copy(gkc.begin(), gkc.begin() + ns, gk.begin());
+    for (size_t i = 0; i < h->active.size(); ++i)
+        std::copy(gkc.begin() + (int64_t)(1 + i) * ns, gkc.begin() + (int64_t)(2 + i) * ns,
+                  gk.begin() + (int64_t)h->active[i] * ns);
     // chain rule through the delay interpolation (host, FP64)
     const bool shift = h->tau != 0.0;
     double w[4] = {0, 1, 0, 0}, dw[4] = {0, 0, 0, 0};

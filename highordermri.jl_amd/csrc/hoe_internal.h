@@ -23,15 +23,31 @@ struct DeviceBuf {
     size_t bytes = 0;
 };
 
+// Stream-K schedule of one contraction (see build_schedule in hoe_kernels.hip).
+struct Sched {
+    bool built = false;
+    int tiles = 0;      // 256-row tiles of the stationary index
+    int n_ks = 0;       // K units (8 streamed rows each)
+    int cpx = 0;        // chunks per XCD part
+    int n_chunks = 0;   // workgroups launched (8 * cpx)
+    int n_seg = 0;      // segments = partial-sum blocks
+    DeviceBuf seg;        // int4 [n_seg]: tile, k0, k1, 0
+    DeviceBuf chunk_off;  // int [n_chunks + 1]: first segment of each chunk
+    DeviceBuf tile_off;   // int [tiles + 1]: CSR offsets into tile_list
+    DeviceBuf tile_list;  // int [n_seg]: segments of each tile, in segment order
+};
+
 struct Handle {
     // ---- model sizes ----
     int64_t n_grid = 0, n_vox = 0, n_sam = 0, n_cha = 0;
     int64_t nv_pad = 0, ns_pad = 0;   // padded to kRowPad
     int32_t L = 0;                    // effective terms (b_1 = 1 included)
+    int32_t LK = 0;                   // kernel record length: active terms + fieldmap, padded to a bucket
+    std::vector<int> active;          // 0-based term indices (>= 1) with a not identically zero basis
     int32_t n_term_in = 0;            // 9 or 16
     int32_t ncs = 0;                  // coil stride of [row][coil] buffers: 32*ceil(n_cha/32)
-    int32_t sb = 0;                   // doubles per voxel record: [b_2..b_L, fieldmap]
-    int32_t sk = 0;                   // doubles per sample record: [k_1..k_L, t]
+    int32_t sb = 0;                   // doubles per voxel record (>= LK)
+    int32_t sk = 0;                   // doubles per sample record (>= LK + 1)
     int32_t backend = HOE_BACKEND_DIRECT;
     int32_t device = 0;
     int32_t cu_count = 256;
@@ -45,14 +61,13 @@ struct Handle {
     std::vector<float> traces;        // L x n_sam, term fastest (prepared coefficients)
     std::vector<float> times;         // n_sam
     std::vector<int32_t> mask_idx_h;  // n_vox
-    std::vector<double> basis_h;      // n_vox x L: [1, b_2..b_L] used for gradient scaling on host
 
     // ---- device model ----
     hipStream_t stream = nullptr;
     hipEvent_t ev[6] = {};
     DeviceBuf mask_idx;   // int32 [n_vox]
-    DeviceBuf basis;      // double [nv_pad][sb]
-    DeviceBuf coef;       // double [ns_pad][sk]
+    DeviceBuf basis;      // double [nv_pad][sb]: b of the active terms, fieldmap
+    DeviceBuf coef;       // double [ns_pad][sk]: k_1, k of the active terms, t
     DeviceBuf dcoef;      // double [ns_pad][sk] d k / d tau (for the delay gradient)
     DeviceBuf csm;        // float2 [nv_pad][ncs]
 
@@ -62,9 +77,9 @@ struct Handle {
     DeviceBuf zfrag;      // MFMA B fragments of csm .* x (mixed path)
     DeviceBuf ybuf;       // float2 [ns_pad][ncs]   weighted signal, coil fastest
     DeviceBuf yfrag;      // MFMA B fragments of the weighted signal (mixed path)
-    DeviceBuf part_f;     // float2 [splits_f][ns_pad][ncs]
-    DeviceBuf part_a;     // float2 [splits_a][nv_pad][ncs]
-    DeviceBuf part_g;     // double [splits_g][ns_pad][L] gradient partials
+    DeviceBuf part_f;     // float2 [n_seg][256][ncs]   forward partial sums
+    DeviceBuf part_a;     // float2 [n_seg][256][ncs]   adjoint partial sums
+    DeviceBuf part_g;     // double [n_seg][batches][256][L] gradient partials
     DeviceBuf dvec;       // float  [ns_pad][ncs] normal weights (coil fastest)
     DeviceBuf io_x;       // float2 [n_grid]  staging for host vectors
     DeviceBuf io_x2;      // float2 [n_grid]
@@ -76,12 +91,23 @@ struct Handle {
     DeviceBuf gk;         // double [L][n_sam] dL/dk reduced
     DeviceBuf cg_log;     // double [iters+1] CG relative residuals
 
-    int splits_f = 1, splits_a = 1, splits_g = 1;
-    int vox_per_split = 0, sam_per_split = 0, vox_per_split_g = 0;
+    // schedules per backend (index = hoe_backend) and for the gradient kernel
+    Sched sched_f[3], sched_a[3], sched_g;
+    int occ_f[3] = {0, 0, 0}, occ_a[3] = {0, 0, 0};   // resident workgroups per CU
 
     // timing of the last call (ms)
     double last_ms[3] = {-1, -1, -1};
 };
+
+// ---- coil batching and schedules (hoe_kernels.hip) ----
+int direct_cb(const Handle *h);        // coils per direct-kernel batch: 8, 16 or 32
+int direct_batches(const Handle *h);
+int mixed_nb(const Handle *h);         // 32-coil groups per wave in the MFMA kernels: 1 or 2
+int mixed_groups(const Handle *h);
+hipError_t build_schedule(Sched &sc, int tiles, int n_ks, int slots);
+// Build the forward/adjoint schedules of a backend and size the partial buffers.
+hipError_t prepare_backend(Handle *h, int backend);
+hipError_t prepare_grad(Handle *h);
 
 // ---- launchers (hoe_kernels.hip) ----
 // All run on h->stream and return hipSuccess or the first HIP error.

@@ -33,23 +33,16 @@ typedef float f16x_t __attribute__((ext_vector_type(16)));
 // helpers
 // ---------------------------------------------------------------------------
 
-// Map blockIdx.x to (tile, split): blocks g = xcd*chunk + j, split-major, so
-// that consecutive tiles of one split share an XCD.
-__device__ __forceinline__ bool map_block(int n_tiles, int n_splits, int &tile, int &split) {
-    const int total = n_tiles * n_splits;
-    const int chunk = (total + 7) >> 3;
-    const int id = blockIdx.x;
-    const int g = (id & 7) * chunk + (id >> 3);
-    if (g >= total) return false;
-    split = g / n_tiles;
-    tile = g - split * n_tiles;
-    return true;
-}
-
-static inline int grid_blocks(int n_tiles, int n_splits) {
-    const int total = n_tiles * n_splits;
-    return 8 * ((total + 7) / 8);
-}
+// Stream-K work schedule (built on the host, see build_schedule): the
+// flattened (row tile, K step) space of one contraction is cut into equal
+// chunks, one per resident workgroup slot, so every workgroup does the same
+// number of K steps and the whole grid runs in a single wave of residency.
+// A chunk covers one or more segments (tile, k0, k1); every segment writes
+// its own partial-sum block, reduced later in segment order (deterministic).
+// The K range is first cut into 8 parts and chunk c of part x is launched as
+// block 8*c + x: blocks are dealt round-robin to the 8 XCDs, so each XCD
+// streams one eighth of the K data and keeps it in its own L2.
+__device__ __forceinline__ int chunk_id(int cpx) { return (blockIdx.x & 7) * cpx + (blockIdx.x >> 3); }
 
 // Fractional part of the phase in cycles (v_fract_f64: exact, in [0, 1)),
 // then sin/cos(2*pi*r) from v_sin_f32 / v_cos_f32 (inputs in revolutions;
@@ -119,63 +112,63 @@ struct mfma_t<__bf16> {
 // direct FP32 kernels (VALU contraction)
 // ---------------------------------------------------------------------------
 //
-// FWD:  part[split][s][c] = sum_{v in split} e(s,v) * z[v][c]
-// !FWD: part[split][v][c] = sum_{s in split} conj(e(s,v)) * y[s][c]
+// FWD:  part[seg][r][c] = sum_{v in seg} e(s,v) * z[v][c]          (s = 256*tile + r)
+// !FWD: part[seg][r][c] = sum_{s in seg} conj(e(s,v)) * y[s][c]    (v = 256*tile + r)
 // Each lane keeps CB complex accumulators; the streamed record and the CB
-// complex operands of the streamed row arrive through scalar loads.
+// complex operands of the streamed row arrive through scalar loads. A K unit
+// is 8 streamed rows; rows at or beyond n_stream (padding) are skipped.
 template <bool FWD, int L, int CB>
 __global__ __launch_bounds__(256) void k_direct(
     const double *__restrict__ coef, int sk, const double *__restrict__ basis, int sb,
-    const float2 *__restrict__ src, float2 *__restrict__ part, int rows_pad, int ncs,
-    int n_tiles, int n_splits, int per_split, int n_stream, int n_batches) {
-    int tile, sg;
-    if (!map_block(n_tiles, n_splits * n_batches, tile, sg)) return;
-    const int batch = sg % n_batches;
-    const int split = sg / n_batches;
-    const int cb0 = batch * CB;
-    const int row = tile * 256 + threadIdx.x;
-
+    const float2 *__restrict__ src, float2 *__restrict__ part, int ncs, int n_stream,
+    const int4 *__restrict__ segs, const int *__restrict__ chunk_off, int cpx) {
+    const int chunk = chunk_id(cpx);
+    const int cb0 = blockIdx.y * CB;
+    const int sg0 = chunk_off[chunk], sg1 = chunk_off[chunk + 1];
     constexpr int NST = FWD ? L + 1 : L;
-    double st[NST];
-    {
-        const double *p = FWD ? coef + (size_t)row * sk : basis + (size_t)row * sb;
+    for (int sgi = sg0; sgi < sg1; ++sgi) {
+        const int4 d = segs[sgi];
+        const int row = d.x * 256 + threadIdx.x;
+        double st[NST];
+        {
+            const double *p = FWD ? coef + (size_t)row * sk : basis + (size_t)row * sb;
 #pragma unroll
-        for (int i = 0; i < NST; ++i) st[i] = p[i];
-    }
-    float ar[CB], ai[CB];
-#pragma unroll
-    for (int c = 0; c < CB; ++c) ar[c] = ai[c] = 0.0f;
-
-    const int q0 = split * per_split;
-    const int q1 = min(q0 + per_split, n_stream);
-    for (int q = q0; q < q1; ++q) {
-        const double *sm = FWD ? basis + (size_t)q * sb : coef + (size_t)q * sk;
-        double ph;
-        if (FWD) {
-            ph = st[0];
-#pragma unroll
-            for (int i = 1; i <= L; ++i) ph = fma(st[i], sm[i - 1], ph);
-        } else {
-            ph = sm[0];
-#pragma unroll
-            for (int i = 1; i <= L; ++i) ph = fma(sm[i], st[i - 1], ph);
+            for (int i = 0; i < NST; ++i) st[i] = p[i];
         }
-        float sn, cs;
-        sincos_cycles(ph, sn, cs);
-        if (!FWD) sn = -sn;   // conj(e)
-        const float2 *w = src + (size_t)q * ncs + cb0;
+        float ar[CB], ai[CB];
 #pragma unroll
-        for (int c = 0; c < CB; ++c) {
-            const float2 u = w[c];
-            ar[c] = fmaf(cs, u.x, ar[c]);
-            ar[c] = fmaf(-sn, u.y, ar[c]);
-            ai[c] = fmaf(cs, u.y, ai[c]);
-            ai[c] = fmaf(sn, u.x, ai[c]);
+        for (int c = 0; c < CB; ++c) ar[c] = ai[c] = 0.0f;
+        const int q0 = d.y * 8;
+        const int q1 = min(d.z * 8, n_stream);
+        for (int q = q0; q < q1; ++q) {
+            const double *sm = FWD ? basis + (size_t)q * sb : coef + (size_t)q * sk;
+            double ph;
+            if (FWD) {
+                ph = st[0];
+#pragma unroll
+                for (int i = 1; i <= L; ++i) ph = fma(st[i], sm[i - 1], ph);
+            } else {
+                ph = sm[0];
+#pragma unroll
+                for (int i = 1; i <= L; ++i) ph = fma(sm[i], st[i - 1], ph);
+            }
+            float sn, cs;
+            sincos_cycles(ph, sn, cs);
+            if (!FWD) sn = -sn;   // conj(e)
+            const float2 *w = src + (size_t)q * ncs + cb0;
+#pragma unroll
+            for (int c = 0; c < CB; ++c) {
+                const float2 u = w[c];
+                ar[c] = fmaf(cs, u.x, ar[c]);
+                ar[c] = fmaf(-sn, u.y, ar[c]);
+                ai[c] = fmaf(cs, u.y, ai[c]);
+                ai[c] = fmaf(sn, u.x, ai[c]);
+            }
         }
-    }
-    float2 *o = part + ((size_t)split * rows_pad + row) * ncs + cb0;
+        float2 *o = part + ((size_t)sgi * 256 + threadIdx.x) * ncs + cb0;
 #pragma unroll
-    for (int c = 0; c < CB; ++c) o[c] = make_float2(ar[c], ai[c]);
+        for (int c = 0; c < CB; ++c) o[c] = make_float2(ar[c], ai[c]);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -192,118 +185,114 @@ __global__ __launch_bounds__(256) void k_direct(
 // Three products per K step: Ahi*Bhi + Alo*Bhi + Ahi*Blo, FP32 accumulate.
 // Output columns: tile 0 = real part, tile 1 = imaginary part of 32 coils.
 template <typename T16, bool FWD, int L, int NB>
-__global__ __launch_bounds__(256) void k_mixed(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? 3 : 2))) void k_mixed(
     const double *__restrict__ coef, int sk, const double *__restrict__ basis, int sb,
-    const uint4 *__restrict__ frag, int n_ks, float2 *__restrict__ part, int rows_pad, int ncs,
-    int n_tiles, int n_splits, int ks_per_split, int n_groups) {
+    const uint4 *__restrict__ frag, int n_ks, float2 *__restrict__ part, int ncs,
+    const int4 *__restrict__ segs, const int *__restrict__ chunk_off, int cpx) {
     typedef typename mfma_t<T16>::vec vec_t;
-    int tile, sg;
-    if (!map_block(n_tiles, n_splits * n_groups, tile, sg)) return;
-    const int grp = sg % n_groups;
-    const int split = sg / n_groups;
+    const int chunk = chunk_id(cpx);
+    const int grp = blockIdx.y;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int row = tile * 256 + wave * 64 + lane;
-
+    const int sg0 = chunk_off[chunk], sg1 = chunk_off[chunk + 1];
     constexpr int NST = FWD ? L + 1 : L;
-    double st[NST];
-    {
-        const double *p = FWD ? coef + (size_t)row * sk : basis + (size_t)row * sb;
-#pragma unroll
-        for (int i = 0; i < NST; ++i) st[i] = p[i];
-    }
-
-    f16x_t acc[2][NB][2];
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            acc[a][b][0] = f16x_t{};
-            acc[a][b][1] = f16x_t{};
-        }
-
-    const int ks0 = split * ks_per_split;
-    const int ks1 = min(ks0 + ks_per_split, n_ks);
-    for (int ks = ks0; ks < ks1; ++ks) {
-        uint4 bfr[NB][4];
-#pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            const uint4 *p = frag + ((((size_t)(grp * NB + b) * n_ks + ks) * 64 + lane) << 2);
-#pragma unroll
-            for (int q = 0; q < 4; ++q) bfr[b][q] = p[q];
-        }
-        unsigned hp[8], lp[8];
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const int q = ks * 8 + j;
-            const double *sm = FWD ? basis + (size_t)q * sb : coef + (size_t)q * sk;
-            double ph;
-            if (FWD) {
-                ph = st[0];
-#pragma unroll
-                for (int i = 1; i <= L; ++i) ph = fma(st[i], sm[i - 1], ph);
-            } else {
-                ph = sm[0];
-#pragma unroll
-                for (int i = 1; i <= L; ++i) ph = fma(sm[i], st[i - 1], ph);
-            }
-            float sn, cs;
-            sincos_cycles(ph, sn, cs);
-            split2<T16>(cs, sn, hp[j], lp[j]);
-        }
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            auto a = __builtin_amdgcn_permlane32_swap(hp[j], hp[j + 4], false, false);
-            hp[j] = a[0];
-            hp[j + 4] = a[1];
-            auto b = __builtin_amdgcn_permlane32_swap(lp[j], lp[j + 4], false, false);
-            lp[j] = b[0];
-            lp[j + 4] = b[1];
-        }
-        vec_t ahi[2], alo[2];
+    for (int sgi = sg0; sgi < sg1; ++sgi) {
+        const int4 d = segs[sgi];
+        const int row = d.x * 256 + wave * 64 + lane;
+        double st[NST];
         {
-            uint4 t0 = make_uint4(hp[0], hp[1], hp[2], hp[3]);
-            uint4 t1 = make_uint4(hp[4], hp[5], hp[6], hp[7]);
-            uint4 u0 = make_uint4(lp[0], lp[1], lp[2], lp[3]);
-            uint4 u1 = make_uint4(lp[4], lp[5], lp[6], lp[7]);
-            ahi[0] = __builtin_bit_cast(vec_t, t0);
-            ahi[1] = __builtin_bit_cast(vec_t, t1);
-            alo[0] = __builtin_bit_cast(vec_t, u0);
-            alo[1] = __builtin_bit_cast(vec_t, u1);
+            const double *p = FWD ? coef + (size_t)row * sk : basis + (size_t)row * sb;
+#pragma unroll
+            for (int i = 0; i < NST; ++i) st[i] = p[i];
         }
+        f16x_t acc[2][NB][2];
 #pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            const vec_t bhr = __builtin_bit_cast(vec_t, bfr[b][0]);
-            const vec_t bhi = __builtin_bit_cast(vec_t, bfr[b][1]);
-            const vec_t blr = __builtin_bit_cast(vec_t, bfr[b][2]);
-            const vec_t bli = __builtin_bit_cast(vec_t, bfr[b][3]);
+        for (int a = 0; a < 2; ++a)
 #pragma unroll
-            for (int rt = 0; rt < 2; ++rt) {
-                acc[rt][b][0] = mfma_t<T16>::run(ahi[rt], bhr, acc[rt][b][0]);
-                acc[rt][b][1] = mfma_t<T16>::run(ahi[rt], bhi, acc[rt][b][1]);
-                acc[rt][b][0] = mfma_t<T16>::run(alo[rt], bhr, acc[rt][b][0]);
-                acc[rt][b][1] = mfma_t<T16>::run(alo[rt], bhi, acc[rt][b][1]);
-                acc[rt][b][0] = mfma_t<T16>::run(ahi[rt], blr, acc[rt][b][0]);
-                acc[rt][b][1] = mfma_t<T16>::run(ahi[rt], bli, acc[rt][b][1]);
+            for (int b = 0; b < NB; ++b) {
+                acc[a][b][0] = f16x_t{};
+                acc[a][b][1] = f16x_t{};
+            }
+        for (int ks = d.y; ks < d.z; ++ks) {
+            uint4 bfr[NB][4];
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                const uint4 *p = frag + ((((size_t)(grp * NB + b) * n_ks + ks) * 64 + lane) << 2);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) bfr[b][q] = p[q];
+            }
+            unsigned hp[8], lp[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int q = ks * 8 + j;
+                const double *sm = FWD ? basis + (size_t)q * sb : coef + (size_t)q * sk;
+                double ph;
+                if (FWD) {
+                    ph = st[0];
+#pragma unroll
+                    for (int i = 1; i <= L; ++i) ph = fma(st[i], sm[i - 1], ph);
+                } else {
+                    ph = sm[0];
+#pragma unroll
+                    for (int i = 1; i <= L; ++i) ph = fma(sm[i], st[i - 1], ph);
+                }
+                float sn, cs;
+                sincos_cycles(ph, sn, cs);
+                split2<T16>(cs, sn, hp[j], lp[j]);
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                auto a = __builtin_amdgcn_permlane32_swap(hp[j], hp[j + 4], false, false);
+                hp[j] = a[0];
+                hp[j + 4] = a[1];
+                auto b = __builtin_amdgcn_permlane32_swap(lp[j], lp[j + 4], false, false);
+                lp[j] = b[0];
+                lp[j + 4] = b[1];
+            }
+            vec_t ahi[2], alo[2];
+            {
+                uint4 t0 = make_uint4(hp[0], hp[1], hp[2], hp[3]);
+                uint4 t1 = make_uint4(hp[4], hp[5], hp[6], hp[7]);
+                uint4 u0 = make_uint4(lp[0], lp[1], lp[2], lp[3]);
+                uint4 u1 = make_uint4(lp[4], lp[5], lp[6], lp[7]);
+                ahi[0] = __builtin_bit_cast(vec_t, t0);
+                ahi[1] = __builtin_bit_cast(vec_t, t1);
+                alo[0] = __builtin_bit_cast(vec_t, u0);
+                alo[1] = __builtin_bit_cast(vec_t, u1);
+            }
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                const vec_t bhr = __builtin_bit_cast(vec_t, bfr[b][0]);
+                const vec_t bhi = __builtin_bit_cast(vec_t, bfr[b][1]);
+                const vec_t blr = __builtin_bit_cast(vec_t, bfr[b][2]);
+                const vec_t bli = __builtin_bit_cast(vec_t, bfr[b][3]);
+#pragma unroll
+                for (int rt = 0; rt < 2; ++rt) {
+                    acc[rt][b][0] = mfma_t<T16>::run(ahi[rt], bhr, acc[rt][b][0]);
+                    acc[rt][b][1] = mfma_t<T16>::run(ahi[rt], bhi, acc[rt][b][1]);
+                    acc[rt][b][0] = mfma_t<T16>::run(alo[rt], bhr, acc[rt][b][0]);
+                    acc[rt][b][1] = mfma_t<T16>::run(alo[rt], bhi, acc[rt][b][1]);
+                    acc[rt][b][0] = mfma_t<T16>::run(ahi[rt], blr, acc[rt][b][0]);
+                    acc[rt][b][1] = mfma_t<T16>::run(ahi[rt], bli, acc[rt][b][1]);
+                }
             }
         }
+        // D layout of v_mfma_f32_32x32x*: register r of lane l holds
+        // row 8*(r/4) + 4*(l/32) + r%4, column l%32.
+        const int col = lane & 31;
+        const int hi32 = lane >> 5;
+#pragma unroll
+        for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                const int coil = (grp * NB + b) * 32 + col;
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const int rr = wave * 64 + rt * 32 + 8 * (r >> 2) + 4 * hi32 + (r & 3);
+                    part[((size_t)sgi * 256 + rr) * ncs + coil] = make_float2(acc[rt][b][0][r], acc[rt][b][1][r]);
+                }
+            }
     }
-    // D layout of v_mfma_f32_32x32x*: register r of lane l holds
-    // row 8*(r/4) + 4*(l/32) + r%4, column l%32.
-    const int col = lane & 31;
-    const int hi32 = lane >> 5;
-#pragma unroll
-    for (int rt = 0; rt < 2; ++rt)
-#pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            const int coil = (grp * NB + b) * 32 + col;
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int rr = tile * 256 + wave * 64 + rt * 32 + 8 * (r >> 2) + 4 * hi32 + (r & 3);
-                part[((size_t)split * rows_pad + rr) * ncs + coil] =
-                    make_float2(acc[rt][b][0][r], acc[rt][b][1][r]);
-            }
-        }
 }
 
 // ---------------------------------------------------------------------------
@@ -313,54 +302,56 @@ __global__ __launch_bounds__(256) void k_mixed(
 template <int L, int CB>
 __global__ __launch_bounds__(256) void k_grad(
     const double *__restrict__ coef, int sk, const double *__restrict__ basis, int sb,
-    const float2 *__restrict__ z, const float2 *__restrict__ res, double *__restrict__ part,
-    int ns_pad, int ncs, int n_tiles, int n_splits, int per_split, int nv, int n_batches) {
-    int tile, sg;
-    if (!map_block(n_tiles, n_splits * n_batches, tile, sg)) return;
-    const int batch = sg % n_batches;
-    const int split = sg / n_batches;
+    const float2 *__restrict__ z, const float2 *__restrict__ res, double *__restrict__ part, int ncs, int nv,
+    const int4 *__restrict__ segs, const int *__restrict__ chunk_off, int cpx) {
+    const int chunk = chunk_id(cpx);
+    const int batch = blockIdx.y, n_batches = gridDim.y;
     const int cb0 = batch * CB;
-    const int s = tile * 256 + threadIdx.x;
-    double st[L + 1];
+    const int sg0 = chunk_off[chunk], sg1 = chunk_off[chunk + 1];
+    for (int sgi = sg0; sgi < sg1; ++sgi) {
+        const int4 d = segs[sgi];
+        const int s = d.x * 256 + threadIdx.x;
+        double st[L + 1];
 #pragma unroll
-    for (int i = 0; i <= L; ++i) st[i] = coef[(size_t)s * sk + i];
-    float rr[CB], ri[CB];
-#pragma unroll
-    for (int c = 0; c < CB; ++c) {
-        const float2 t = res[(size_t)s * ncs + cb0 + c];
-        rr[c] = t.x;
-        ri[c] = t.y;
-    }
-    double g[L];
-#pragma unroll
-    for (int l = 0; l < L; ++l) g[l] = 0.0;
-    const int v0 = split * per_split;
-    const int v1 = min(v0 + per_split, nv);
-    for (int v = v0; v < v1; ++v) {
-        const double *sm = basis + (size_t)v * sb;
-        double ph = st[0];
-#pragma unroll
-        for (int i = 1; i <= L; ++i) ph = fma(st[i], sm[i - 1], ph);
-        float sn, cs;
-        sincos_cycles(ph, sn, cs);
-        const float2 *w = z + (size_t)v * ncs + cb0;
-        float ur = 0.0f, ui = 0.0f;
+        for (int i = 0; i <= L; ++i) st[i] = coef[(size_t)s * sk + i];
+        float rr[CB], ri[CB];
 #pragma unroll
         for (int c = 0; c < CB; ++c) {
-            const float2 u = w[c];
-            ur = fmaf(rr[c], u.x, ur);
-            ur = fmaf(ri[c], u.y, ur);
-            ui = fmaf(rr[c], u.y, ui);
-            ui = fmaf(-ri[c], u.x, ui);
+            const float2 t = res[(size_t)s * ncs + cb0 + c];
+            rr[c] = t.x;
+            ri[c] = t.y;
         }
-        const double m = (double)fmaf(cs, ui, sn * ur);
-        g[0] += m;
+        double g[L];
 #pragma unroll
-        for (int l = 1; l < L; ++l) g[l] = fma(m, sm[l - 1], g[l]);
+        for (int l = 0; l < L; ++l) g[l] = 0.0;
+        const int v0 = d.y * 8;
+        const int v1 = min(d.z * 8, nv);
+        for (int v = v0; v < v1; ++v) {
+            const double *sm = basis + (size_t)v * sb;
+            double ph = st[0];
+#pragma unroll
+            for (int i = 1; i <= L; ++i) ph = fma(st[i], sm[i - 1], ph);
+            float sn, cs;
+            sincos_cycles(ph, sn, cs);
+            const float2 *w = z + (size_t)v * ncs + cb0;
+            float ur = 0.0f, ui = 0.0f;
+#pragma unroll
+            for (int c = 0; c < CB; ++c) {
+                const float2 u = w[c];
+                ur = fmaf(rr[c], u.x, ur);
+                ur = fmaf(ri[c], u.y, ur);
+                ui = fmaf(rr[c], u.y, ui);
+                ui = fmaf(-ri[c], u.x, ui);
+            }
+            const double m = (double)fmaf(cs, ui, sn * ur);
+            g[0] += m;
+#pragma unroll
+            for (int l = 1; l < L; ++l) g[l] = fma(m, sm[l - 1], g[l]);
+        }
+        double *o = part + (((size_t)sgi * n_batches + batch) * 256 + threadIdx.x) * L;
+#pragma unroll
+        for (int l = 0; l < L; ++l) o[l] = g[l];
     }
-    double *o = part + ((size_t)sg * ns_pad + s) * L;
-#pragma unroll
-    for (int l = 0; l < L; ++l) o[l] = g[l];
 }
 
 // ---------------------------------------------------------------------------
@@ -491,19 +482,45 @@ __global__ void k_build_yfrag(const float2 *__restrict__ y, uint4 *__restrict__ 
     o[3] = make_uint4(li[0], li[1], li[2], li[3]);
 }
 
-// Forward reduction over splits. Tile of 32 samples x 32 coils per block.
+// Forward reduction over the segments of each sample tile (segment order).
+// Tile of 32 samples x 32 coils per block.
 // mode 0: user layout y_out[c*ns + s] = scale * sum
 // mode 1: ybuf[s][c] = w[s][c] * (scale * sum - y_data[c*ns + s]) (w, y_data optional),
 //         zero for padded rows/coils, optional |.|max into maxslot.
-// The effective scale is scale / pow2_scale(max from scale_slot) when scale_slot
-// is given (undoing the fragment scaling of the mixed path).
+// The effective scale is scale / pow2_scale(max from scale_slot * scale_mul)
+// when scale_slot is given (undoing the fragment scaling of the mixed path).
+__device__ __forceinline__ float2 sum_segments(const float2 *__restrict__ part, const int *__restrict__ list,
+                                               int n, int r, int c, int ncs) {
+    float2 a[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) a[q] = make_float2(0.0f, 0.0f);
+    int i = 0;
+    for (; i + 3 < n; i += 4) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const float2 t = part[((size_t)list[i + q] * 256 + r) * ncs + c];
+            a[q].x += t.x;
+            a[q].y += t.y;
+        }
+    }
+    for (; i < n; ++i) {
+        const float2 t = part[((size_t)list[i] * 256 + r) * ncs + c];
+        a[0].x += t.x;
+        a[0].y += t.y;
+    }
+    return make_float2((a[0].x + a[1].x) + (a[2].x + a[3].x), (a[0].y + a[1].y) + (a[2].y + a[3].y));
+}
+
 __global__ __launch_bounds__(256) void k_reduce_fwd(
-    const float2 *__restrict__ part, int n_splits, int ns_pad, int ncs, int ns, int nc, float scale,
-    const unsigned *__restrict__ scale_slot, float scale_mul, int mode, float2 *__restrict__ out,
-    const float *__restrict__ w, const float2 *__restrict__ y_data, unsigned *maxslot) {
+    const float2 *__restrict__ part, const int *__restrict__ tile_off, const int *__restrict__ tile_list, int ncs,
+    int ns, int nc, float scale, const unsigned *__restrict__ scale_slot, float scale_mul, int mode,
+    float2 *__restrict__ out, const float *__restrict__ w, const float2 *__restrict__ y_data, unsigned *maxslot) {
     __shared__ float2 tileb[32][33];
     const int s0 = blockIdx.x * 32, c0 = blockIdx.y * 32;
     const int tc = threadIdx.x & 31, tr = threadIdx.x >> 5;   // 8 row groups
+    const int tile = s0 >> 8;
+    const int *list = tile_list + tile_off[tile];
+    const int nseg = tile_off[tile + 1] - tile_off[tile];
     float sc = scale;
     if (scale_slot) sc = scale / pow2_scale(__uint_as_float(*scale_slot) * scale_mul);
     if (mode == 1 && y_data) {
@@ -514,15 +531,25 @@ __global__ __launch_bounds__(256) void k_reduce_fwd(
         }
         __syncthreads();
     }
-    float m = 0.0f;
-    for (int k = tr; k < 32; k += 8) {
-        const int s = s0 + k, c = c0 + tc;
-        float2 acc = make_float2(0.0f, 0.0f);
-        for (int p = 0; p < n_splits; ++p) {
-            const float2 t = part[((size_t)p * ns_pad + s) * ncs + c];
-            acc.x += t.x;
-            acc.y += t.y;
+    // four rows per thread, accumulated together so that 4 loads are in flight
+    float2 rows[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) rows[q] = make_float2(0.0f, 0.0f);
+    for (int j = 0; j < nseg; ++j) {
+        const size_t base = (size_t)list[j] * 256;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const float2 t = part[(base + ((s0 + tr + 8 * q) & 255)) * ncs + c0 + tc];
+            rows[q].x += t.x;
+            rows[q].y += t.y;
         }
+    }
+    float m = 0.0f;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int k = tr + 8 * q;
+        const int s = s0 + k, c = c0 + tc;
+        float2 acc = rows[q];
         acc.x *= sc;
         acc.y *= sc;
         if (mode == 0) {
@@ -582,27 +609,25 @@ __global__ __launch_bounds__(256) void k_load_ybuf(const float2 *__restrict__ y,
     if (maxslot) wave_absmax_to(maxslot, m);
 }
 
-// Adjoint reduction: xm_out[v] = sc * sum_c conj(csm[v][c]) * sum_split part[split][v][c]
+// Adjoint reduction: xm_out[v] = sc * sum_c conj(csm[v][c]) * sum_segments part[seg][v%256][c]
 //                               (+ lambda * x_in[v]).
 // Half-wave per voxel: lanes over coils, shuffle reduction over 32 lanes.
 __global__ __launch_bounds__(256) void k_reduce_adj(
-    const float2 *__restrict__ part, int n_splits, int nv_pad, int ncs, const float2 *__restrict__ csm, int nv,
-    float scale, const unsigned *__restrict__ scale_slot, float2 *__restrict__ xm_out, float lambda,
+    const float2 *__restrict__ part, const int *__restrict__ tile_off, const int *__restrict__ tile_list,
+    int nv_pad, int ncs, const float2 *__restrict__ csm, int nv, float scale,
+    const unsigned *__restrict__ scale_slot, float2 *__restrict__ xm_out, float lambda,
     const float2 *__restrict__ x_in) {
-    const int hw = (blockIdx.x * blockDim.x + threadIdx.x) >> 5;   // half-wave id = voxel
+    const int v = (blockIdx.x * blockDim.x + threadIdx.x) >> 5;   // half-wave id = voxel
     const int cl = threadIdx.x & 31;
     float sc = scale;
     if (scale_slot) sc = scale / pow2_scale(__uint_as_float(*scale_slot));
-    const int v = hw;
     float ar = 0.0f, ai = 0.0f;
     if (v < nv_pad) {
+        const int tile = v >> 8;
+        const int *list = tile_list + tile_off[tile];
+        const int nseg = tile_off[tile + 1] - tile_off[tile];
         for (int c = cl; c < ncs; c += 32) {
-            float2 acc = make_float2(0.0f, 0.0f);
-            for (int p = 0; p < n_splits; ++p) {
-                const float2 t = part[((size_t)p * nv_pad + v) * ncs + c];
-                acc.x += t.x;
-                acc.y += t.y;
-            }
+            const float2 acc = sum_segments(part, list, nseg, v & 255, c, ncs);
             const float2 s = csm[(size_t)v * ncs + c];
             ar += s.x * acc.x + s.y * acc.y;   // conj(s) * acc
             ai += s.x * acc.y - s.y * acc.x;
@@ -624,14 +649,18 @@ __global__ __launch_bounds__(256) void k_reduce_adj(
     }
 }
 
-// gk[l][s] = scale * sum_p part[p][s][l]  (FP64)
-__global__ void k_reduce_grad(const double *__restrict__ part, int n_parts, int ns_pad, int L, int ns,
-                              double scale, double *__restrict__ gk) {
+// gk[l][s] = scale * sum_segments sum_batches part[seg][batch][s%256][l]  (FP64)
+__global__ void k_reduce_grad(const double *__restrict__ part, const int *__restrict__ tile_off,
+                              const int *__restrict__ tile_list, int n_batches, int L, int ns, double scale,
+                              double *__restrict__ gk) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (int64_t)ns * L) return;
     const int s = (int)(i / L), l = (int)(i % L);
+    const int tile = s >> 8;
     double acc = 0.0;
-    for (int p = 0; p < n_parts; ++p) acc += part[((size_t)p * ns_pad + s) * L + l];
+    for (int j = tile_off[tile]; j < tile_off[tile + 1]; ++j)
+        for (int b = 0; b < n_batches; ++b)
+            acc += part[(((size_t)tile_list[j] * n_batches + b) * 256 + (s & 255)) * L + l];
     gk[(size_t)l * ns + s] = acc * scale;
 }
 
@@ -766,83 +795,210 @@ hipError_t launch_scatter(Handle *h, const void *xm, void *x_full, float scale) 
 
 static void zero_slots(Handle *h) { k_zero_u32<<<1, 64, 0, h->stream>>>(slot(h, 0), 64); }
 
-// ---- kernel dispatch tables over (L, coil batch) ----
+// ---- kernel instances over (L, coil batch): visitors hand the instance to a launcher ----
 
-template <bool FWD, int L>
-static void launch_direct_L(Handle *h, int cb, dim3 grid, const float2 *src, float2 *part, int rows_pad,
-                            int n_tiles, int n_splits, int per_split, int n_stream, int n_batches) {
-    const double *coef = (const double *)h->coef.p, *basis = (const double *)h->basis.p;
-    if (cb == 8)
-        k_direct<FWD, L, 8><<<grid, 256, 0, h->stream>>>(coef, h->sk, basis, h->sb, src, part, rows_pad, h->ncs,
-                                                         n_tiles, n_splits, per_split, n_stream, n_batches);
-    else if (cb == 16)
-        k_direct<FWD, L, 16><<<grid, 256, 0, h->stream>>>(coef, h->sk, basis, h->sb, src, part, rows_pad, h->ncs,
-                                                          n_tiles, n_splits, per_split, n_stream, n_batches);
-    else
-        k_direct<FWD, L, 32><<<grid, 256, 0, h->stream>>>(coef, h->sk, basis, h->sb, src, part, rows_pad, h->ncs,
-                                                          n_tiles, n_splits, per_split, n_stream, n_batches);
+template <bool FWD, int L, class F>
+static void visit_direct_cb(int cb, F &&f) {
+    if (cb == 8) f(k_direct<FWD, L, 8>);
+    else if (cb == 16) f(k_direct<FWD, L, 16>);
+    else f(k_direct<FWD, L, 32>);
 }
+template <bool FWD, class F>
+static void visit_direct(int L, int cb, F &&f) {
+    switch (L) {
+        case 6: visit_direct_cb<FWD, 6>(cb, f); break;
+        case 10: visit_direct_cb<FWD, 10>(cb, f); break;
+        case 12: visit_direct_cb<FWD, 12>(cb, f); break;
+        case 16: visit_direct_cb<FWD, 16>(cb, f); break;
+        default: visit_direct_cb<FWD, 20>(cb, f); break;
+    }
+}
+template <typename T16, bool FWD, int L, class F>
+static void visit_mixed_nb(int nb, F &&f) {
+    if (nb == 1) f(k_mixed<T16, FWD, L, 1>);
+    else f(k_mixed<T16, FWD, L, 2>);
+}
+template <typename T16, bool FWD, class F>
+static void visit_mixed(int L, int nb, F &&f) {
+    switch (L) {
+        case 6: visit_mixed_nb<T16, FWD, 6>(nb, f); break;
+        case 10: visit_mixed_nb<T16, FWD, 10>(nb, f); break;
+        case 12: visit_mixed_nb<T16, FWD, 12>(nb, f); break;
+        case 16: visit_mixed_nb<T16, FWD, 16>(nb, f); break;
+        default: visit_mixed_nb<T16, FWD, 20>(nb, f); break;
+    }
+}
+template <int L, class F>
+static void visit_grad_cb(int cb, F &&f) {
+    if (cb == 8) f(k_grad<L, 8>);
+    else if (cb == 16) f(k_grad<L, 16>);
+    else f(k_grad<L, 32>);
+}
+template <class F>
+static void visit_grad(int L, int cb, F &&f) {
+    switch (L) {
+        case 6: visit_grad_cb<6>(cb, f); break;
+        case 10: visit_grad_cb<10>(cb, f); break;
+        case 12: visit_grad_cb<12>(cb, f); break;
+        case 16: visit_grad_cb<16>(cb, f); break;
+        default: visit_grad_cb<20>(cb, f); break;
+    }
+}
+// Visit the contraction kernel of a backend and direction.
+template <bool FWD, class F>
+static void visit_contract(Handle *h, int backend, F &&f) {
+    if (backend == HOE_BACKEND_DIRECT) visit_direct<FWD>(h->LK, direct_cb(h), f);
+    else if (backend == HOE_BACKEND_MIXED_FP16) visit_mixed<_Float16, FWD>(h->LK, mixed_nb(h), f);
+    else visit_mixed<__bf16, FWD>(h->LK, mixed_nb(h), f);
+}
+
+// ---- coil batching ----
+int direct_cb(const Handle *h) { return h->n_cha <= 8 ? 8 : (h->n_cha <= 16 ? 16 : 32); }
+int direct_batches(const Handle *h) { return (int)((h->n_cha + direct_cb(h) - 1) / direct_cb(h)); }
+int mixed_nb(const Handle *h) { return h->ncs / 32 >= 2 ? 2 : 1; }
+int mixed_groups(const Handle *h) { return (h->ncs / 32 + mixed_nb(h) - 1) / mixed_nb(h); }
+// fragment groups of 32 coils, rounded up to whole per-wave batches
+static inline int frag_groups(const Handle *h) { return mixed_groups(h) * mixed_nb(h); }
+
+// ---- stream-K schedules ----
+
+static hipError_t upload_ints(DeviceBuf &b, const std::vector<int> &v) {
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = std::max<size_t>(v.size(), 1) * sizeof(int);
+    hipError_t e = hipMalloc(&b.p, b.bytes);
+    if (e != hipSuccess) return e;
+    return hipMemcpy(b.p, v.data(), v.size() * sizeof(int), hipMemcpyHostToDevice);
+}
+
+// Cut tiles x n_ks units into 8 K-parts (one per XCD) and each part into
+// `slots/8` equal chunks of consecutive (tile, k) units; record the segments.
+hipError_t build_schedule(Sched &sc, int tiles, int n_ks, int slots) {
+    const int X = 8;
+    sc.tiles = tiles;
+    sc.n_ks = n_ks;
+    sc.cpx = std::max(1, slots / X);
+    sc.n_chunks = sc.cpx * X;
+    std::vector<int> seg, chunk_off(sc.n_chunks + 1, 0);
+    for (int x = 0; x < X; ++x) {
+        const int K0 = (int)((int64_t)x * n_ks / X), K1 = (int)((int64_t)(x + 1) * n_ks / X);
+        const int Kx = K1 - K0;
+        const int64_t units = (int64_t)tiles * Kx;
+        const int64_t U = (units + sc.cpx - 1) / sc.cpx;
+        for (int c = 0; c < sc.cpx; ++c) {
+            chunk_off[x * sc.cpx + c] = (int)(seg.size() / 4);
+            int64_t u = (int64_t)c * U;
+            const int64_t u1 = std::min(units, (int64_t)(c + 1) * U);
+            while (u < u1) {
+                const int t = (int)(u / Kx), kk = (int)(u % Kx);
+                const int len = (int)std::min<int64_t>(u1 - u, Kx - kk);
+                seg.push_back(t);
+                seg.push_back(K0 + kk);
+                seg.push_back(K0 + kk + len);
+                seg.push_back(0);
+                u += len;
+            }
+        }
+    }
+    sc.n_seg = (int)(seg.size() / 4);
+    chunk_off[sc.n_chunks] = sc.n_seg;
+    std::vector<int> tile_off(tiles + 1, 0), tile_list(sc.n_seg);
+    for (int i = 0; i < sc.n_seg; ++i) tile_off[seg[4 * i] + 1]++;
+    for (int t = 0; t < tiles; ++t) tile_off[t + 1] += tile_off[t];
+    std::vector<int> fill(tile_off.begin(), tile_off.end() - 1);
+    for (int i = 0; i < sc.n_seg; ++i) tile_list[fill[seg[4 * i]]++] = i;
+    hipError_t e;
+    if ((e = upload_ints(sc.seg, seg)) != hipSuccess) return e;
+    if ((e = upload_ints(sc.chunk_off, chunk_off)) != hipSuccess) return e;
+    if ((e = upload_ints(sc.tile_off, tile_off)) != hipSuccess) return e;
+    if ((e = upload_ints(sc.tile_list, tile_list)) != hipSuccess) return e;
+    sc.built = true;
+    return hipSuccess;
+}
+
+static hipError_t grow(DeviceBuf &b, size_t bytes) {
+    if (b.bytes >= bytes && b.p) return hipSuccess;
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+    hipError_t e = hipMalloc(&b.p, bytes);
+    if (e == hipSuccess) b.bytes = bytes;
+    return e;
+}
+
+template <class K>
+static int blocks_per_cu(K kern) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, 256, 0) != hipSuccess || nb < 1) nb = 1;
+    return nb;
+}
+
+hipError_t prepare_backend(Handle *h, int backend) {
+    Sched &sf = h->sched_f[backend], &sa = h->sched_a[backend];
+    if (sf.built && sa.built) return hipSuccess;
+    const int groups = backend == HOE_BACKEND_DIRECT ? direct_batches(h) : mixed_groups(h);
+    int occ_f = 1, occ_a = 1;
+    visit_contract<true>(h, backend, [&](auto k) { occ_f = blocks_per_cu(k); });
+    visit_contract<false>(h, backend, [&](auto k) { occ_a = blocks_per_cu(k); });
+    const int slots_f = std::max(8, h->cu_count * occ_f / groups);
+    const int slots_a = std::max(8, h->cu_count * occ_a / groups);
+    hipError_t e;
+    if ((e = build_schedule(sf, (int)(h->ns_pad / 256), (int)(h->nv_pad / 8), slots_f)) != hipSuccess) return e;
+    if ((e = build_schedule(sa, (int)(h->nv_pad / 256), (int)(h->ns_pad / 8), slots_a)) != hipSuccess) return e;
+    h->occ_f[backend] = occ_f;
+    h->occ_a[backend] = occ_a;
+    if ((e = grow(h->part_f, (size_t)sf.n_seg * 256 * h->ncs * sizeof(float2))) != hipSuccess) return e;
+    return grow(h->part_a, (size_t)sa.n_seg * 256 * h->ncs * sizeof(float2));
+}
+
+hipError_t prepare_grad(Handle *h) {
+    Sched &sg = h->sched_g;
+    if (sg.built) return hipSuccess;
+    int occ = 1;
+    visit_grad(h->LK, direct_cb(h), [&](auto k) { occ = blocks_per_cu(k); });
+    const int nb = direct_batches(h);
+    const int slots = std::max(8, h->cu_count * occ / nb);
+    hipError_t e = build_schedule(sg, (int)(h->ns_pad / 256), (int)(h->nv_pad / 8), slots);
+    if (e != hipSuccess) return e;
+    return grow(h->part_g, (size_t)sg.n_seg * nb * 256 * h->LK * sizeof(double));
+}
+
+// ---- contraction launches ----
 
 template <bool FWD>
-static void launch_direct(Handle *h, const float2 *src, float2 *part, int rows_pad, int n_tiles, int n_splits,
-                          int per_split, int n_stream) {
-    const int cb = h->n_cha <= 8 ? 8 : (h->n_cha <= 16 ? 16 : 32);
-    const int n_batches = (int)((h->n_cha + cb - 1) / cb);
-    dim3 grid(grid_blocks(n_tiles, n_splits * n_batches));
-    switch (h->L) {
-        case 9: launch_direct_L<FWD, 9>(h, cb, grid, src, part, rows_pad, n_tiles, n_splits, per_split, n_stream, n_batches); break;
-        case 13: launch_direct_L<FWD, 13>(h, cb, grid, src, part, rows_pad, n_tiles, n_splits, per_split, n_stream, n_batches); break;
-        case 16: launch_direct_L<FWD, 16>(h, cb, grid, src, part, rows_pad, n_tiles, n_splits, per_split, n_stream, n_batches); break;
-        default: launch_direct_L<FWD, 20>(h, cb, grid, src, part, rows_pad, n_tiles, n_splits, per_split, n_stream, n_batches); break;
-    }
-}
-
-template <typename T16, bool FWD, int L>
-static void launch_mixed_L(Handle *h, int nb, dim3 grid, const uint4 *frag, int n_ks, float2 *part, int rows_pad,
-                           int n_tiles, int n_splits, int ks_per_split, int n_groups) {
+static void launch_contract(Handle *h, const void *operand, float2 *part) {
+    const int backend = h->backend;
+    const Sched &sc = FWD ? h->sched_f[backend] : h->sched_a[backend];
     const double *coef = (const double *)h->coef.p, *basis = (const double *)h->basis.p;
-    if (nb == 1)
-        k_mixed<T16, FWD, L, 1><<<grid, 256, 0, h->stream>>>(coef, h->sk, basis, h->sb, frag, n_ks, part, rows_pad,
-                                                             h->ncs, n_tiles, n_splits, ks_per_split, n_groups);
-    else
-        k_mixed<T16, FWD, L, 2><<<grid, 256, 0, h->stream>>>(coef, h->sk, basis, h->sb, frag, n_ks, part, rows_pad,
-                                                             h->ncs, n_tiles, n_splits, ks_per_split, n_groups);
-}
-
-template <typename T16, bool FWD>
-static void launch_mixed(Handle *h, const uint4 *frag, int n_ks, float2 *part, int rows_pad, int n_tiles,
-                         int n_splits, int ks_per_split) {
-    const int n_gb = h->ncs / 32;
-    const int nb = n_gb >= 2 ? 2 : 1;
-    const int n_groups = (n_gb + nb - 1) / nb;
-    dim3 grid(grid_blocks(n_tiles, n_splits * n_groups));
-    switch (h->L) {
-        case 9: launch_mixed_L<T16, FWD, 9>(h, nb, grid, frag, n_ks, part, rows_pad, n_tiles, n_splits, ks_per_split, n_groups); break;
-        case 13: launch_mixed_L<T16, FWD, 13>(h, nb, grid, frag, n_ks, part, rows_pad, n_tiles, n_splits, ks_per_split, n_groups); break;
-        case 16: launch_mixed_L<T16, FWD, 16>(h, nb, grid, frag, n_ks, part, rows_pad, n_tiles, n_splits, ks_per_split, n_groups); break;
-        default: launch_mixed_L<T16, FWD, 20>(h, nb, grid, frag, n_ks, part, rows_pad, n_tiles, n_splits, ks_per_split, n_groups); break;
+    const int4 *segs = (const int4 *)sc.seg.p;
+    const int *coff = (const int *)sc.chunk_off.p;
+    if (backend == HOE_BACKEND_DIRECT) {
+        dim3 grid(sc.n_chunks, direct_batches(h));
+        const int n_stream = (int)(FWD ? h->n_vox : h->n_sam);
+        visit_direct<FWD>(h->LK, direct_cb(h), [&](auto k) {
+            k<<<grid, 256, 0, h->stream>>>(coef, h->sk, basis, h->sb, (const float2 *)operand, part, h->ncs,
+                                           n_stream, segs, coff, sc.cpx);
+        });
+    } else {
+        dim3 grid(sc.n_chunks, mixed_groups(h));
+        auto launch = [&](auto k) {
+            k<<<grid, 256, 0, h->stream>>>(coef, h->sk, basis, h->sb, (const uint4 *)operand, sc.n_ks, part,
+                                           h->ncs, segs, coff, sc.cpx);
+        };
+        if (backend == HOE_BACKEND_MIXED_FP16) visit_mixed<_Float16, FWD>(h->LK, mixed_nb(h), launch);
+        else visit_mixed<__bf16, FWD>(h->LK, mixed_nb(h), launch);
     }
-}
-
-// Number of coil-group fragments (32 coils each) in a padded layout where the
-// group count is rounded up to the per-wave batch count.
-static inline int frag_groups(Handle *h) {
-    const int n_gb = h->ncs / 32;
-    const int nb = n_gb >= 2 ? 2 : 1;
-    return ((n_gb + nb - 1) / nb) * nb;
 }
 
 // Builds the forward operand from masked xm (device) and runs the forward
 // contraction into part_f.
 static hipError_t forward_from_xm(Handle *h, const float2 *xm, bool time_it) {
-    const int n_tiles = (int)(h->ns_pad / 256);
     if (h->backend == HOE_BACKEND_DIRECT) {
         const int64_t nz = h->nv_pad * h->ncs;
         k_build_z<<<blocks_for(nz), 256, 0, h->stream>>>((const float2 *)h->csm.p, xm, (float2 *)h->zbuf.p,
                                                           (int)h->nv_pad, h->ncs);
         if (time_it) (void)hipEventRecord(h->ev[0], h->stream);
-        launch_direct<true>(h, (const float2 *)h->zbuf.p, (float2 *)h->part_f.p, (int)h->ns_pad, n_tiles,
-                            h->splits_f, h->vox_per_split, (int)h->n_vox);
+        launch_contract<true>(h, h->zbuf.p, (float2 *)h->part_f.p);
         if (time_it) (void)hipEventRecord(h->ev[1], h->stream);
     } else {
         const int n_ks = (int)(h->nv_pad / 8);
@@ -857,13 +1013,7 @@ static hipError_t forward_from_xm(Handle *h, const float2 *xm, bool time_it) {
             k_build_zfrag<__bf16><<<blocks_for(nt), 256, 0, h->stream>>>(
                 (const float2 *)h->csm.p, xm, (uint4 *)h->zfrag.p, n_ks, h->ncs, n_gb, slot(h, 0), h->csm_max);
         if (time_it) (void)hipEventRecord(h->ev[0], h->stream);
-        const int ks_per_split = h->vox_per_split / 8;
-        if (h->backend == HOE_BACKEND_MIXED_FP16)
-            launch_mixed<_Float16, true>(h, (const uint4 *)h->zfrag.p, n_ks, (float2 *)h->part_f.p, (int)h->ns_pad,
-                                         n_tiles, h->splits_f, ks_per_split);
-        else
-            launch_mixed<__bf16, true>(h, (const uint4 *)h->zfrag.p, n_ks, (float2 *)h->part_f.p, (int)h->ns_pad,
-                                       n_tiles, h->splits_f, ks_per_split);
+        launch_contract<true>(h, h->zfrag.p, (float2 *)h->part_f.p);
         if (time_it) (void)hipEventRecord(h->ev[1], h->stream);
     }
     return hipGetLastError();
@@ -878,19 +1028,22 @@ hipError_t run_forward_contract(Handle *h, const void *x_full, bool time_it) {
 static inline bool mixed(Handle *h) { return h->backend != HOE_BACKEND_DIRECT; }
 
 hipError_t launch_reduce_fwd_user(Handle *h, void *y_out, float scale) {
+    const Sched &sc = h->sched_f[h->backend];
     dim3 grid((unsigned)(h->ns_pad / 32), (unsigned)(h->ncs / 32));
-    k_reduce_fwd<<<grid, 256, 0, h->stream>>>((const float2 *)h->part_f.p, h->splits_f, (int)h->ns_pad, h->ncs,
-                                              (int)h->n_sam, (int)h->n_cha, scale, mixed(h) ? slot(h, 0) : nullptr,
-                                              h->csm_max, 0, (float2 *)y_out, nullptr, nullptr, nullptr);
+    k_reduce_fwd<<<grid, 256, 0, h->stream>>>((const float2 *)h->part_f.p, (const int *)sc.tile_off.p,
+                                              (const int *)sc.tile_list.p, h->ncs, (int)h->n_sam, (int)h->n_cha,
+                                              scale, mixed(h) ? slot(h, 0) : nullptr, h->csm_max, 0,
+                                              (float2 *)y_out, nullptr, nullptr, nullptr);
     return hipGetLastError();
 }
 
 hipError_t launch_reduce_fwd_ybuf(Handle *h, float scale, bool weights, const void *y_data) {
+    const Sched &sc = h->sched_f[h->backend];
     dim3 grid((unsigned)(h->ns_pad / 32), (unsigned)(h->ncs / 32));
-    k_reduce_fwd<<<grid, 256, 0, h->stream>>>((const float2 *)h->part_f.p, h->splits_f, (int)h->ns_pad, h->ncs,
-                                              (int)h->n_sam, (int)h->n_cha, scale, mixed(h) ? slot(h, 0) : nullptr,
-                                              h->csm_max, 1, (float2 *)h->ybuf.p,
-                                              weights ? (const float *)h->dvec.p : nullptr,
+    k_reduce_fwd<<<grid, 256, 0, h->stream>>>((const float2 *)h->part_f.p, (const int *)sc.tile_off.p,
+                                              (const int *)sc.tile_list.p, h->ncs, (int)h->n_sam, (int)h->n_cha,
+                                              scale, mixed(h) ? slot(h, 0) : nullptr, h->csm_max, 1,
+                                              (float2 *)h->ybuf.p, weights ? (const float *)h->dvec.p : nullptr,
                                               (const float2 *)y_data, slot(h, 1));
     return hipGetLastError();
 }
@@ -906,11 +1059,9 @@ hipError_t launch_load_ybuf(Handle *h, const void *y_user, bool weights) {
 
 // Adjoint contraction from ybuf (slot 1 must hold |ybuf|max for the mixed path).
 static hipError_t adjoint_contract_only(Handle *h, bool time_it) {
-    const int n_tiles = (int)(h->nv_pad / 256);
     if (h->backend == HOE_BACKEND_DIRECT) {
         if (time_it) (void)hipEventRecord(h->ev[2], h->stream);
-        launch_direct<false>(h, (const float2 *)h->ybuf.p, (float2 *)h->part_a.p, (int)h->nv_pad, n_tiles,
-                             h->splits_a, h->sam_per_split, (int)h->n_sam);
+        launch_contract<false>(h, h->ybuf.p, (float2 *)h->part_a.p);
         if (time_it) (void)hipEventRecord(h->ev[3], h->stream);
     } else {
         const int n_ks = (int)(h->ns_pad / 8);
@@ -925,24 +1076,19 @@ static hipError_t adjoint_contract_only(Handle *h, bool time_it) {
                                                                           (uint4 *)h->yfrag.p, n_ks, h->ncs, n_gb,
                                                                           slot(h, 1));
         if (time_it) (void)hipEventRecord(h->ev[2], h->stream);
-        const int ks_per_split = h->sam_per_split / 8;
-        if (h->backend == HOE_BACKEND_MIXED_FP16)
-            launch_mixed<_Float16, false>(h, (const uint4 *)h->yfrag.p, n_ks, (float2 *)h->part_a.p,
-                                          (int)h->nv_pad, n_tiles, h->splits_a, ks_per_split);
-        else
-            launch_mixed<__bf16, false>(h, (const uint4 *)h->yfrag.p, n_ks, (float2 *)h->part_a.p, (int)h->nv_pad,
-                                        n_tiles, h->splits_a, ks_per_split);
+        launch_contract<false>(h, h->yfrag.p, (float2 *)h->part_a.p);
         if (time_it) (void)hipEventRecord(h->ev[3], h->stream);
     }
     return hipGetLastError();
 }
 
 static hipError_t adjoint_reduce(Handle *h, float scale, void *xm_out, float lambda, const float2 *x_in) {
+    const Sched &sc = h->sched_a[h->backend];
     const int64_t threads = h->nv_pad * 32;
-    k_reduce_adj<<<blocks_for(threads), 256, 0, h->stream>>>((const float2 *)h->part_a.p, h->splits_a,
-                                                             (int)h->nv_pad, h->ncs, (const float2 *)h->csm.p,
-                                                             (int)h->n_vox, scale, mixed(h) ? slot(h, 1) : nullptr,
-                                                             (float2 *)xm_out, lambda, x_in);
+    k_reduce_adj<<<blocks_for(threads), 256, 0, h->stream>>>(
+        (const float2 *)h->part_a.p, (const int *)sc.tile_off.p, (const int *)sc.tile_list.p, (int)h->nv_pad,
+        h->ncs, (const float2 *)h->csm.p, (int)h->n_vox, scale, mixed(h) ? slot(h, 1) : nullptr, (float2 *)xm_out,
+        lambda, x_in);
     return hipGetLastError();
 }
 
@@ -967,34 +1113,22 @@ hipError_t run_normal_masked(Handle *h, const void *xm_in, void *xm_out, float s
 
 hipError_t run_grad_contract(Handle *h, bool time_it) {
     // zbuf must hold csm .* xm, ybuf the residual (coil fastest)
-    const int cb = h->n_cha <= 8 ? 8 : (h->n_cha <= 16 ? 16 : 32);
-    const int n_batches = (int)((h->n_cha + cb - 1) / cb);
-    const int n_tiles = (int)(h->ns_pad / 256);
-    const int n_parts = h->splits_g * n_batches;
-    dim3 grid(grid_blocks(n_tiles, n_parts));
+    const Sched &sc = h->sched_g;
+    const int nb = direct_batches(h);
+    dim3 grid(sc.n_chunks, nb);
     const double *coef = (const double *)h->coef.p, *basis = (const double *)h->basis.p;
     const float2 *z = (const float2 *)h->zbuf.p, *res = (const float2 *)h->ybuf.p;
     double *part = (double *)h->part_g.p;
     if (time_it) (void)hipEventRecord(h->ev[0], h->stream);
-#define HOE_GRAD(LL, CC)                                                                                           \
-    k_grad<LL, CC><<<grid, 256, 0, h->stream>>>(coef, h->sk, basis, h->sb, z, res, part, (int)h->ns_pad, h->ncs,  \
-                                                n_tiles, h->splits_g, h->vox_per_split_g, (int)h->n_vox, n_batches)
-#define HOE_GRAD_L(LL)                  \
-    if (cb == 8) HOE_GRAD(LL, 8);       \
-    else if (cb == 16) HOE_GRAD(LL, 16); \
-    else HOE_GRAD(LL, 32);
-    switch (h->L) {
-        case 9: HOE_GRAD_L(9) break;
-        case 13: HOE_GRAD_L(13) break;
-        case 16: HOE_GRAD_L(16) break;
-        default: HOE_GRAD_L(20) break;
-    }
-#undef HOE_GRAD_L
-#undef HOE_GRAD
+    visit_grad(h->LK, direct_cb(h), [&](auto k) {
+        k<<<grid, 256, 0, h->stream>>>(coef, h->sk, basis, h->sb, z, res, part, h->ncs, (int)h->n_vox,
+                                       (const int4 *)sc.seg.p, (const int *)sc.chunk_off.p, sc.cpx);
+    });
     if (time_it) (void)hipEventRecord(h->ev[1], h->stream);
     const double scale = -4.0 * M_PI / std::sqrt((double)h->n_vox);
-    k_reduce_grad<<<blocks_for(h->n_sam * h->L), 256, 0, h->stream>>>(part, n_parts, (int)h->ns_pad, h->L,
-                                                                      (int)h->n_sam, scale, (double *)h->gk.p);
+    k_reduce_grad<<<blocks_for(h->n_sam * h->LK), 256, 0, h->stream>>>(
+        part, (const int *)sc.tile_off.p, (const int *)sc.tile_list.p, nb, h->LK, (int)h->n_sam, scale,
+        (double *)h->gk.p);
     return hipGetLastError();
 }
 

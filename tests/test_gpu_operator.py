@@ -186,3 +186,31 @@ def test_config1_subset_matches_cpu_reference(backend, gpu_lib):
         print(f"config1 subset {backend}: fwd {e1:.2e} adj {e2:.2e}")
         assert e1 < tol
         assert e2 < tol
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+@pytest.mark.parametrize("nz,zoff", [(1, 0.0), (1, 0.03)])
+def test_concomitant_planar(backend, nz, zoff, gpu_lib):
+    """2-D grids with concomitant terms: on the z = 0 plane the z-dependent basis
+    columns vanish and are dropped from the kernel records (record length 12);
+    an off-centre plane keeps all 20 (test_Concomitant.jl:10 moves the plane)."""
+    TOL32 = TOLS[backend][0]
+    rng = np.random.default_rng(99)
+    grid = hoe.Grid(7, 6, nz, np.float32(0.03), np.float32(0.03), np.float32(0.03))
+    if zoff:
+        grid.z[:] = np.float32(zoff)
+    ns, nc = 17, 4
+    kspha = rng.normal(size=(16, ns)).astype(np.float32)
+    times = np.linspace(0, 2e-3, ns).astype(np.float32)
+    fm = rng.uniform(-40, 40, grid.n).astype(np.float32)
+    csm = (rng.normal(size=(grid.n, nc)) + 1j * rng.normal(size=(grid.n, nc))).astype(np.complex64)
+    conc = (rng.normal(size=(4, ns)) * 5).astype(np.float32)
+    x = (rng.normal(size=grid.n) + 1j * rng.normal(size=grid.n)).astype(np.complex64)
+    y = (rng.normal(size=ns * nc) + 1j * rng.normal(size=ns * nc)).astype(np.complex64)
+    p = dict(grid=grid, kspha=kspha, times=times, fieldmap=fm, csm=csm, mask=None)
+    m = _model(p, concomitant=conc)
+    A = O.dense_matrix_f64(m)
+    with _op(p, backend, concomitant=conc) as op:
+        assert O.relerr(op * x, A @ x) < TOL32
+        assert O.relerr(op.H * y, A.conj().T @ y) < TOL32
+        assert O.relerr(hoe.normalOperator(op) * x, A.conj().T @ (A @ x)) < TOL32
