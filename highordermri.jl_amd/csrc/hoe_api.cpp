@@ -89,6 +89,39 @@ void solid_harmonics(double x, double y, double z, double *out) {
     out[14] = x * x * x - 3.0 * x * y * y;        // 16
 }
 
+// Coefficients of basis term l (0-based: 1..15 solid harmonics 2..16, n_term..n_term+3
+// the concomitant columns z^2, x^2+y^2, xz, yz) as a cubic in x at fixed (y, z):
+// b_l(x, y, z) = p[0] + p[1] x + p[2] x^2 + p[3] x^3.
+void x_poly(int l, int n_term, double y, double z, double p[4]) {
+    p[0] = p[1] = p[2] = p[3] = 0.0;
+    if (l >= n_term) {
+        switch (l - n_term) {
+            case 0: p[0] = z * z; break;
+            case 1: p[0] = y * y; p[2] = 1.0; break;
+            case 2: p[1] = z; break;
+            default: p[0] = y * z; break;
+        }
+        return;
+    }
+    switch (l + 1) {
+        case 2: p[1] = 1.0; break;
+        case 3: p[0] = y; break;
+        case 4: p[0] = z; break;
+        case 5: p[1] = y; break;
+        case 6: p[0] = z * y; break;
+        case 7: p[0] = 2.0 * z * z - y * y; p[2] = -1.0; break;
+        case 8: p[1] = z; break;
+        case 9: p[0] = -y * y; p[2] = 1.0; break;
+        case 10: p[0] = -y * y * y; p[2] = 3.0 * y; break;
+        case 11: p[1] = y * z; break;
+        case 12: p[0] = (4.0 * z * z - y * y) * y; p[2] = -y; break;
+        case 13: p[0] = 2.0 * z * z * z - 3.0 * z * y * y; p[2] = -3.0 * z; break;
+        case 14: p[1] = 4.0 * z * z - y * y; p[3] = -1.0; break;
+        case 15: p[0] = -y * y * z; p[2] = z; break;
+        default: p[1] = -3.0 * y * y; p[3] = 1.0; break;   // 16: x^3 - 3 x y^2
+    }
+}
+
 // Uniform Catmull-Rom weights at fractional offset f in [0,1) for taps
 // -1, 0, 1, 2, and their derivatives with respect to f.
 void catmull_rom(double f, double w[4], double dw[4]) {
@@ -190,6 +223,7 @@ void release(Handle *h) {
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     hoe::DeviceBuf *bufs[] = {&h->mask_idx, &h->basis, &h->coef, &h->dcoef, &h->csm,   &h->xm,     &h->zbuf,
+                              &h->hrec,     &h->vrun,  &h->beta,
                               &h->zfrag,    &h->ybuf,  &h->yfrag, &h->part_f, &h->part_a, &h->part_g, &h->dvec,
                               &h->io_x,     &h->io_x2, &h->io_y,  &h->io_y2,  &h->scal,   &h->cg_vec, &h->red,
                               &h->gk,       &h->cg_log};
@@ -364,6 +398,47 @@ int hoe_create(const hoe_model_desc *d, hoe_handle **out) {
     }
     h->csm_max = csm_max;
 
+    // ---- runs of voxels sharing (y, z) and their x-polynomial tables (Horner forward) ----
+    std::vector<double> hrec((size_t)h->nv_pad * 2, 0.0);
+    std::vector<int32_t> vrun((size_t)h->nv_pad, 0);
+    std::vector<double> beta;
+    {
+        int run = -1;
+        float py = 0.0f, pz = 0.0f;
+        for (int64_t v = 0; v < nv; ++v) {
+            const int64_t g = idx[(size_t)v];
+            const float yv = d->y[g], zv = d->z[g];
+            if (run < 0 || yv != py || zv != pz) {
+                ++run;
+                py = yv;
+                pz = zv;
+                beta.resize((size_t)(run + 1) * h->LK * 4, 0.0);
+                double *bt = &beta[(size_t)run * h->LK * 4];
+                bt[0] = 1.0;                                   // k_1 * 1
+                for (int i = 0; i < na; ++i) x_poly(h->active[i], n_term, (double)yv, (double)zv, bt + 4 * (1 + i));
+            }
+            vrun[(size_t)v] = run;
+            hrec[2 * (size_t)v] = (double)d->x[g];
+            hrec[2 * (size_t)v + 1] = d->fieldmap ? (double)d->fieldmap[g] : 0.0;
+        }
+        for (int64_t v = nv; v < h->nv_pad; ++v) vrun[(size_t)v] = run;
+        h->n_runs = run + 1;
+        h->t_index = na + 1;
+        // consistency of the run polynomials with the basis records
+        bool ok = true;
+        for (int64_t v = 0; v < nv && ok; v += std::max<int64_t>(1, nv / 997)) {
+            const double x = hrec[2 * (size_t)v];
+            const double *bt = &beta[(size_t)vrun[(size_t)v] * h->LK * 4];
+            for (int i = 0; i < na; ++i) {
+                const double *p = bt + 4 * (1 + i);
+                const double val = p[0] + x * (p[1] + x * (p[2] + x * p[3]));
+                const double ref = bfull[(size_t)v * L + h->active[i]];
+                if (std::fabs(val - ref) > 1e-12 * (1.0 + std::fabs(ref))) ok = false;
+            }
+        }
+        h->horner = ok && (double)nv / (double)h->n_runs >= 16.0;
+    }
+
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, h->device) == hipSuccess && prop.multiProcessorCount > 0)
         h->cu_count = prop.multiProcessorCount;
@@ -393,6 +468,9 @@ int hoe_create(const hoe_model_desc *d, hoe_handle **out) {
         {&h->basis, sizeof(double) * brec.size()},
         {&h->coef, sizeof(double) * (size_t)h->ns_pad * h->sk},
         {&h->csm, sizeof(float2) * csmv.size()},
+        {&h->hrec, sizeof(double) * hrec.size()},
+        {&h->vrun, sizeof(int32_t) * vrun.size()},
+        {&h->beta, sizeof(double) * std::max<size_t>(beta.size(), 1)},
         {&h->xm, sizeof(float2) * (size_t)h->nv_pad},
         {&h->zbuf, sizeof(float2) * (size_t)h->nv_pad * h->ncs},
         {&h->zfrag, (size_t)n_gb_pad * (size_t)(h->nv_pad / 8) * 64 * 64},
@@ -417,6 +495,10 @@ int hoe_create(const hoe_model_desc *d, hoe_handle **out) {
     e = hipMemcpy(h->mask_idx.p, idx.data(), sizeof(int32_t) * nv, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(h->basis.p, brec.data(), sizeof(double) * brec.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(h->csm.p, csmv.data(), sizeof(float2) * csmv.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(h->hrec.p, hrec.data(), sizeof(double) * hrec.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(h->vrun.p, vrun.data(), sizeof(int32_t) * vrun.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !beta.empty())
+        e = hipMemcpy(h->beta.p, beta.data(), sizeof(double) * beta.size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) return fail(set_err(nullptr, HOE_EHIP, std::string("upload failed: ") + hipGetErrorString(e)));
     e = hoe::prepare_backend(h, h->backend);
     if (e == hipSuccess) e = hoe::prepare_grad(h);

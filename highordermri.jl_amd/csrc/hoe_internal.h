@@ -44,6 +44,9 @@ struct Handle {
     int32_t L = 0;                    // effective terms (b_1 = 1 included)
     int32_t LK = 0;                   // kernel record length: active terms + fieldmap, padded to a bucket
     std::vector<int> active;          // 0-based term indices (>= 1) with a not identically zero basis
+    bool horner = false;              // forward kernels evaluate the phase per run by Horner in x
+    int32_t n_runs = 0;               // runs: consecutive masked voxels sharing (y, z)
+    int32_t t_index = 0;              // index of t in the sample record (= number of active terms + 1)
     int32_t n_term_in = 0;            // 9 or 16
     int32_t ncs = 0;                  // coil stride of [row][coil] buffers: 32*ceil(n_cha/32)
     int32_t sb = 0;                   // doubles per voxel record (>= LK)
@@ -70,6 +73,9 @@ struct Handle {
     DeviceBuf coef;       // double [ns_pad][sk]: k_1, k of the active terms, t
     DeviceBuf dcoef;      // double [ns_pad][sk] d k / d tau (for the delay gradient)
     DeviceBuf csm;        // float2 [nv_pad][ncs]
+    DeviceBuf hrec;       // double2 [nv_pad]: (x, fieldmap) of each voxel, for the Horner forward
+    DeviceBuf vrun;       // int [nv_pad]: run of each voxel
+    DeviceBuf beta;       // double [n_runs][LK][4]: x-polynomial coefficients of each record entry
 
     // ---- workspaces ----
     DeviceBuf xm;         // float2 [nv_pad]        masked image

@@ -91,6 +91,28 @@ __device__ __forceinline__ void split2<__bf16>(float c, float s, unsigned &hi, u
     lo = __builtin_bit_cast(unsigned, l);
 }
 
+// Horner forward: voxels of one run share (y, z), so every record entry is a
+// cubic in x and phi(s,v) = c0 + x (c1 + x (c2 + x c3)) + t_s * fieldmap_v with
+// c_j(s) = sum_i A_s[i] * beta[run][i][j], set up once per run and lane.
+template <int L>
+__device__ __forceinline__ void run_setup(const double *__restrict__ arec, const double *__restrict__ beta_run,
+                                          double c[4]) {
+    c[0] = c[1] = c[2] = c[3] = 0.0;
+#pragma unroll
+    for (int i = 0; i < L; ++i) {
+        const double a = arec[i];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) c[j] = fma(a, beta_run[4 * i + j], c[j]);
+    }
+}
+
+struct HornerArgs {
+    const double2 *hrec;   // [nv_pad] (x, fieldmap)
+    const int *vrun;       // [nv_pad] run of each voxel
+    const double *beta;    // [n_runs][L][4]
+    int t_index;           // position of t in the sample record
+};
+
 template <typename T16>
 struct mfma_t;
 template <>
@@ -117,20 +139,22 @@ struct mfma_t<__bf16> {
 // Each lane keeps CB complex accumulators; the streamed record and the CB
 // complex operands of the streamed row arrive through scalar loads. A K unit
 // is 8 streamed rows; rows at or beyond n_stream (padding) are skipped.
-template <bool FWD, int L, int CB>
+template <bool FWD, bool HX, int L, int CB>
 __global__ __launch_bounds__(256) void k_direct(
     const double *__restrict__ coef, int sk, const double *__restrict__ basis, int sb,
     const float2 *__restrict__ src, float2 *__restrict__ part, int ncs, int n_stream,
-    const int4 *__restrict__ segs, const int *__restrict__ chunk_off, int cpx) {
+    const int4 *__restrict__ segs, const int *__restrict__ chunk_off, int cpx, HornerArgs hx) {
     const int chunk = chunk_id(cpx);
     const int cb0 = blockIdx.y * CB;
     const int sg0 = chunk_off[chunk], sg1 = chunk_off[chunk + 1];
-    constexpr int NST = FWD ? L + 1 : L;
+    constexpr int NST = HX ? 1 : (FWD ? L + 1 : L);
     for (int sgi = sg0; sgi < sg1; ++sgi) {
         const int4 d = segs[sgi];
         const int row = d.x * 256 + threadIdx.x;
         double st[NST];
-        {
+        if (HX) {
+            st[0] = coef[(size_t)row * sk + hx.t_index];
+        } else {
             const double *p = FWD ? coef + (size_t)row * sk : basis + (size_t)row * sb;
 #pragma unroll
             for (int i = 0; i < NST; ++i) st[i] = p[i];
@@ -140,10 +164,21 @@ __global__ __launch_bounds__(256) void k_direct(
         for (int c = 0; c < CB; ++c) ar[c] = ai[c] = 0.0f;
         const int q0 = d.y * 8;
         const int q1 = min(d.z * 8, n_stream);
+        double hc[4] = {0.0, 0.0, 0.0, 0.0};
+        int cur = -1;
         for (int q = q0; q < q1; ++q) {
             const double *sm = FWD ? basis + (size_t)q * sb : coef + (size_t)q * sk;
             double ph;
-            if (FWD) {
+            if (HX) {
+                const int r = hx.vrun[q];
+                if (r != cur) {
+                    run_setup<L>(coef + (size_t)row * sk, hx.beta + (size_t)r * L * 4, hc);
+                    cur = r;
+                }
+                const double2 xd = hx.hrec[q];
+                ph = fma(fma(fma(hc[3], xd.x, hc[2]), xd.x, hc[1]), xd.x, hc[0]);
+                ph = fma(st[0], xd.y, ph);
+            } else if (FWD) {
                 ph = st[0];
 #pragma unroll
                 for (int i = 1; i <= L; ++i) ph = fma(st[i], sm[i - 1], ph);
@@ -184,27 +219,33 @@ __global__ __launch_bounds__(256) void k_direct(
 // FP16/BF16 hi+lo, scaled by a power of two) is read with 16-byte loads.
 // Three products per K step: Ahi*Bhi + Alo*Bhi + Ahi*Blo, FP32 accumulate.
 // Output columns: tile 0 = real part, tile 1 = imaginary part of 32 coils.
-template <typename T16, bool FWD, int L, int NB>
+template <typename T16, bool FWD, bool HX, int L, int NB>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? 3 : 2))) void k_mixed(
     const double *__restrict__ coef, int sk, const double *__restrict__ basis, int sb,
     const uint4 *__restrict__ frag, int n_ks, float2 *__restrict__ part, int ncs,
-    const int4 *__restrict__ segs, const int *__restrict__ chunk_off, int cpx) {
+    const int4 *__restrict__ segs, const int *__restrict__ chunk_off, int cpx, HornerArgs hx) {
     typedef typename mfma_t<T16>::vec vec_t;
     const int chunk = chunk_id(cpx);
     const int grp = blockIdx.y;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int sg0 = chunk_off[chunk], sg1 = chunk_off[chunk + 1];
-    constexpr int NST = FWD ? L + 1 : L;
+    // Horner mode keeps only t and the run coefficients per lane; the record
+    // is re-read (cached) at run changes.
+    constexpr int NST = HX ? 1 : (FWD ? L + 1 : L);
     for (int sgi = sg0; sgi < sg1; ++sgi) {
         const int4 d = segs[sgi];
         const int row = d.x * 256 + wave * 64 + lane;
         double st[NST];
-        {
+        if (HX) {
+            st[0] = coef[(size_t)row * sk + hx.t_index];
+        } else {
             const double *p = FWD ? coef + (size_t)row * sk : basis + (size_t)row * sb;
 #pragma unroll
             for (int i = 0; i < NST; ++i) st[i] = p[i];
         }
+        double hc[4] = {0.0, 0.0, 0.0, 0.0};
+        int cur = -1;
         f16x_t acc[2][NB][2];
 #pragma unroll
         for (int a = 0; a < 2; ++a)
@@ -227,7 +268,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? 3
                 const int q = ks * 8 + j;
                 const double *sm = FWD ? basis + (size_t)q * sb : coef + (size_t)q * sk;
                 double ph;
-                if (FWD) {
+                if (HX) {
+                    const int r = hx.vrun[q];
+                    if (r != cur) {
+                        run_setup<L>(coef + (size_t)row * sk, hx.beta + (size_t)r * L * 4, hc);
+                        cur = r;
+                    }
+                    const double2 xd = hx.hrec[q];
+                    ph = fma(fma(fma(hc[3], xd.x, hc[2]), xd.x, hc[1]), xd.x, hc[0]);
+                    ph = fma(st[0], xd.y, ph);
+                } else if (FWD) {
                     ph = st[0];
 #pragma unroll
                     for (int i = 1; i <= L; ++i) ph = fma(st[i], sm[i - 1], ph);
@@ -797,35 +847,35 @@ static void zero_slots(Handle *h) { k_zero_u32<<<1, 64, 0, h->stream>>>(slot(h, 
 
 // ---- kernel instances over (L, coil batch): visitors hand the instance to a launcher ----
 
-template <bool FWD, int L, class F>
+template <bool FWD, bool HX, int L, class F>
 static void visit_direct_cb(int cb, F &&f) {
-    if (cb == 8) f(k_direct<FWD, L, 8>);
-    else if (cb == 16) f(k_direct<FWD, L, 16>);
-    else f(k_direct<FWD, L, 32>);
+    if (cb == 8) f(k_direct<FWD, HX, L, 8>);
+    else if (cb == 16) f(k_direct<FWD, HX, L, 16>);
+    else f(k_direct<FWD, HX, L, 32>);
 }
-template <bool FWD, class F>
+template <bool FWD, bool HX, class F>
 static void visit_direct(int L, int cb, F &&f) {
     switch (L) {
-        case 6: visit_direct_cb<FWD, 6>(cb, f); break;
-        case 10: visit_direct_cb<FWD, 10>(cb, f); break;
-        case 12: visit_direct_cb<FWD, 12>(cb, f); break;
-        case 16: visit_direct_cb<FWD, 16>(cb, f); break;
-        default: visit_direct_cb<FWD, 20>(cb, f); break;
+        case 6: visit_direct_cb<FWD, HX, 6>(cb, f); break;
+        case 10: visit_direct_cb<FWD, HX, 10>(cb, f); break;
+        case 12: visit_direct_cb<FWD, HX, 12>(cb, f); break;
+        case 16: visit_direct_cb<FWD, HX, 16>(cb, f); break;
+        default: visit_direct_cb<FWD, HX, 20>(cb, f); break;
     }
 }
-template <typename T16, bool FWD, int L, class F>
+template <typename T16, bool FWD, bool HX, int L, class F>
 static void visit_mixed_nb(int nb, F &&f) {
-    if (nb == 1) f(k_mixed<T16, FWD, L, 1>);
-    else f(k_mixed<T16, FWD, L, 2>);
+    if (nb == 1) f(k_mixed<T16, FWD, HX, L, 1>);
+    else f(k_mixed<T16, FWD, HX, L, 2>);
 }
-template <typename T16, bool FWD, class F>
+template <typename T16, bool FWD, bool HX, class F>
 static void visit_mixed(int L, int nb, F &&f) {
     switch (L) {
-        case 6: visit_mixed_nb<T16, FWD, 6>(nb, f); break;
-        case 10: visit_mixed_nb<T16, FWD, 10>(nb, f); break;
-        case 12: visit_mixed_nb<T16, FWD, 12>(nb, f); break;
-        case 16: visit_mixed_nb<T16, FWD, 16>(nb, f); break;
-        default: visit_mixed_nb<T16, FWD, 20>(nb, f); break;
+        case 6: visit_mixed_nb<T16, FWD, HX, 6>(nb, f); break;
+        case 10: visit_mixed_nb<T16, FWD, HX, 10>(nb, f); break;
+        case 12: visit_mixed_nb<T16, FWD, HX, 12>(nb, f); break;
+        case 16: visit_mixed_nb<T16, FWD, HX, 16>(nb, f); break;
+        default: visit_mixed_nb<T16, FWD, HX, 20>(nb, f); break;
     }
 }
 template <int L, class F>
@@ -844,12 +894,23 @@ static void visit_grad(int L, int cb, F &&f) {
         default: visit_grad_cb<20>(cb, f); break;
     }
 }
-// Visit the contraction kernel of a backend and direction.
+// Visit the contraction kernel of a backend and direction (the forward uses
+// the Horner variant when the handle's runs are long enough).
 template <bool FWD, class F>
 static void visit_contract(Handle *h, int backend, F &&f) {
-    if (backend == HOE_BACKEND_DIRECT) visit_direct<FWD>(h->LK, direct_cb(h), f);
-    else if (backend == HOE_BACKEND_MIXED_FP16) visit_mixed<_Float16, FWD>(h->LK, mixed_nb(h), f);
-    else visit_mixed<__bf16, FWD>(h->LK, mixed_nb(h), f);
+    bool hx = false;
+    if constexpr (FWD) hx = h->horner;
+    if (hx) {
+        if constexpr (FWD) {
+            if (backend == HOE_BACKEND_DIRECT) visit_direct<true, true>(h->LK, direct_cb(h), f);
+            else if (backend == HOE_BACKEND_MIXED_FP16) visit_mixed<_Float16, true, true>(h->LK, mixed_nb(h), f);
+            else visit_mixed<__bf16, true, true>(h->LK, mixed_nb(h), f);
+        }
+    } else {
+        if (backend == HOE_BACKEND_DIRECT) visit_direct<FWD, false>(h->LK, direct_cb(h), f);
+        else if (backend == HOE_BACKEND_MIXED_FP16) visit_mixed<_Float16, FWD, false>(h->LK, mixed_nb(h), f);
+        else visit_mixed<__bf16, FWD, false>(h->LK, mixed_nb(h), f);
+    }
 }
 
 // ---- coil batching ----
@@ -972,21 +1033,37 @@ static void launch_contract(Handle *h, const void *operand, float2 *part) {
     const double *coef = (const double *)h->coef.p, *basis = (const double *)h->basis.p;
     const int4 *segs = (const int4 *)sc.seg.p;
     const int *coff = (const int *)sc.chunk_off.p;
+    HornerArgs hx{(const double2 *)h->hrec.p, (const int *)h->vrun.p, (const double *)h->beta.p, h->t_index};
+    bool hxm = false;
+    if constexpr (FWD) hxm = h->horner;
     if (backend == HOE_BACKEND_DIRECT) {
         dim3 grid(sc.n_chunks, direct_batches(h));
         const int n_stream = (int)(FWD ? h->n_vox : h->n_sam);
-        visit_direct<FWD>(h->LK, direct_cb(h), [&](auto k) {
+        auto launch = [&](auto k) {
             k<<<grid, 256, 0, h->stream>>>(coef, h->sk, basis, h->sb, (const float2 *)operand, part, h->ncs,
-                                           n_stream, segs, coff, sc.cpx);
-        });
+                                           n_stream, segs, coff, sc.cpx, hx);
+        };
+        if (hxm) {
+            if constexpr (FWD) visit_direct<true, true>(h->LK, direct_cb(h), launch);
+        } else {
+            visit_direct<FWD, false>(h->LK, direct_cb(h), launch);
+        }
     } else {
         dim3 grid(sc.n_chunks, mixed_groups(h));
         auto launch = [&](auto k) {
             k<<<grid, 256, 0, h->stream>>>(coef, h->sk, basis, h->sb, (const uint4 *)operand, sc.n_ks, part,
-                                           h->ncs, segs, coff, sc.cpx);
+                                           h->ncs, segs, coff, sc.cpx, hx);
         };
-        if (backend == HOE_BACKEND_MIXED_FP16) visit_mixed<_Float16, FWD>(h->LK, mixed_nb(h), launch);
-        else visit_mixed<__bf16, FWD>(h->LK, mixed_nb(h), launch);
+        const bool f16 = backend == HOE_BACKEND_MIXED_FP16;
+        if (hxm) {
+            if constexpr (FWD) {
+                if (f16) visit_mixed<_Float16, true, true>(h->LK, mixed_nb(h), launch);
+                else visit_mixed<__bf16, true, true>(h->LK, mixed_nb(h), launch);
+            }
+        } else {
+            if (f16) visit_mixed<_Float16, FWD, false>(h->LK, mixed_nb(h), launch);
+            else visit_mixed<__bf16, FWD, false>(h->LK, mixed_nb(h), launch);
+        }
     }
 }
 
