@@ -92,6 +92,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-grad", action="store_true")
     ap.add_argument("--no-direct", action="store_true")
+    ap.add_argument("--no-cg", action="store_true")
     ap.add_argument("--cpu-samples", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -143,6 +144,32 @@ def main():
                                      backend=be) as so:
             fwd_err[be] = O.relerr(so * cfg["image"], y64)
     log(f"errors: normal mixed vs direct {err_mixed_vs_direct:.2e}, fwd vs fp64 {fwd_err}")
+    # ---- 30-iteration CGNR (recon_HOOp path, device resident) on noiseless synthetic data ----
+    cg = {}
+    if not args.no_cg:
+        from hoe import _lib as L_
+        op.set_backend("mixed_fp16")
+        y_syn = op.prod(x_dev)
+        torch.cuda.synchronize()
+        imgs = {}
+        for be in ("mixed_fp16", "direct"):
+            op.set_backend(be)
+            out = torch.empty(n_grid, dtype=torch.complex64, device=dev)
+            log_ = np.zeros(31)
+            D = np.ones(ns, np.float32)
+            ts = []
+            for rep in range(2):
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                L_.check(op._lib.hoe_cg(op._handle(), y_syn.data_ptr(), D.ctypes.data, D.size, 30, 0.0, 0.0,
+                                        out.data_ptr(), log_.ctypes.data, L_.HOE_DEVICE_PTRS), op._handle())
+                torch.cuda.synchronize()
+                ts.append(time.perf_counter() - t0)
+            imgs[be] = out.cpu().numpy()
+            cg[be] = {"ms": round(1e3 * min(ts), 3), "final_rel_residual": float(log_[-1])}
+        cg["image_mixed_vs_fp32"] = O.relerr(imgs["mixed_fp16"], imgs["direct"])
+        cg["image_vs_truth_fp32"] = O.relerr(imgs["direct"], cfg["image"])
+        log(f"cg: {cg}")
     # ---- gradient (workload 3 convention: y from perturbed coefficients and a 3 us delay) ----
     grad_ms = grad_kernel_ms = float("nan")
     if not args.no_grad:
@@ -182,7 +209,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, 16)
-        n_sub = args.cpu_samples or max(64, min(ns, 1024 * 16 // max(threads, 1)))
+        n_sub = args.cpu_samples or max(256, min(ns, 4096 * threads // 16))
         t_cpu, _ = cpu_baseline(cfg, n_sub, threads)
         cpu = {"value": round(1.0 / t_cpu, 5), "unit": "E^H E applies/s", "cores": threads, "kind": "port",
                "sample": f"oracle cpu_ref (restated array_operator.jl:241-348, OpenMP) E^H E on {n_sub} of {ns} "
@@ -205,6 +232,7 @@ def main():
         "errors": {"normal_mixed_vs_fp32": err_mixed_vs_direct,
                    "forward_vs_fp64_sampled": {k: float(v) for k, v in fwd_err.items()}},
         "grad_ms": round(grad_ms, 3), "grad_kernel_ms": round(grad_kernel_ms, 3),
+        "cg30": cg,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -25,6 +25,11 @@
 
 namespace hoe {
 
+// Resident waves per SIMD requested for the one-group MFMA kernels (register budget 512/N).
+#ifndef HOE_MIXED_WAVES
+#define HOE_MIXED_WAVES 3
+#endif
+
 typedef _Float16 h8_t __attribute__((ext_vector_type(8)));
 typedef __bf16 b8_t __attribute__((ext_vector_type(8)));
 typedef float f16x_t __attribute__((ext_vector_type(16)));
@@ -70,13 +75,26 @@ __device__ __forceinline__ float pow2_scale(float maxabs) {
 template <typename T16>
 __device__ __forceinline__ void split2(float c, float s, unsigned &hi, unsigned &lo);
 
+// c - (float)h.lo16 and c - (float)h.hi16 in one v_fma_mix_f32 each (the f16
+// half is widened inside the FMA; exact because c - h is exact in FP32).
+__device__ __forceinline__ float sub_f16lo(float c, unsigned h) {
+    float r;
+    asm("v_fma_mix_f32 %0, %1, -1.0, %2 op_sel_hi:[1,0,0]" : "=v"(r) : "v"(h), "v"(c));
+    return r;
+}
+__device__ __forceinline__ float sub_f16hi(float c, unsigned h) {
+    float r;
+    asm("v_fma_mix_f32 %0, %1, -1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(r) : "v"(h), "v"(c));
+    return r;
+}
+
 template <>
 __device__ __forceinline__ void split2<_Float16>(float c, float s, unsigned &hi, unsigned &lo) {
     typedef __fp16 fp16x2_t __attribute__((ext_vector_type(2)));
     const fp16x2_t h = __builtin_amdgcn_cvt_pkrtz(c, s);
-    const float cl = c - (float)h[0], sl = s - (float)h[1];
-    const fp16x2_t l = __builtin_amdgcn_cvt_pkrtz(cl, sl);
-    hi = __builtin_bit_cast(unsigned, h);
+    const unsigned hb = __builtin_bit_cast(unsigned, h);
+    const fp16x2_t l = __builtin_amdgcn_cvt_pkrtz(sub_f16lo(c, hb), sub_f16hi(s, hb));
+    hi = hb;
     lo = __builtin_bit_cast(unsigned, l);
 }
 
@@ -111,6 +129,11 @@ struct HornerArgs {
     const int *vrun;       // [nv_pad] run of each voxel
     const double *beta;    // [n_runs][L][4]
     int t_index;           // position of t in the sample record
+    // adjoint: waves whose 64 voxels share one run read per-(run, sample)
+    // coefficients c0..c3 and t from a table (8 doubles per entry)
+    const int *wave_run;   // [nv_pad / 64] run of the wave, or -1
+    const double *rowtab;  // [n_runs][ns_pad][8]
+    int ns_pad;
 };
 
 template <typename T16>
@@ -183,9 +206,10 @@ __global__ __launch_bounds__(256) void k_direct(
 #pragma unroll
                 for (int i = 1; i <= L; ++i) ph = fma(st[i], sm[i - 1], ph);
             } else {
-                ph = sm[0];
+                ph = sm[L] * st[L - 1];
 #pragma unroll
-                for (int i = 1; i <= L; ++i) ph = fma(sm[i], st[i - 1], ph);
+                for (int i = 1; i < L; ++i) ph = fma(sm[i], st[i - 1], ph);
+                ph += sm[0];
             }
             float sn, cs;
             sincos_cycles(ph, sn, cs);
@@ -220,7 +244,7 @@ __global__ __launch_bounds__(256) void k_direct(
 // Three products per K step: Ahi*Bhi + Alo*Bhi + Ahi*Blo, FP32 accumulate.
 // Output columns: tile 0 = real part, tile 1 = imaginary part of 32 coils.
 template <typename T16, bool FWD, bool HX, int L, int NB>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? 3 : 2))) void k_mixed(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? HOE_MIXED_WAVES : 2))) void k_mixed(
     const double *__restrict__ coef, int sk, const double *__restrict__ basis, int sb,
     const uint4 *__restrict__ frag, int n_ks, float2 *__restrict__ part, int ncs,
     const int4 *__restrict__ segs, const int *__restrict__ chunk_off, int cpx, HornerArgs hx) {
@@ -230,22 +254,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? 3
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int sg0 = chunk_off[chunk], sg1 = chunk_off[chunk + 1];
-    // Horner mode keeps only t and the run coefficients per lane; the record
-    // is re-read (cached) at run changes.
-    constexpr int NST = HX ? 1 : (FWD ? L + 1 : L);
+    constexpr int NST = FWD ? L + 1 : L;
     for (int sgi = sg0; sgi < sg1; ++sgi) {
         const int4 d = segs[sgi];
         const int row = d.x * 256 + wave * 64 + lane;
-        double st[NST];
-        if (HX) {
-            st[0] = coef[(size_t)row * sk + hx.t_index];
-        } else {
-            const double *p = FWD ? coef + (size_t)row * sk : basis + (size_t)row * sb;
-#pragma unroll
-            for (int i = 0; i < NST; ++i) st[i] = p[i];
-        }
-        double hc[4] = {0.0, 0.0, 0.0, 0.0};
-        int cur = -1;
         f16x_t acc[2][NB][2];
 #pragma unroll
         for (int a = 0; a < 2; ++a)
@@ -254,77 +266,138 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? 3
                 acc[a][b][0] = f16x_t{};
                 acc[a][b][1] = f16x_t{};
             }
-        for (int ks = d.y; ks < d.z; ++ks) {
-            uint4 bfr[NB][4];
+        // K loop over this segment; phases_of(ks, ph) fills the phases of the
+        // 8 streamed rows of K step ks.
+        auto kloop = [&](auto phases_of) {
+            for (int ks = d.y; ks < d.z; ++ks) {
+                uint4 bfr[NB][4];
 #pragma unroll
-            for (int b = 0; b < NB; ++b) {
-                const uint4 *p = frag + ((((size_t)(grp * NB + b) * n_ks + ks) * 64 + lane) << 2);
+                for (int b = 0; b < NB; ++b) {
+                    const uint4 *p = frag + ((((size_t)(grp * NB + b) * n_ks + ks) * 64 + lane) << 2);
 #pragma unroll
-                for (int q = 0; q < 4; ++q) bfr[b][q] = p[q];
-            }
-            unsigned hp[8], lp[8];
+                    for (int q = 0; q < 4; ++q) bfr[b][q] = p[q];
+                }
+                double ph[8];
+                phases_of(ks, ph);
+                unsigned hp[8], lp[8];
 #pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                const int q = ks * 8 + j;
-                const double *sm = FWD ? basis + (size_t)q * sb : coef + (size_t)q * sk;
-                double ph;
-                if (HX) {
-                    const int r = hx.vrun[q];
-                    if (r != cur) {
-                        run_setup<L>(coef + (size_t)row * sk, hx.beta + (size_t)r * L * 4, hc);
-                        cur = r;
+                for (int j = 0; j < 8; ++j) {
+                    float sn, cs;
+                    sincos_cycles(ph[j], sn, cs);
+                    split2<T16>(cs, sn, hp[j], lp[j]);
+                }
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    auto a = __builtin_amdgcn_permlane32_swap(hp[j], hp[j + 4], false, false);
+                    hp[j] = a[0];
+                    hp[j + 4] = a[1];
+                    auto b = __builtin_amdgcn_permlane32_swap(lp[j], lp[j + 4], false, false);
+                    lp[j] = b[0];
+                    lp[j + 4] = b[1];
+                }
+                vec_t ahi[2], alo[2];
+                {
+                    uint4 t0 = make_uint4(hp[0], hp[1], hp[2], hp[3]);
+                    uint4 t1 = make_uint4(hp[4], hp[5], hp[6], hp[7]);
+                    uint4 u0 = make_uint4(lp[0], lp[1], lp[2], lp[3]);
+                    uint4 u1 = make_uint4(lp[4], lp[5], lp[6], lp[7]);
+                    ahi[0] = __builtin_bit_cast(vec_t, t0);
+                    ahi[1] = __builtin_bit_cast(vec_t, t1);
+                    alo[0] = __builtin_bit_cast(vec_t, u0);
+                    alo[1] = __builtin_bit_cast(vec_t, u1);
+                }
+#pragma unroll
+                for (int b = 0; b < NB; ++b) {
+                    const vec_t bhr = __builtin_bit_cast(vec_t, bfr[b][0]);
+                    const vec_t bhi = __builtin_bit_cast(vec_t, bfr[b][1]);
+                    const vec_t blr = __builtin_bit_cast(vec_t, bfr[b][2]);
+                    const vec_t bli = __builtin_bit_cast(vec_t, bfr[b][3]);
+#pragma unroll
+                    for (int rt = 0; rt < 2; ++rt) {
+                        acc[rt][b][0] = mfma_t<T16>::run(ahi[rt], bhr, acc[rt][b][0]);
+                        acc[rt][b][1] = mfma_t<T16>::run(ahi[rt], bhi, acc[rt][b][1]);
+                        acc[rt][b][0] = mfma_t<T16>::run(alo[rt], bhr, acc[rt][b][0]);
+                        acc[rt][b][1] = mfma_t<T16>::run(alo[rt], bhi, acc[rt][b][1]);
+                        acc[rt][b][0] = mfma_t<T16>::run(ahi[rt], blr, acc[rt][b][0]);
+                        acc[rt][b][1] = mfma_t<T16>::run(ahi[rt], bli, acc[rt][b][1]);
                     }
-                    const double2 xd = hx.hrec[q];
-                    ph = fma(fma(fma(hc[3], xd.x, hc[2]), xd.x, hc[1]), xd.x, hc[0]);
-                    ph = fma(st[0], xd.y, ph);
-                } else if (FWD) {
-                    ph = st[0];
+                }
+            }
+        };
+        if constexpr (HX && FWD) {
+            // Horner forward: c_j set up per run, 4 FP64 FMAs per phase
+            // Runs are contiguous, so the 8 rows of a K step share one run when
+            // the first and the last do: then one run check per K step.
+            const double t = coef[(size_t)row * sk + hx.t_index];
+            double hc[4] = {0.0, 0.0, 0.0, 0.0};
+            int cur = -1;
+            kloop([&](int ks, double *ph) {
+                const int q0 = ks * 8;
+                const int r0 = hx.vrun[q0], r7 = hx.vrun[q0 + 7];
+                if (r0 == r7) {
+                    if (r0 != cur) {
+                        run_setup<L>(coef + (size_t)row * sk, hx.beta + (size_t)r0 * L * 4, hc);
+                        cur = r0;
+                    }
 #pragma unroll
-                    for (int i = 1; i <= L; ++i) ph = fma(st[i], sm[i - 1], ph);
+                    for (int j = 0; j < 8; ++j) {
+                        const double2 xd = hx.hrec[q0 + j];
+                        ph[j] = fma(t, xd.y, fma(fma(fma(hc[3], xd.x, hc[2]), xd.x, hc[1]), xd.x, hc[0]));
+                    }
                 } else {
-                    ph = sm[0];
 #pragma unroll
-                    for (int i = 1; i <= L; ++i) ph = fma(sm[i], st[i - 1], ph);
+                    for (int j = 0; j < 8; ++j) {
+                        const int r = hx.vrun[q0 + j];
+                        if (r != cur) {
+                            run_setup<L>(coef + (size_t)row * sk, hx.beta + (size_t)r * L * 4, hc);
+                            cur = r;
+                        }
+                        const double2 xd = hx.hrec[q0 + j];
+                        ph[j] = fma(t, xd.y, fma(fma(fma(hc[3], xd.x, hc[2]), xd.x, hc[1]), xd.x, hc[0]));
+                    }
                 }
-                float sn, cs;
-                sincos_cycles(ph, sn, cs);
-                split2<T16>(cs, sn, hp[j], lp[j]);
-            }
+            });
+        } else {
+            // wave-uniform (readfirstlane): the table is then read with scalar loads
+            const int wr = (HX && !FWD) ? __builtin_amdgcn_readfirstlane(hx.wave_run[d.x * 4 + wave]) : -1;
+            if (HX && !FWD && wr >= 0) {
+                // Horner adjoint: the wave's voxels share one run; per-sample c_j and t from the table
+                const double2 xd = hx.hrec[row];
+                const double x1 = xd.x, x2 = x1 * x1, x3 = x2 * x1, df = xd.y;
+                const double *tab = hx.rowtab + (size_t)wr * hx.ns_pad * 8;
+                kloop([&](int ks, double *ph) {
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                auto a = __builtin_amdgcn_permlane32_swap(hp[j], hp[j + 4], false, false);
-                hp[j] = a[0];
-                hp[j + 4] = a[1];
-                auto b = __builtin_amdgcn_permlane32_swap(lp[j], lp[j + 4], false, false);
-                lp[j] = b[0];
-                lp[j + 4] = b[1];
-            }
-            vec_t ahi[2], alo[2];
-            {
-                uint4 t0 = make_uint4(hp[0], hp[1], hp[2], hp[3]);
-                uint4 t1 = make_uint4(hp[4], hp[5], hp[6], hp[7]);
-                uint4 u0 = make_uint4(lp[0], lp[1], lp[2], lp[3]);
-                uint4 u1 = make_uint4(lp[4], lp[5], lp[6], lp[7]);
-                ahi[0] = __builtin_bit_cast(vec_t, t0);
-                ahi[1] = __builtin_bit_cast(vec_t, t1);
-                alo[0] = __builtin_bit_cast(vec_t, u0);
-                alo[1] = __builtin_bit_cast(vec_t, u1);
-            }
+                    for (int j = 0; j < 8; ++j) {
+                        const double *c = tab + (size_t)(ks * 8 + j) * 8;
+                        ph[j] = fma(c[4], df, fma(c[3], x3, fma(c[2], x2, fma(c[1], x1, c[0]))));
+                    }
+                });
+            } else {
+                double st[NST];
+                {
+                    const double *p = FWD ? coef + (size_t)row * sk : basis + (size_t)row * sb;
 #pragma unroll
-            for (int b = 0; b < NB; ++b) {
-                const vec_t bhr = __builtin_bit_cast(vec_t, bfr[b][0]);
-                const vec_t bhi = __builtin_bit_cast(vec_t, bfr[b][1]);
-                const vec_t blr = __builtin_bit_cast(vec_t, bfr[b][2]);
-                const vec_t bli = __builtin_bit_cast(vec_t, bfr[b][3]);
-#pragma unroll
-                for (int rt = 0; rt < 2; ++rt) {
-                    acc[rt][b][0] = mfma_t<T16>::run(ahi[rt], bhr, acc[rt][b][0]);
-                    acc[rt][b][1] = mfma_t<T16>::run(ahi[rt], bhi, acc[rt][b][1]);
-                    acc[rt][b][0] = mfma_t<T16>::run(alo[rt], bhr, acc[rt][b][0]);
-                    acc[rt][b][1] = mfma_t<T16>::run(alo[rt], bhi, acc[rt][b][1]);
-                    acc[rt][b][0] = mfma_t<T16>::run(ahi[rt], blr, acc[rt][b][0]);
-                    acc[rt][b][1] = mfma_t<T16>::run(ahi[rt], bli, acc[rt][b][1]);
+                    for (int i = 0; i < NST; ++i) st[i] = p[i];
                 }
+                kloop([&](int ks, double *ph) {
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        const int q = ks * 8 + j;
+                        const double *sm = FWD ? basis + (size_t)q * sb : coef + (size_t)q * sk;
+                        double p;
+                        if (FWD) {
+                            p = st[0];
+#pragma unroll
+                            for (int i = 1; i <= L; ++i) p = fma(st[i], sm[i - 1], p);
+                        } else {
+                            p = sm[L] * st[L - 1];
+#pragma unroll
+                            for (int i = 1; i < L; ++i) p = fma(sm[i], st[i - 1], p);
+                            p += sm[0];
+                        }
+                        ph[j] = p;
+                    }
+                });
             }
         }
         // D layout of v_mfma_f32_32x32x*: register r of lane l holds
@@ -896,21 +969,33 @@ static void visit_grad(int L, int cb, F &&f) {
 }
 // Visit the contraction kernel of a backend and direction (the forward uses
 // the Horner variant when the handle's runs are long enough).
+// The contraction kernel of a backend and direction: the forward uses the
+// Horner variant when the runs are long enough (all backends), the mixed
+// adjoint when the run table exists.
 template <bool FWD, class F>
-static void visit_contract(Handle *h, int backend, F &&f) {
-    bool hx = false;
-    if constexpr (FWD) hx = h->horner;
-    if (hx) {
-        if constexpr (FWD) {
-            if (backend == HOE_BACKEND_DIRECT) visit_direct<true, true>(h->LK, direct_cb(h), f);
-            else if (backend == HOE_BACKEND_MIXED_FP16) visit_mixed<_Float16, true, true>(h->LK, mixed_nb(h), f);
-            else visit_mixed<__bf16, true, true>(h->LK, mixed_nb(h), f);
-        }
+static void visit_contract_direct(Handle *h, F &&f) {
+    if constexpr (FWD) {
+        if (h->horner) visit_direct<true, true>(h->LK, direct_cb(h), f);
+        else visit_direct<true, false>(h->LK, direct_cb(h), f);
     } else {
-        if (backend == HOE_BACKEND_DIRECT) visit_direct<FWD, false>(h->LK, direct_cb(h), f);
-        else if (backend == HOE_BACKEND_MIXED_FP16) visit_mixed<_Float16, FWD, false>(h->LK, mixed_nb(h), f);
+        visit_direct<false, false>(h->LK, direct_cb(h), f);
+    }
+}
+template <bool FWD, class F>
+static void visit_contract_mixed(Handle *h, int backend, F &&f) {
+    const bool hx = FWD ? h->horner : h->horner_adj;
+    if (backend == HOE_BACKEND_MIXED_FP16) {
+        if (hx) visit_mixed<_Float16, FWD, true>(h->LK, mixed_nb(h), f);
+        else visit_mixed<_Float16, FWD, false>(h->LK, mixed_nb(h), f);
+    } else {
+        if (hx) visit_mixed<__bf16, FWD, true>(h->LK, mixed_nb(h), f);
         else visit_mixed<__bf16, FWD, false>(h->LK, mixed_nb(h), f);
     }
+}
+template <bool FWD, class F>
+static void visit_contract(Handle *h, int backend, F &&f) {
+    if (backend == HOE_BACKEND_DIRECT) visit_contract_direct<FWD>(h, f);
+    else visit_contract_mixed<FWD>(h, backend, f);
 }
 
 // ---- coil batching ----
@@ -1033,38 +1118,52 @@ static void launch_contract(Handle *h, const void *operand, float2 *part) {
     const double *coef = (const double *)h->coef.p, *basis = (const double *)h->basis.p;
     const int4 *segs = (const int4 *)sc.seg.p;
     const int *coff = (const int *)sc.chunk_off.p;
-    HornerArgs hx{(const double2 *)h->hrec.p, (const int *)h->vrun.p, (const double *)h->beta.p, h->t_index};
-    bool hxm = false;
-    if constexpr (FWD) hxm = h->horner;
+    HornerArgs hx{(const double2 *)h->hrec.p, (const int *)h->vrun.p, (const double *)h->beta.p, h->t_index,
+                  (const int *)h->wave_run.p, (const double *)h->rowtab.p, (int)h->ns_pad};
     if (backend == HOE_BACKEND_DIRECT) {
         dim3 grid(sc.n_chunks, direct_batches(h));
         const int n_stream = (int)(FWD ? h->n_vox : h->n_sam);
-        auto launch = [&](auto k) {
+        visit_contract_direct<FWD>(h, [&](auto k) {
             k<<<grid, 256, 0, h->stream>>>(coef, h->sk, basis, h->sb, (const float2 *)operand, part, h->ncs,
                                            n_stream, segs, coff, sc.cpx, hx);
-        };
-        if (hxm) {
-            if constexpr (FWD) visit_direct<true, true>(h->LK, direct_cb(h), launch);
-        } else {
-            visit_direct<FWD, false>(h->LK, direct_cb(h), launch);
-        }
+        });
     } else {
         dim3 grid(sc.n_chunks, mixed_groups(h));
-        auto launch = [&](auto k) {
+        visit_contract_mixed<FWD>(h, backend, [&](auto k) {
             k<<<grid, 256, 0, h->stream>>>(coef, h->sk, basis, h->sb, (const uint4 *)operand, sc.n_ks, part,
                                            h->ncs, segs, coff, sc.cpx, hx);
-        };
-        const bool f16 = backend == HOE_BACKEND_MIXED_FP16;
-        if (hxm) {
-            if constexpr (FWD) {
-                if (f16) visit_mixed<_Float16, true, true>(h->LK, mixed_nb(h), launch);
-                else visit_mixed<__bf16, true, true>(h->LK, mixed_nb(h), launch);
-            }
-        } else {
-            if (f16) visit_mixed<_Float16, FWD, false>(h->LK, mixed_nb(h), launch);
-            else visit_mixed<__bf16, FWD, false>(h->LK, mixed_nb(h), launch);
-        }
+        });
     }
+}
+
+// Per-(run, sample) cubic coefficients for the Horner adjoint:
+// tab[r][s] = [c0, c1, c2, c3, t_s, 0, 0, 0], c_j = sum_i A_s[i] * beta[r][i][j].
+__global__ void k_build_rowtab(const double *__restrict__ coef, int sk, const double *__restrict__ beta, int L,
+                               int t_index, int n_runs, int ns_pad, double *__restrict__ tab) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)n_runs * ns_pad) return;
+    const int r = (int)(i / ns_pad), s = (int)(i % ns_pad);
+    const double *a = coef + (size_t)s * sk;
+    const double *bt = beta + (size_t)r * L * 4;
+    double c[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int k = 0; k < L; ++k)
+        for (int j = 0; j < 4; ++j) c[j] = fma(a[k], bt[4 * k + j], c[j]);
+    double *o = tab + (size_t)i * 8;
+    o[0] = c[0];
+    o[1] = c[1];
+    o[2] = c[2];
+    o[3] = c[3];
+    o[4] = a[t_index];
+    o[5] = o[6] = o[7] = 0.0;
+}
+
+hipError_t build_rowtab(Handle *h) {
+    if (!h->horner_adj) return hipSuccess;
+    const int64_t n = (int64_t)h->n_runs * h->ns_pad;
+    k_build_rowtab<<<blocks_for(n), 256, 0, h->stream>>>((const double *)h->coef.p, h->sk,
+                                                         (const double *)h->beta.p, h->LK, h->t_index, h->n_runs,
+                                                         (int)h->ns_pad, (double *)h->rowtab.p);
+    return hipGetLastError();
 }
 
 // Builds the forward operand from masked xm (device) and runs the forward
