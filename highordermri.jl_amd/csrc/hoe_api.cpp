@@ -176,7 +176,6 @@ int upload_coef(Handle *h) {
     std::vector<double> rec;
     build_coef_records(h, rec);
     HOE_TRY_HIP(h, hipMemcpy(h->coef.p, rec.data(), rec.size() * sizeof(double), hipMemcpyHostToDevice));
-    HOE_TRY_HIP(h, hoe::build_rowtab(h));
     HOE_TRY_HIP(h, hipStreamSynchronize(h->stream));
     return HOE_OK;
 }
@@ -225,7 +224,7 @@ void release(Handle *h) {
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     hoe::DeviceBuf *bufs[] = {&h->mask_idx, &h->basis, &h->coef, &h->dcoef, &h->csm,   &h->xm,     &h->zbuf,
-                              &h->hrec,     &h->vrun,  &h->beta,  &h->wave_run, &h->rowtab,
+                              &h->hrec,     &h->vrun,  &h->beta,
                               &h->zfrag,    &h->ybuf,  &h->yfrag, &h->part_f, &h->part_a, &h->part_g, &h->dvec,
                               &h->io_x,     &h->io_x2, &h->io_y,  &h->io_y2,  &h->scal,   &h->cg_vec, &h->red,
                               &h->gk,       &h->cg_log};
@@ -440,22 +439,6 @@ int hoe_create(const hoe_model_desc *d, hoe_handle **out) {
         }
         h->horner = ok && (double)nv / (double)h->n_runs >= 16.0;
     }
-    std::vector<int32_t> wave_run((size_t)(h->nv_pad / 64), -1);
-    {
-        int64_t single = 0;
-        for (int64_t g = 0; g < h->nv_pad / 64; ++g) {
-            int r = -2;
-            for (int64_t v = 64 * g; v < std::min<int64_t>(64 * g + 64, nv); ++v) {
-                if (r == -2) r = vrun[(size_t)v];
-                else if (r != vrun[(size_t)v]) r = -1;
-            }
-            wave_run[(size_t)g] = (r == -2) ? vrun[(size_t)std::min<int64_t>(64 * g, h->nv_pad - 1)] : r;
-            if (wave_run[(size_t)g] >= 0) ++single;
-        }
-        const double tab_bytes = (double)h->n_runs * (double)h->ns_pad * 64.0;
-        h->horner_adj = h->horner && tab_bytes <= 2.0 * 1024 * 1024 * 1024 &&
-                        single * 4 >= h->nv_pad / 64;
-    }
 
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, h->device) == hipSuccess && prop.multiProcessorCount > 0)
@@ -489,8 +472,6 @@ int hoe_create(const hoe_model_desc *d, hoe_handle **out) {
         {&h->hrec, sizeof(double) * hrec.size()},
         {&h->vrun, sizeof(int32_t) * vrun.size()},
         {&h->beta, sizeof(double) * std::max<size_t>(beta.size(), 1)},
-        {&h->wave_run, sizeof(int32_t) * wave_run.size()},
-        {&h->rowtab, h->horner_adj ? (size_t)h->n_runs * h->ns_pad * 64 : 256},
         {&h->xm, sizeof(float2) * (size_t)h->nv_pad},
         {&h->zbuf, sizeof(float2) * (size_t)h->nv_pad * h->ncs},
         {&h->zfrag, (size_t)n_gb_pad * (size_t)(h->nv_pad / 8) * 64 * 64},
@@ -519,8 +500,6 @@ int hoe_create(const hoe_model_desc *d, hoe_handle **out) {
     if (e == hipSuccess) e = hipMemcpy(h->vrun.p, vrun.data(), sizeof(int32_t) * vrun.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess && !beta.empty())
         e = hipMemcpy(h->beta.p, beta.data(), sizeof(double) * beta.size(), hipMemcpyHostToDevice);
-    if (e == hipSuccess)
-        e = hipMemcpy(h->wave_run.p, wave_run.data(), sizeof(int32_t) * wave_run.size(), hipMemcpyHostToDevice);
     if (e != hipSuccess) return fail(set_err(nullptr, HOE_EHIP, std::string("upload failed: ") + hipGetErrorString(e)));
     e = hoe::prepare_backend(h, h->backend);
     if (e == hipSuccess) e = hoe::prepare_grad(h);

@@ -45,7 +45,6 @@ struct Handle {
     int32_t LK = 0;                   // kernel record length: active terms + fieldmap, padded to a bucket
     std::vector<int> active;          // 0-based term indices (>= 1) with a not identically zero basis
     bool horner = false;              // forward kernels evaluate the phase per run by Horner in x
-    bool horner_adj = false;          // mixed adjoint uses the per-(run, sample) table for one-run waves
     int32_t n_runs = 0;               // runs: consecutive masked voxels sharing (y, z)
     int32_t t_index = 0;              // index of t in the sample record (= number of active terms + 1)
     int32_t n_term_in = 0;            // 9 or 16
@@ -77,8 +76,6 @@ struct Handle {
     DeviceBuf hrec;       // double2 [nv_pad]: (x, fieldmap) of each voxel, for the Horner forward
     DeviceBuf vrun;       // int [nv_pad]: run of each voxel
     DeviceBuf beta;       // double [n_runs][LK][4]: x-polynomial coefficients of each record entry
-    DeviceBuf wave_run;   // int [nv_pad / 64]: run shared by a wave's voxels, or -1
-    DeviceBuf rowtab;     // double [n_runs][ns_pad][8]: per-(run, sample) cubic coefficients and t
 
     // ---- workspaces ----
     DeviceBuf xm;         // float2 [nv_pad]        masked image
@@ -118,7 +115,6 @@ hipError_t build_schedule(Sched &sc, int tiles, int n_ks, int slots);
 hipError_t prepare_backend(Handle *h, int backend);
 hipError_t prepare_grad(Handle *h);
 // Rebuild the per-(run, sample) table of the Horner adjoint from the coefficient records.
-hipError_t build_rowtab(Handle *h);
 
 // ---- launchers (hoe_kernels.hip) ----
 // All run on h->stream and return hipSuccess or the first HIP error.

@@ -25,9 +25,13 @@
 
 namespace hoe {
 
-// Resident waves per SIMD requested for the one-group MFMA kernels (register budget 512/N).
-#ifndef HOE_MIXED_WAVES
-#define HOE_MIXED_WAVES 3
+// Resident waves per SIMD requested for the one-group MFMA kernels, forward
+// and adjoint (register budget 512/N per lane).
+#ifndef HOE_WAVES_FWD
+#define HOE_WAVES_FWD 3
+#endif
+#ifndef HOE_WAVES_ADJ
+#define HOE_WAVES_ADJ 3
 #endif
 
 typedef _Float16 h8_t __attribute__((ext_vector_type(8)));
@@ -129,11 +133,6 @@ struct HornerArgs {
     const int *vrun;       // [nv_pad] run of each voxel
     const double *beta;    // [n_runs][L][4]
     int t_index;           // position of t in the sample record
-    // adjoint: waves whose 64 voxels share one run read per-(run, sample)
-    // coefficients c0..c3 and t from a table (8 doubles per entry)
-    const int *wave_run;   // [nv_pad / 64] run of the wave, or -1
-    const double *rowtab;  // [n_runs][ns_pad][8]
-    int ns_pad;
 };
 
 template <typename T16>
@@ -244,7 +243,7 @@ __global__ __launch_bounds__(256) void k_direct(
 // Three products per K step: Ahi*Bhi + Alo*Bhi + Ahi*Blo, FP32 accumulate.
 // Output columns: tile 0 = real part, tile 1 = imaginary part of 32 coils.
 template <typename T16, bool FWD, bool HX, int L, int NB>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? HOE_MIXED_WAVES : 2))) void k_mixed(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (FWD ? HOE_WAVES_FWD : HOE_WAVES_ADJ) : 2))) void k_mixed(
     const double *__restrict__ coef, int sk, const double *__restrict__ basis, int sb,
     const uint4 *__restrict__ frag, int n_ks, float2 *__restrict__ part, int ncs,
     const int4 *__restrict__ segs, const int *__restrict__ chunk_off, int cpx, HornerArgs hx) {
@@ -296,16 +295,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? H
                     lp[j + 4] = b[1];
                 }
                 vec_t ahi[2], alo[2];
-                {
-                    uint4 t0 = make_uint4(hp[0], hp[1], hp[2], hp[3]);
-                    uint4 t1 = make_uint4(hp[4], hp[5], hp[6], hp[7]);
-                    uint4 u0 = make_uint4(lp[0], lp[1], lp[2], lp[3]);
-                    uint4 u1 = make_uint4(lp[4], lp[5], lp[6], lp[7]);
-                    ahi[0] = __builtin_bit_cast(vec_t, t0);
-                    ahi[1] = __builtin_bit_cast(vec_t, t1);
-                    alo[0] = __builtin_bit_cast(vec_t, u0);
-                    alo[1] = __builtin_bit_cast(vec_t, u1);
-                }
+                ahi[0] = __builtin_bit_cast(vec_t, make_uint4(hp[0], hp[1], hp[2], hp[3]));
+                ahi[1] = __builtin_bit_cast(vec_t, make_uint4(hp[4], hp[5], hp[6], hp[7]));
+                alo[0] = __builtin_bit_cast(vec_t, make_uint4(lp[0], lp[1], lp[2], lp[3]));
+                alo[1] = __builtin_bit_cast(vec_t, make_uint4(lp[4], lp[5], lp[6], lp[7]));
 #pragma unroll
                 for (int b = 0; b < NB; ++b) {
                     const vec_t bhr = __builtin_bit_cast(vec_t, bfr[b][0]);
@@ -358,47 +351,31 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? H
                 }
             });
         } else {
-            // wave-uniform (readfirstlane): the table is then read with scalar loads
-            const int wr = (HX && !FWD) ? __builtin_amdgcn_readfirstlane(hx.wave_run[d.x * 4 + wave]) : -1;
-            if (HX && !FWD && wr >= 0) {
-                // Horner adjoint: the wave's voxels share one run; per-sample c_j and t from the table
-                const double2 xd = hx.hrec[row];
-                const double x1 = xd.x, x2 = x1 * x1, x3 = x2 * x1, df = xd.y;
-                const double *tab = hx.rowtab + (size_t)wr * hx.ns_pad * 8;
-                kloop([&](int ks, double *ph) {
+            double st[NST];
+            {
+                const double *p = FWD ? coef + (size_t)row * sk : basis + (size_t)row * sb;
 #pragma unroll
-                    for (int j = 0; j < 8; ++j) {
-                        const double *c = tab + (size_t)(ks * 8 + j) * 8;
-                        ph[j] = fma(c[4], df, fma(c[3], x3, fma(c[2], x2, fma(c[1], x1, c[0]))));
-                    }
-                });
-            } else {
-                double st[NST];
-                {
-                    const double *p = FWD ? coef + (size_t)row * sk : basis + (size_t)row * sb;
-#pragma unroll
-                    for (int i = 0; i < NST; ++i) st[i] = p[i];
-                }
-                kloop([&](int ks, double *ph) {
-#pragma unroll
-                    for (int j = 0; j < 8; ++j) {
-                        const int q = ks * 8 + j;
-                        const double *sm = FWD ? basis + (size_t)q * sb : coef + (size_t)q * sk;
-                        double p;
-                        if (FWD) {
-                            p = st[0];
-#pragma unroll
-                            for (int i = 1; i <= L; ++i) p = fma(st[i], sm[i - 1], p);
-                        } else {
-                            p = sm[L] * st[L - 1];
-#pragma unroll
-                            for (int i = 1; i < L; ++i) p = fma(sm[i], st[i - 1], p);
-                            p += sm[0];
-                        }
-                        ph[j] = p;
-                    }
-                });
+                for (int i = 0; i < NST; ++i) st[i] = p[i];
             }
+            kloop([&](int ks, double *ph) {
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const int q = ks * 8 + j;
+                    const double *sm = FWD ? basis + (size_t)q * sb : coef + (size_t)q * sk;
+                    double p;
+                    if (FWD) {
+                        p = st[0];
+#pragma unroll
+                        for (int i = 1; i <= L; ++i) p = fma(st[i], sm[i - 1], p);
+                    } else {
+                        p = sm[L] * st[L - 1];
+#pragma unroll
+                        for (int i = 1; i < L; ++i) p = fma(sm[i], st[i - 1], p);
+                        p += sm[0];
+                    }
+                    ph[j] = p;
+                }
+            });
         }
         // D layout of v_mfma_f32_32x32x*: register r of lane l holds
         // row 8*(r/4) + 4*(l/32) + r%4, column l%32.
@@ -983,13 +960,17 @@ static void visit_contract_direct(Handle *h, F &&f) {
 }
 template <bool FWD, class F>
 static void visit_contract_mixed(Handle *h, int backend, F &&f) {
-    const bool hx = FWD ? h->horner : h->horner_adj;
-    if (backend == HOE_BACKEND_MIXED_FP16) {
-        if (hx) visit_mixed<_Float16, FWD, true>(h->LK, mixed_nb(h), f);
-        else visit_mixed<_Float16, FWD, false>(h->LK, mixed_nb(h), f);
+    if constexpr (FWD) {
+        if (backend == HOE_BACKEND_MIXED_FP16) {
+            if (h->horner) visit_mixed<_Float16, true, true>(h->LK, mixed_nb(h), f);
+            else visit_mixed<_Float16, true, false>(h->LK, mixed_nb(h), f);
+        } else {
+            if (h->horner) visit_mixed<__bf16, true, true>(h->LK, mixed_nb(h), f);
+            else visit_mixed<__bf16, true, false>(h->LK, mixed_nb(h), f);
+        }
     } else {
-        if (hx) visit_mixed<__bf16, FWD, true>(h->LK, mixed_nb(h), f);
-        else visit_mixed<__bf16, FWD, false>(h->LK, mixed_nb(h), f);
+        if (backend == HOE_BACKEND_MIXED_FP16) visit_mixed<_Float16, false, false>(h->LK, mixed_nb(h), f);
+        else visit_mixed<__bf16, false, false>(h->LK, mixed_nb(h), f);
     }
 }
 template <bool FWD, class F>
@@ -1118,8 +1099,7 @@ static void launch_contract(Handle *h, const void *operand, float2 *part) {
     const double *coef = (const double *)h->coef.p, *basis = (const double *)h->basis.p;
     const int4 *segs = (const int4 *)sc.seg.p;
     const int *coff = (const int *)sc.chunk_off.p;
-    HornerArgs hx{(const double2 *)h->hrec.p, (const int *)h->vrun.p, (const double *)h->beta.p, h->t_index,
-                  (const int *)h->wave_run.p, (const double *)h->rowtab.p, (int)h->ns_pad};
+    HornerArgs hx{(const double2 *)h->hrec.p, (const int *)h->vrun.p, (const double *)h->beta.p, h->t_index};
     if (backend == HOE_BACKEND_DIRECT) {
         dim3 grid(sc.n_chunks, direct_batches(h));
         const int n_stream = (int)(FWD ? h->n_vox : h->n_sam);
@@ -1134,36 +1114,6 @@ static void launch_contract(Handle *h, const void *operand, float2 *part) {
                                            h->ncs, segs, coff, sc.cpx, hx);
         });
     }
-}
-
-// Per-(run, sample) cubic coefficients for the Horner adjoint:
-// tab[r][s] = [c0, c1, c2, c3, t_s, 0, 0, 0], c_j = sum_i A_s[i] * beta[r][i][j].
-__global__ void k_build_rowtab(const double *__restrict__ coef, int sk, const double *__restrict__ beta, int L,
-                               int t_index, int n_runs, int ns_pad, double *__restrict__ tab) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (int64_t)n_runs * ns_pad) return;
-    const int r = (int)(i / ns_pad), s = (int)(i % ns_pad);
-    const double *a = coef + (size_t)s * sk;
-    const double *bt = beta + (size_t)r * L * 4;
-    double c[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int k = 0; k < L; ++k)
-        for (int j = 0; j < 4; ++j) c[j] = fma(a[k], bt[4 * k + j], c[j]);
-    double *o = tab + (size_t)i * 8;
-    o[0] = c[0];
-    o[1] = c[1];
-    o[2] = c[2];
-    o[3] = c[3];
-    o[4] = a[t_index];
-    o[5] = o[6] = o[7] = 0.0;
-}
-
-hipError_t build_rowtab(Handle *h) {
-    if (!h->horner_adj) return hipSuccess;
-    const int64_t n = (int64_t)h->n_runs * h->ns_pad;
-    k_build_rowtab<<<blocks_for(n), 256, 0, h->stream>>>((const double *)h->coef.p, h->sk,
-                                                         (const double *)h->beta.p, h->LK, h->t_index, h->n_runs,
-                                                         (int)h->ns_pad, (double *)h->rowtab.p);
-    return hipGetLastError();
 }
 
 // Builds the forward operand from masked xm (device) and runs the forward
