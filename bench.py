@@ -172,24 +172,32 @@ def main():
         log(f"cg: {cg}")
     # ---- gradient (workload 3 convention: y from perturbed coefficients and a 3 us delay) ----
     grad_ms = grad_kernel_ms = float("nan")
+    grad = {}
     if not args.no_grad:
-        op.set_backend("direct")
         kp = synth.perturb(cfg)
         gop = make_op(dict(cfg, kspha=kp), "direct")
         gop.set_coefficients(None, 3e-6)
         y_obs = gop * x_dev
         gop.close()
         torch.cuda.synchronize()
-        gts = []
-        for i in range(4):
-            t0 = time.perf_counter()
-            g, gtau, loss = op.grad(x_dev, y_obs)
-            torch.cuda.synchronize()
-            if i > 0:
-                gts.append(time.perf_counter() - t0)
-        grad_ms = 1e3 * float(np.median(gts))
-        grad_kernel_ms = op.last_kernel_ms(0)
-        log(f"grad: {grad_ms:.2f} ms (kernel {grad_kernel_ms:.2f} ms), loss {loss:.4e}, dL/dtau {gtau:.4e}")
+        gres = {}
+        for be in ("mixed_fp16", "direct"):
+            op.set_backend(be)
+            gts = []
+            for i in range(4):
+                t0 = time.perf_counter()
+                g, gtau, loss = op.grad(x_dev, y_obs)
+                torch.cuda.synchronize()
+                if i > 0:
+                    gts.append(time.perf_counter() - t0)
+            gres[be] = g
+            grad[be] = {"ms": round(1e3 * float(np.median(gts)), 3), "kernel_ms": round(op.last_kernel_ms(0), 3),
+                        "loss": loss, "dL_dtau": gtau}
+            log(f"grad {be}: {grad[be]['ms']:.2f} ms (kernel {grad[be]['kernel_ms']:.2f} ms), loss {loss:.6e}, "
+                f"dL/dtau {gtau:.6e}")
+        grad["mixed_vs_fp32"] = O.relerr(gres["mixed_fp16"], gres["direct"])
+        grad_ms, grad_kernel_ms = grad["mixed_fp16"]["ms"], grad["mixed_fp16"]["kernel_ms"]
+        log(f"grad mixed vs fp32: {grad['mixed_vs_fp32']:.2e}")
     op.close()
 
     value = args.steps / t_mixed
@@ -231,7 +239,7 @@ def main():
         "mixed_over_fp32": round(value / fp32_value, 3),
         "errors": {"normal_mixed_vs_fp32": err_mixed_vs_direct,
                    "forward_vs_fp64_sampled": {k: float(v) for k, v in fwd_err.items()}},
-        "grad_ms": round(grad_ms, 3), "grad_kernel_ms": round(grad_kernel_ms, 3),
+        "grad_ms": round(grad_ms, 3), "grad_kernel_ms": round(grad_kernel_ms, 3), "grad": grad,
         "cg30": cg,
         "roofline": roofline,
         "cpu_baseline": cpu,

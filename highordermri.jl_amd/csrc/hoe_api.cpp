@@ -172,10 +172,38 @@ void build_coef_records(const Handle *h, std::vector<double> &rec) {
     }
 }
 
+// d k / d tau of the kernel rows (term 1, active terms) per sample, [ns_pad][sk]:
+// the derivative of the Catmull-Rom resampling at s + tau/dwell (at tau = 0 the
+// interpolant through the samples, u = 0). All zero when dwell is unset.
+void build_dcoef_records(const Handle *h, std::vector<double> &rec) {
+    const int64_t ns = h->n_sam;
+    const int L = h->L;
+    rec.assign((size_t)h->ns_pad * h->sk, 0.0);
+    if (!(h->dwell > 0.0)) return;
+    const double u = h->tau / h->dwell;
+    const int64_t m = (int64_t)std::floor(u);
+    double w[4], dw[4];
+    catmull_rom(u - (double)m, w, dw);
+    const int na = (int)h->active.size();
+    for (int64_t s = 0; s < ns; ++s)
+        for (int r = 0; r <= na; ++r) {
+            const int l = r == 0 ? 0 : h->active[r - 1];
+            double acc = 0.0;
+            for (int t = 0; t < 4; ++t) {
+                int64_t j = s + m - 1 + t;
+                j = std::min<int64_t>(std::max<int64_t>(j, 0), ns - 1);
+                acc += dw[t] * (double)h->traces[(size_t)j * L + l];
+            }
+            rec[(size_t)s * h->sk + r] = acc / h->dwell;
+        }
+}
+
 int upload_coef(Handle *h) {
-    std::vector<double> rec;
+    std::vector<double> rec, drec;
     build_coef_records(h, rec);
+    build_dcoef_records(h, drec);
     HOE_TRY_HIP(h, hipMemcpy(h->coef.p, rec.data(), rec.size() * sizeof(double), hipMemcpyHostToDevice));
+    HOE_TRY_HIP(h, hipMemcpy(h->dcoef.p, drec.data(), drec.size() * sizeof(double), hipMemcpyHostToDevice));
     HOE_TRY_HIP(h, hipStreamSynchronize(h->stream));
     return HOE_OK;
 }
@@ -223,14 +251,14 @@ void release(Handle *h) {
     if (!h) return;
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    hoe::DeviceBuf *bufs[] = {&h->mask_idx, &h->basis, &h->coef, &h->dcoef, &h->csm,   &h->xm,     &h->zbuf,
-                              &h->hrec,     &h->vrun,  &h->beta,
-                              &h->zfrag,    &h->ybuf,  &h->yfrag, &h->part_f, &h->part_a, &h->part_g, &h->dvec,
-                              &h->io_x,     &h->io_x2, &h->io_y,  &h->io_y2,  &h->scal,   &h->cg_vec, &h->red,
-                              &h->gk,       &h->cg_log};
+    hoe::DeviceBuf *bufs[] = {&h->mask_idx, &h->basis,  &h->coef,   &h->dcoef,   &h->csm,    &h->xm,     &h->zbuf,
+                              &h->hrec,     &h->vrun,   &h->beta,   &h->zfrag,   &h->ybuf,   &h->yfrag,  &h->part_f,
+                              &h->part_a,   &h->part_g, &h->dvec,   &h->io_x,    &h->io_x2,  &h->io_y,   &h->io_y2,
+                              &h->scal,     &h->cg_vec, &h->red,    &h->gk,      &h->cg_log, &h->gzfrag, &h->grfrag,
+                              &h->gbfrag,   &h->gbscale, &h->part_gm, &h->ghbuf, &h->growmap};
     for (auto *b : bufs) dfree(*b);
     hoe::Sched *scheds[] = {&h->sched_f[0], &h->sched_f[1], &h->sched_f[2], &h->sched_a[0],
-                            &h->sched_a[1], &h->sched_a[2], &h->sched_g};
+                            &h->sched_a[1], &h->sched_a[2], &h->sched_g,    &h->sched_gm};
     for (auto *sc : scheds) {
         dfree(sc->seg);
         dfree(sc->chunk_off);
@@ -238,6 +266,7 @@ void release(Handle *h) {
         dfree(sc->tile_list);
         sc->built = false;
     }
+    h->gb_type = -1;
     for (auto &e : h->ev)
         if (e) {
             (void)hipEventDestroy(e);
@@ -398,6 +427,23 @@ int hoe_create(const hoe_model_desc *d, hoe_handle **out) {
         }
     }
     h->csm_max = csm_max;
+    // power-of-two scale of each gradient basis column (constant, active terms):
+    // max |b| * scale < 2^12, as the device's fragment scaling
+    {
+        auto pow2 = [](double m) {
+            if (!(m > 0.0) || !std::isfinite(m)) return 1.0f;
+            int e;
+            (void)std::frexp(m, &e);
+            return (float)std::ldexp(1.0, 12 - e);
+        };
+        h->gbscale_h.assign(32, 1.0f);
+        h->gbscale_h[0] = pow2(1.0);
+        for (int i = 0; i < na && 1 + i < 32; ++i) {
+            double m = 0.0;
+            for (int64_t v = 0; v < nv; ++v) m = std::max(m, std::fabs((double)(float)brec[(size_t)v * h->sb + i]));
+            h->gbscale_h[1 + i] = pow2(m);
+        }
+    }
 
     // ---- runs of voxels sharing (y, z) and their x-polynomial tables (Horner forward) ----
     std::vector<double> hrec((size_t)h->nv_pad * 2, 0.0);
@@ -468,6 +514,9 @@ int hoe_create(const hoe_model_desc *d, hoe_handle **out) {
         {&h->mask_idx, sizeof(int32_t) * (size_t)nv},
         {&h->basis, sizeof(double) * brec.size()},
         {&h->coef, sizeof(double) * (size_t)h->ns_pad * h->sk},
+        {&h->dcoef, sizeof(double) * (size_t)h->ns_pad * h->sk},
+        {&h->ghbuf, sizeof(float) * (size_t)ns * L},
+        {&h->growmap, sizeof(int32_t) * (size_t)L},
         {&h->csm, sizeof(float2) * csmv.size()},
         {&h->hrec, sizeof(double) * hrec.size()},
         {&h->vrun, sizeof(int32_t) * vrun.size()},
@@ -500,6 +549,13 @@ int hoe_create(const hoe_model_desc *d, hoe_handle **out) {
     if (e == hipSuccess) e = hipMemcpy(h->vrun.p, vrun.data(), sizeof(int32_t) * vrun.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess && !beta.empty())
         e = hipMemcpy(h->beta.p, beta.data(), sizeof(double) * beta.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        // kernel row of each original term for the gradient (-1: identically zero basis)
+        std::vector<int32_t> rowmap((size_t)L, -1);
+        rowmap[0] = 0;
+        for (int i = 0; i < na; ++i) rowmap[(size_t)h->active[i]] = 1 + i;
+        e = hipMemcpy(h->growmap.p, rowmap.data(), sizeof(int32_t) * rowmap.size(), hipMemcpyHostToDevice);
+    }
     if (e != hipSuccess) return fail(set_err(nullptr, HOE_EHIP, std::string("upload failed: ") + hipGetErrorString(e)));
     e = hoe::prepare_backend(h, h->backend);
     if (e == hipSuccess) e = hoe::prepare_grad(h);
@@ -663,77 +719,30 @@ int hoe_grad(hoe_handle *hh, const void *x, const void *y, float *g_h, double *g
     HOE_TRY_HIP(h, hoe::launch_reduce_fwd_ybuf(h, (float)(1.0 / std::sqrt((double)h->n_vox)), false, dy));
     double *loss_dev = reinterpret_cast<double *>(h->scal.p) + 16;
     HOE_TRY_HIP(h, hoe::launch_norm2(h, h->ybuf.p, h->ns_pad * h->ncs, loss_dev));
-    HOE_TRY_HIP(h, hoe::launch_build_zbuf(h, h->xm.p));
     HOE_TRY_HIP(h, hoe::run_grad_contract(h, true));
-    (void)hipEventRecord(h->ev[5], h->stream);
-    const int L = h->L;
-    const int64_t ns = h->n_sam;
-    std::vector<double> gkc((size_t)h->LK * ns);      // kernel rows: term 1, active terms, ...
-    double lossv = 0.0;
-    HOE_TRY_HIP(h, hipMemcpyAsync(gkc.data(), h->gk.p, sizeof(double) * gkc.size(), hipMemcpyDeviceToHost,
-                                  h->stream));
-    HOE_TRY_HIP(h, hipMemcpyAsync(&lossv, loss_dev, sizeof(double), hipMemcpyDeviceToHost, h->stream));
-    HOE_TRY_HIP(h, hipStreamSynchronize(h->stream));
-    h->last_ms[0] = ev_ms(h->ev[0], h->ev[1]);
-    h->last_ms[1] = -1.0;
-    h->last_ms[2] = ev_ms(h->ev[4], h->ev[5]);
-    if (loss) *loss = lossv;
-    // dL/dk for every original term (inactive terms have an identically zero basis)
-    std::vector<double> gk((size_t)L * ns, 0.0);
-    std::copy(gkc.begin(), gkc.begin() + ns, gk.begin());
-    for (size_t i = 0; i < h->active.size(); ++i)
-        std::copy(gkc.begin() + (int64_t)(1 + i) * ns, gkc.begin() + (int64_t)(2 + i) * ns,
-                  gk.begin() + (int64_t)h->active[i] * ns);
-    // chain rule through the delay interpolation (host, FP64)
+    // dL/dh (through the delay interpolation) and dL/dtau on the device
+    double *gtau_dev = reinterpret_cast<double *>(h->scal.p) + 17;
     const bool shift = h->tau != 0.0;
-    double w[4] = {0, 1, 0, 0}, dw[4] = {0, 0, 0, 0};
     int64_t m = 0;
+    double w[4] = {0.0, 1.0, 0.0, 0.0}, dw[4];
     if (shift) {
         const double u = h->tau / h->dwell;
         m = (int64_t)std::floor(u);
         catmull_rom(u - (double)m, w, dw);
     }
-    std::vector<double> gh;
-    if (g_h) gh.assign((size_t)L * ns, 0.0);
-    double gt = 0.0;
-    for (int64_t s = 0; s < ns; ++s)
-        for (int l = 0; l < L; ++l) {
-            const double g = gk[(size_t)l * ns + s];
-            if (!shift) {
-                if (g_h) gh[(size_t)s * L + l] += g;
-                continue;
-            }
-            double dk = 0.0;
-            for (int t = 0; t < 4; ++t) {
-                int64_t j = s + m - 1 + t;
-                j = std::min<int64_t>(std::max<int64_t>(j, 0), ns - 1);
-                if (g_h) gh[(size_t)j * L + l] += w[t] * g;
-                dk += dw[t] * (double)h->traces[(size_t)j * L + l];
-            }
-            gt += g * dk / h->dwell;
-        }
-    if (!shift && g_tau) {
-        // d k / d tau at tau = 0 from the same interpolant: u = 0 -> f = 0
-        double w0[4], dw0[4];
-        catmull_rom(0.0, w0, dw0);
-        if (h->dwell > 0.0) {
-            for (int64_t s = 0; s < ns; ++s)
-                for (int l = 0; l < L; ++l) {
-                    double dk = 0.0;
-                    for (int t = 0; t < 4; ++t) {
-                        int64_t j = s - 1 + t;
-                        j = std::min<int64_t>(std::max<int64_t>(j, 0), ns - 1);
-                        dk += dw0[t] * (double)h->traces[(size_t)j * L + l];
-                    }
-                    gt += gk[(size_t)l * ns + s] * dk / h->dwell;
-                }
-        } else {
-            gt = NAN;
-        }
-    }
+    HOE_TRY_HIP(h, hoe::launch_grad_chain(h, gtau_dev, shift, (int)m, w));
+    (void)hipEventRecord(h->ev[5], h->stream);
+    double host_sc[2] = {0.0, 0.0};
     if (g_h)
-        for (size_t i = 0; i < gh.size(); ++i) g_h[i] = (float)gh[i];
-    if (g_tau) *g_tau = gt;
+        HOE_TRY_HIP(h, hipMemcpyAsync(g_h, h->ghbuf.p, sizeof(float) * (size_t)h->n_sam * h->L,
+                                      hipMemcpyDeviceToHost, h->stream));
+    HOE_TRY_HIP(h, hipMemcpyAsync(host_sc, loss_dev, 2 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HOE_TRY_HIP(h, hipStreamSynchronize(h->stream));
+    h->last_ms[0] = ev_ms(h->ev[0], h->ev[1]);
+    h->last_ms[1] = -1.0;
+    h->last_ms[2] = ev_ms(h->ev[4], h->ev[5]);
+    if (loss) *loss = host_sc[0];
+    if (g_tau) *g_tau = h->dwell > 0.0 ? host_sc[1] : NAN;
     return HOE_OK;
 }
 

@@ -96,9 +96,19 @@ struct Handle {
     DeviceBuf red;        // double [4096] block partial sums
     DeviceBuf gk;         // double [L][n_sam] dL/dk reduced
     DeviceBuf cg_log;     // double [iters+1] CG relative residuals
+    // mixed-backend gradient (32 coils): MFMA fragments and partials
+    DeviceBuf gzfrag;     // A fragments of csm .* x: [nv_pad/32][ncs/8][64 lanes][4] uint4
+    DeviceBuf grfrag;     // B fragments of the residual: [ns_pad/32][ncs/8][64][2] uint4
+    DeviceBuf gbfrag;     // B fragments of the basis columns: [nv_pad/32][2][64][2] uint4
+    DeviceBuf gbscale;    // float [32]: power-of-two scale of each basis column
+    DeviceBuf part_gm;    // float [n_seg][256][32] gradient partials
+    DeviceBuf ghbuf;      // float [n_sam][L] dL/dh (term fastest)
+    DeviceBuf growmap;    // int [L]: kernel row of each original term, -1 if its basis is zero
+    std::vector<float> gbscale_h;
+    int gb_type = -1;     // backend whose element type gbfrag holds
 
     // schedules per backend (index = hoe_backend) and for the gradient kernel
-    Sched sched_f[3], sched_a[3], sched_g;
+    Sched sched_f[3], sched_a[3], sched_g, sched_gm;
     int occ_f[3] = {0, 0, 0}, occ_a[3] = {0, 0, 0};   // resident workgroups per CU
 
     // timing of the last call (ms)
@@ -114,6 +124,8 @@ hipError_t build_schedule(Sched &sc, int tiles, int n_ks, int slots);
 // Build the forward/adjoint schedules of a backend and size the partial buffers.
 hipError_t prepare_backend(Handle *h, int backend);
 hipError_t prepare_grad(Handle *h);
+// True when the current backend computes the gradient on the matrix cores.
+bool grad_on_mfma(const Handle *h);
 // Rebuild the per-(run, sample) table of the Horner adjoint from the coefficient records.
 
 // ---- launchers (hoe_kernels.hip) ----
@@ -143,5 +155,8 @@ hipError_t cg_run(Handle *h, const void *rhs_m, int iters, double lambda, double
                   void *x_m, double *resid_log_dev);
 // squared norm of a complex vector (n entries) into a device double.
 hipError_t launch_norm2(Handle *h, const void *v, int64_t n, double *out_dev);
+// dL/dh into ghbuf and dL/dtau into *gtau_dev from gk (after run_grad_contract);
+// shift, m, w: the delay resampling k(s) = sum_t w[t] h(clamp(s + m - 1 + t)).
+hipError_t launch_grad_chain(Handle *h, double *gtau_dev, bool shift, int m, const double w[4]);
 
 }  // namespace hoe

@@ -21,6 +21,7 @@
 #include "hoe_internal.h"
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 
 namespace hoe {
@@ -455,6 +456,302 @@ __global__ __launch_bounds__(256) void k_grad(
 }
 
 // ---------------------------------------------------------------------------
+// gradient on the matrix cores (mixed backends)
+// ---------------------------------------------------------------------------
+//
+// G[s][l] = sum_v b_l(v) W[s][v],  W = Im(e(s,v) U[s][v]),  U = sum_c conj(r[s][c]) z[v][c],
+// for a wave's 64 samples (two 32-sample tiles) and a segment of 32-voxel blocks:
+//  1. U^T (32 voxels x 32 samples, per tile) on v_mfma_f32_32x32x16: A = Z
+//     fragments (voxel rows, K = 2*ncs real: [zr | zi] for Re, [zi | -zr] for
+//     Im), B = residual fragments (K = [rr | ri], sample columns), hi/lo split.
+//     The residual fragments of the wave's samples sit in LDS for the segment.
+//  2. Phases with lane = sample (as in the forward: wave-uniform voxel
+//     records), then v_permlane32_swap moves (cos, sin) of voxel pairs (j, j+4)
+//     into the D layout of U^T: register r of lane l = voxel 8(r/4)+4(l/32)+r%4,
+//     sample l%32 of the tile. W is formed there.
+//  3. W (power-of-two scaled, hi/lo) is the A operand of the second product
+//     with the basis fragments (columns = terms, each scaled by a power of
+//     two); its K index 8(l/32)+i is the voxel of W's register 8h+i.
+// Scales: Z by sZ (max|x| max|csm|), residual by sR (max|r|), W by sW from the
+// bound 2 ncs max|z| max|r|; the reduction divides them out.
+__device__ __forceinline__ void grad_scales(const unsigned *__restrict__ slots, float csm_max, int ncs, float &sz,
+                                            float &sr, float &sw) {
+    const float mz = __uint_as_float(slots[0]) * csm_max, mr = __uint_as_float(slots[1]);
+    sz = pow2_scale(mz);
+    sr = pow2_scale(mr);
+    sw = pow2_scale(2.0f * (float)ncs * (mz * sz) * (mr * sr));
+}
+
+template <typename T16, bool HX, int L, int NKC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_gmixed(
+    const double *__restrict__ coef, int sk, const double *__restrict__ basis, int sb,
+    const uint4 *__restrict__ zfr, const uint4 *__restrict__ rfr, const uint4 *__restrict__ bfr,
+    float *__restrict__ part, const int4 *__restrict__ segs, const int *__restrict__ chunk_off, int cpx,
+    HornerArgs hx, const unsigned *__restrict__ slots, float csm_max, int ncs) {
+    constexpr int n_kc = NKC;   // 16-wide real K chunks: 2 ncs / 16
+    typedef typename mfma_t<T16>::vec vec_t;
+    extern __shared__ uint4 rlds[];   // [wave][kc][tile][hi, lo][64 lanes]
+    const int chunk = chunk_id(cpx);
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    float sz, sr, sw;
+    grad_scales(slots, csm_max, ncs, sz, sr, sw);
+    (void)sz;
+    (void)sr;
+    uint4 *my = rlds + (size_t)wave * n_kc * 4 * 64 + lane;
+    const int sg0 = chunk_off[chunk], sg1 = chunk_off[chunk + 1];
+    for (int sgi = sg0; sgi < sg1; ++sgi) {
+        const int4 d = segs[sgi];
+        const int s0 = d.x * 256 + wave * 64;   // first sample of the wave
+        const int row = s0 + lane;
+        // residual fragments of the wave's two sample tiles -> LDS (each lane keeps its own)
+#pragma unroll
+        for (int kc = 0; kc < n_kc; ++kc)
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                const uint4 *src = rfr + ((((size_t)(s0 / 32 + t) * n_kc + kc) * 64 + lane) << 1);
+                my[((kc * 2 + t) * 2 + 0) * 64] = src[0];
+                my[((kc * 2 + t) * 2 + 1) * 64] = src[1];
+            }
+        f16x_t g[2] = {f16x_t{}, f16x_t{}};
+        // Horner: t of the lane's sample; generic: its full record
+        double st[HX ? 1 : L + 1];
+        double hc[4] = {0.0, 0.0, 0.0, 0.0};
+        int cur = -1;
+        if (HX) {
+            st[0] = coef[(size_t)row * sk + hx.t_index];
+        } else {
+#pragma unroll
+            for (int i = 0; i <= L; ++i) st[i] = coef[(size_t)row * sk + i];
+        }
+        for (int kb = d.y; kb < d.z; ++kb) {
+            // ---- stage 1: U^T for both sample tiles ----
+            f16x_t ure[2] = {f16x_t{}, f16x_t{}}, uim[2] = {f16x_t{}, f16x_t{}};
+#pragma unroll
+            for (int kc = 0; kc < n_kc; ++kc) {
+                const uint4 *zp = zfr + ((((size_t)kb * n_kc + kc) * 64 + lane) << 2);
+                const vec_t zrh = __builtin_bit_cast(vec_t, zp[0]);
+                const vec_t zrl = __builtin_bit_cast(vec_t, zp[1]);
+                const vec_t zih = __builtin_bit_cast(vec_t, zp[2]);
+                const vec_t zil = __builtin_bit_cast(vec_t, zp[3]);
+#pragma unroll
+                for (int t = 0; t < 2; ++t) {
+                    const vec_t rh = __builtin_bit_cast(vec_t, my[((kc * 2 + t) * 2 + 0) * 64]);
+                    const vec_t rl = __builtin_bit_cast(vec_t, my[((kc * 2 + t) * 2 + 1) * 64]);
+                    ure[t] = mfma_t<T16>::run(zrh, rh, ure[t]);
+                    uim[t] = mfma_t<T16>::run(zih, rh, uim[t]);
+                    ure[t] = mfma_t<T16>::run(zrl, rh, ure[t]);
+                    uim[t] = mfma_t<T16>::run(zil, rh, uim[t]);
+                    ure[t] = mfma_t<T16>::run(zrh, rl, ure[t]);
+                    uim[t] = mfma_t<T16>::run(zih, rl, uim[t]);
+                }
+            }
+            // ---- stages 2 and 3, per half block of 16 voxels ----
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int q0 = kb * 32 + h * 16;
+                float cs[16], sn[16];
+                if constexpr (HX) {
+                    const int r0 = hx.vrun[q0], r15 = hx.vrun[q0 + 15];
+                    if (r0 == r15) {
+                        if (r0 != cur) {
+                            run_setup<L>(coef + (size_t)row * sk, hx.beta + (size_t)r0 * L * 4, hc);
+                            cur = r0;
+                        }
+#pragma unroll
+                        for (int j = 0; j < 16; ++j) {
+                            const double2 xd = hx.hrec[q0 + j];
+                            sincos_cycles(
+                                fma(st[0], xd.y, fma(fma(fma(hc[3], xd.x, hc[2]), xd.x, hc[1]), xd.x, hc[0])),
+                                sn[j], cs[j]);
+                        }
+                    } else {
+                        // a run boundary inside the half block: full records (generic phase)
+                        const double *ap = coef + (size_t)row * sk;
+                        double a[L + 1];
+#pragma unroll
+                        for (int i = 0; i <= L; ++i) a[i] = ap[i];
+#pragma unroll
+                        for (int j = 0; j < 16; ++j) {
+                            const double *sm = basis + (size_t)(q0 + j) * sb;
+                            double p = a[0];
+#pragma unroll
+                            for (int i = 1; i <= L; ++i) p = fma(a[i], sm[i - 1], p);
+                            sincos_cycles(p, sn[j], cs[j]);
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int j = 0; j < 16; ++j) {
+                        const double *sm = basis + (size_t)(q0 + j) * sb;
+                        double p = st[0];
+#pragma unroll
+                        for (int i = 1; i <= L; ++i) p = fma(st[i], sm[i - 1], p);
+                        sincos_cycles(p, sn[j], cs[j]);
+                    }
+                }
+                // (cos, sin) of voxel pairs (j, j+4) into the D layout of U^T
+#pragma unroll
+                for (int a = 0; a < 8; ++a) {
+                    const int j = 8 * (a >> 2) + (a & 3);
+                    auto c2 = __builtin_amdgcn_permlane32_swap(__float_as_uint(cs[j]), __float_as_uint(cs[j + 4]),
+                                                               false, false);
+                    cs[j] = __uint_as_float(c2[0]);
+                    cs[j + 4] = __uint_as_float(c2[1]);
+                    auto s2 = __builtin_amdgcn_permlane32_swap(__float_as_uint(sn[j]), __float_as_uint(sn[j + 4]),
+                                                               false, false);
+                    sn[j] = __uint_as_float(s2[0]);
+                    sn[j + 4] = __uint_as_float(s2[1]);
+                }
+                const uint4 *bp = bfr + ((((size_t)kb * 2 + h) * 64 + lane) << 1);
+                const vec_t bh = __builtin_bit_cast(vec_t, bp[0]);
+                const vec_t bl = __builtin_bit_cast(vec_t, bp[1]);
+#pragma unroll
+                for (int t = 0; t < 2; ++t) {
+                    unsigned wh[4], wl[4];
+#pragma unroll
+                    for (int i = 0; i < 8; i += 2) {
+                        const int j0 = 8 * (i >> 2) + (i & 3) + 4 * t, j1 = j0 + 1;
+                        const float w0 = fmaf(sn[j0], ure[t][8 * h + i], cs[j0] * uim[t][8 * h + i]) * sw;
+                        const float w1 = fmaf(sn[j1], ure[t][8 * h + i + 1], cs[j1] * uim[t][8 * h + i + 1]) * sw;
+                        split2<T16>(w0, w1, wh[i >> 1], wl[i >> 1]);
+                    }
+                    const vec_t ah = __builtin_bit_cast(vec_t, make_uint4(wh[0], wh[1], wh[2], wh[3]));
+                    const vec_t al = __builtin_bit_cast(vec_t, make_uint4(wl[0], wl[1], wl[2], wl[3]));
+                    g[t] = mfma_t<T16>::run(ah, bh, g[t]);
+                    g[t] = mfma_t<T16>::run(al, bh, g[t]);
+                    g[t] = mfma_t<T16>::run(ah, bl, g[t]);
+                }
+            }
+        }
+        // D layout: register r of lane l = sample 8(r/4)+4(l/32)+r%4 of the tile, term l%32
+        const int col = lane & 31, hi32 = lane >> 5;
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int rr = wave * 64 + t * 32 + 8 * (r >> 2) + 4 * hi32 + (r & 3);
+                part[((size_t)sgi * 256 + rr) * 32 + col] = g[t][r];
+            }
+    }
+}
+
+// Z fragments of the gradient (A operand, voxel rows):
+// zfr[kb][kc][lane] = {Re hi, Re lo, Im hi, Im lo}, 8 x T16 each, lane l:
+// voxel 32 kb + l%32, real K index k = 16 kc + 8 (l/32) + i: k < ncs ->
+// (zr_k | zi_k), else (zi_{k-ncs} | -zr_{k-ncs}).
+template <typename T16>
+__global__ void k_build_gzfrag(const float2 *__restrict__ csm, const float2 *__restrict__ xm, uint4 *__restrict__ frag,
+                               int n_kb, int n_kc, int ncs, const unsigned *__restrict__ slots, float csm_max) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)n_kb * n_kc * 64) return;
+    const int lane = (int)(t & 63);
+    const int kc = (int)((t >> 6) % n_kc);
+    const int kb = (int)((t >> 6) / n_kc);
+    float sz, sr, sw;
+    grad_scales(slots, csm_max, ncs, sz, sr, sw);
+    const int v = kb * 32 + (lane & 31);
+    const float2 xv = xm[v];
+    float re[8], im[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int k = 16 * kc + 8 * (lane >> 5) + i;
+        const int c = k < ncs ? k : k - ncs;
+        const float2 a = csm[(size_t)v * ncs + c];
+        const float zr = (a.x * xv.x - a.y * xv.y) * sz;
+        const float zi = (a.x * xv.y + a.y * xv.x) * sz;
+        re[i] = k < ncs ? zr : zi;
+        im[i] = k < ncs ? zi : -zr;
+    }
+    unsigned rh[4], rl[4], ih[4], il[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        split2<T16>(re[2 * i], re[2 * i + 1], rh[i], rl[i]);
+        split2<T16>(im[2 * i], im[2 * i + 1], ih[i], il[i]);
+    }
+    uint4 *o = frag + (t << 2);
+    o[0] = make_uint4(rh[0], rh[1], rh[2], rh[3]);
+    o[1] = make_uint4(rl[0], rl[1], rl[2], rl[3]);
+    o[2] = make_uint4(ih[0], ih[1], ih[2], ih[3]);
+    o[3] = make_uint4(il[0], il[1], il[2], il[3]);
+}
+
+// Residual fragments (B operand, sample columns): rfr[st][kc][lane] = {hi, lo},
+// lane l: sample 32 st + l%32, K index k = 16 kc + 8 (l/32) + i: rr_k (k < ncs)
+// or ri_{k-ncs}.  r = ybuf (coil fastest).
+template <typename T16>
+__global__ void k_build_grfrag(const float2 *__restrict__ r, uint4 *__restrict__ frag, int n_st, int n_kc, int ncs,
+                               const unsigned *__restrict__ slots, float csm_max) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)n_st * n_kc * 64) return;
+    const int lane = (int)(t & 63);
+    const int kc = (int)((t >> 6) % n_kc);
+    const int st = (int)((t >> 6) / n_kc);
+    float sz, sr, sw;
+    grad_scales(slots, csm_max, ncs, sz, sr, sw);
+    const int s = st * 32 + (lane & 31);
+    float val[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int k = 16 * kc + 8 * (lane >> 5) + i;
+        const float2 a = r[(size_t)s * ncs + (k < ncs ? k : k - ncs)];
+        val[i] = (k < ncs ? a.x : a.y) * sr;
+    }
+    unsigned hh[4], ll[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) split2<T16>(val[2 * i], val[2 * i + 1], hh[i], ll[i]);
+    uint4 *o = frag + (t << 1);
+    o[0] = make_uint4(hh[0], hh[1], hh[2], hh[3]);
+    o[1] = make_uint4(ll[0], ll[1], ll[2], ll[3]);
+}
+
+// Basis fragments (B operand of the second product): bfr[kb][h][lane] = {hi, lo},
+// lane l: term n = l%32 (n = 0: the constant 1; 1 <= n < ncols: record entry
+// n-1; else 0), K entry i: voxel 32 kb + 16 h + 8 (i/4) + 4 (l/32) + i%4, scaled by
+// bscale[n] (a power of two).
+template <typename T16>
+__global__ void k_build_gbfrag(const double *__restrict__ basis, int sb, int ncols, const float *__restrict__ bscale,
+                               uint4 *__restrict__ frag, int n_kb, int nv) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)n_kb * 2 * 64) return;
+    const int lane = (int)(t & 63);
+    const int h = (int)((t >> 6) & 1);
+    const int kb = (int)(t >> 7);
+    const int n = lane & 31;
+    float val[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int v = kb * 32 + 16 * h + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
+        double b = 0.0;
+        if (v < nv && n < ncols) b = n == 0 ? 1.0 : basis[(size_t)v * sb + (n - 1)];
+        val[i] = (float)b * bscale[n];
+    }
+    unsigned hh[4], ll[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) split2<T16>(val[2 * i], val[2 * i + 1], hh[i], ll[i]);
+    uint4 *o = frag + (t << 1);
+    o[0] = make_uint4(hh[0], hh[1], hh[2], hh[3]);
+    o[1] = make_uint4(ll[0], ll[1], ll[2], ll[3]);
+}
+
+// gk[l][s] = scale / (sZ sR sW bscale[l]) * sum over the tile's segments of part[seg][s%256][l]  (FP64)
+__global__ void k_reduce_gmixed(const float *__restrict__ part, const int *__restrict__ tile_off,
+                                const int *__restrict__ tile_list, int L, int ns, double scale,
+                                const float *__restrict__ bscale, const unsigned *__restrict__ slots, float csm_max,
+                                int ncs, double *__restrict__ gk) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)ns * L) return;
+    const int s = (int)(i / L), l = (int)(i % L);
+    const int tile = s >> 8;
+    float sz, sr, sw;
+    grad_scales(slots, csm_max, ncs, sz, sr, sw);
+    double acc = 0.0;
+    for (int j = tile_off[tile]; j < tile_off[tile + 1]; ++j)
+        acc += (double)part[((size_t)tile_list[j] * 256 + (s & 255)) * 32 + l];
+    gk[(size_t)l * ns + s] = acc * (scale / ((double)sz * (double)sr * (double)sw * (double)bscale[l]));
+}
+
+// ---------------------------------------------------------------------------
 // data movement and reductions
 // ---------------------------------------------------------------------------
 
@@ -764,6 +1061,59 @@ __global__ void k_reduce_grad(const double *__restrict__ part, const int *__rest
     gk[(size_t)l * ns + s] = acc * scale;
 }
 
+// ---- gradient chain rule through the delay interpolation ----
+
+// gh[j][l] = sum_t w[t] sum_{s : clamp(s+m-1+t) = j} gk[row(l)][s]   (shift != 0)
+//          = gk[row(l)][j]                                            (shift == 0)
+// (the adjoint of the Catmull-Rom resampling k(s) = sum_t w[t] h(clamp(s+m-1+t)));
+// rows of identically zero basis terms give 0. FP64 sums, FP32 output.
+__global__ void k_grad_chain(const double *__restrict__ gk, int ns, const int *__restrict__ rowmap, int L,
+                             int shift, int m, double w0, double w1, double w2, double w3, float *__restrict__ gh) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)ns * L) return;
+    const int j = (int)(i / L), l = (int)(i % L);
+    const int row = rowmap[l];
+    if (row < 0) {
+        gh[i] = 0.0f;
+        return;
+    }
+    const double *g = gk + (size_t)row * ns;
+    if (!shift) {
+        gh[i] = (float)g[j];
+        return;
+    }
+    const double w[4] = {w0, w1, w2, w3};
+    double acc = 0.0;
+    for (int t = 0; t < 4; ++t) {
+        // s with clamp(s + m - 1 + t) == j
+        int lo = j - m + 1 - t, hi = lo;
+        if (j == 0) lo = INT_MIN / 2;
+        if (j == ns - 1) hi = INT_MAX / 2;
+        lo = max(lo, 0);
+        hi = min(hi, ns - 1);
+        double a = 0.0;
+        for (int s = lo; s <= hi; ++s) a += g[s];
+        acc += w[t] * a;
+    }
+    gh[i] = (float)acc;
+}
+
+// block partial sums of sum_{s, r < nrow} gk[r][s] * dk[s][r]  (dL/dtau)
+__global__ __launch_bounds__(256) void k_gtau_partial(const double *__restrict__ gk, const double *__restrict__ dk,
+                                                      int ns, int nrow, int sk, double *__restrict__ red) {
+    __shared__ double sh[4];
+    double acc = 0.0;
+    const int64_t n = (int64_t)ns * nrow;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int r = (int)(i / ns), s = (int)(i % ns);
+        acc += gk[i] * dk[(size_t)s * sk + r];
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) red[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
 // ---- FP64 reductions for CG and losses ----
 
 // block partial sums of Re<a,b> (b == nullptr: |a|^2)
@@ -944,11 +1294,35 @@ static void visit_grad(int L, int cb, F &&f) {
         default: visit_grad_cb<20>(cb, f); break;
     }
 }
+template <typename T16, bool HX, int L, class F>
+static void visit_gmixed_nkc(int nkc, F &&f) {
+    (void)nkc;   // 32 coils: 4 chunks (the residual fragments of 4 waves fill 64 KB of LDS)
+    f(k_gmixed<T16, HX, L, 4>);
+}
+template <typename T16, bool HX, class F>
+static void visit_gmixed_l(int L, int nkc, F &&f) {
+    switch (L) {
+        case 6: visit_gmixed_nkc<T16, HX, 6>(nkc, f); break;
+        case 10: visit_gmixed_nkc<T16, HX, 10>(nkc, f); break;
+        case 12: visit_gmixed_nkc<T16, HX, 12>(nkc, f); break;
+        case 16: visit_gmixed_nkc<T16, HX, 16>(nkc, f); break;
+        default: visit_gmixed_nkc<T16, HX, 20>(nkc, f); break;
+    }
+}
+// The MFMA gradient kernel of the handle's mixed backend (Horner phases when the runs allow).
+template <class F>
+static void visit_gmixed(Handle *h, F &&f) {
+    const int nkc = h->ncs / 8;
+    if (h->backend == HOE_BACKEND_MIXED_FP16) {
+        if (h->horner) visit_gmixed_l<_Float16, true>(h->LK, nkc, f);
+        else visit_gmixed_l<_Float16, false>(h->LK, nkc, f);
+    } else {
+        if (h->horner) visit_gmixed_l<__bf16, true>(h->LK, nkc, f);
+        else visit_gmixed_l<__bf16, false>(h->LK, nkc, f);
+    }
+}
 // Visit the contraction kernel of a backend and direction (the forward uses
 // the Horner variant when the handle's runs are long enough).
-// The contraction kernel of a backend and direction: the forward uses the
-// Horner variant when the runs are long enough (all backends), the mixed
-// adjoint when the run table exists.
 template <bool FWD, class F>
 static void visit_contract_direct(Handle *h, F &&f) {
     if constexpr (FWD) {
@@ -1088,6 +1462,48 @@ hipError_t prepare_grad(Handle *h) {
     hipError_t e = build_schedule(sg, (int)(h->ns_pad / 256), (int)(h->nv_pad / 8), slots);
     if (e != hipSuccess) return e;
     return grow(h->part_g, (size_t)sg.n_seg * nb * 256 * h->LK * sizeof(double));
+}
+
+static constexpr int kGradLds = 4 * 4 * 4 * 64 * 16;   // waves x chunks x (tile, hi/lo) x lanes x uint4
+
+bool grad_on_mfma(const Handle *h) { return h->backend != HOE_BACKEND_DIRECT && h->ncs == 32; }
+
+static hipError_t prepare_grad_mfma(Handle *h) {
+    hipError_t e;
+    Sched &sg = h->sched_gm;
+    if (!sg.built) {
+        int occ = 1;
+        visit_gmixed(h, [&](auto k) {
+            int nb = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, 256, kGradLds) == hipSuccess && nb > 0) occ = nb;
+        });
+        const int slots = std::max(8, h->cu_count * occ);
+        if ((e = build_schedule(sg, (int)(h->ns_pad / 256), (int)(h->nv_pad / 32), slots)) != hipSuccess) return e;
+        if ((e = grow(h->part_gm, (size_t)sg.n_seg * 256 * 32 * sizeof(float))) != hipSuccess) return e;
+        const size_t nkc = (size_t)h->ncs / 8;
+        if ((e = grow(h->gzfrag, (size_t)(h->nv_pad / 32) * nkc * 64 * 64)) != hipSuccess) return e;
+        if ((e = grow(h->grfrag, (size_t)(h->ns_pad / 32) * nkc * 64 * 32)) != hipSuccess) return e;
+        if ((e = grow(h->gbfrag, (size_t)(h->nv_pad / 32) * 2 * 64 * 32)) != hipSuccess) return e;
+        if ((e = grow(h->gbscale, 32 * sizeof(float))) != hipSuccess) return e;
+        if ((e = hipMemcpy(h->gbscale.p, h->gbscale_h.data(), 32 * sizeof(float), hipMemcpyHostToDevice)) !=
+            hipSuccess)
+            return e;
+    }
+    if (h->gb_type != h->backend) {
+        const int n_kb = (int)(h->nv_pad / 32);
+        const int64_t nt = (int64_t)n_kb * 2 * 64;
+        if (h->backend == HOE_BACKEND_MIXED_FP16)
+            k_build_gbfrag<_Float16><<<blocks_for(nt), 256, 0, h->stream>>>(
+                (const double *)h->basis.p, h->sb, h->t_index, (const float *)h->gbscale.p, (uint4 *)h->gbfrag.p,
+                n_kb, (int)h->n_vox);
+        else
+            k_build_gbfrag<__bf16><<<blocks_for(nt), 256, 0, h->stream>>>(
+                (const double *)h->basis.p, h->sb, h->t_index, (const float *)h->gbscale.p, (uint4 *)h->gbfrag.p,
+                n_kb, (int)h->n_vox);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        h->gb_type = h->backend;
+    }
+    return hipSuccess;
 }
 
 // ---- contraction launches ----
@@ -1237,8 +1653,50 @@ hipError_t run_normal_masked(Handle *h, const void *xm_in, void *xm_out, float s
     return adjoint_reduce(h, scale, xm_out, (float)lambda, lambda != 0.0 ? (const float2 *)xm_in : nullptr);
 }
 
+// Mixed backends with 32 coils: Z and residual fragments (slot 0 = max|xm|,
+// slot 1 = max|r| from the forward and its reduction), k_gmixed, reduction.
+static hipError_t run_grad_mfma(Handle *h, bool time_it) {
+    hipError_t e = prepare_grad_mfma(h);
+    if (e != hipSuccess) return e;
+    const Sched &sc = h->sched_gm;
+    const int nkc = h->ncs / 8;
+    const int n_kb = (int)(h->nv_pad / 32), n_st = (int)(h->ns_pad / 32);
+    const int64_t nz = (int64_t)n_kb * nkc * 64, nr = (int64_t)n_st * nkc * 64;
+    const unsigned *slots = slot(h, 0);
+    if (h->backend == HOE_BACKEND_MIXED_FP16) {
+        k_build_gzfrag<_Float16><<<blocks_for(nz), 256, 0, h->stream>>>(
+            (const float2 *)h->csm.p, (const float2 *)h->xm.p, (uint4 *)h->gzfrag.p, n_kb, nkc, h->ncs, slots,
+            h->csm_max);
+        k_build_grfrag<_Float16><<<blocks_for(nr), 256, 0, h->stream>>>(
+            (const float2 *)h->ybuf.p, (uint4 *)h->grfrag.p, n_st, nkc, h->ncs, slots, h->csm_max);
+    } else {
+        k_build_gzfrag<__bf16><<<blocks_for(nz), 256, 0, h->stream>>>(
+            (const float2 *)h->csm.p, (const float2 *)h->xm.p, (uint4 *)h->gzfrag.p, n_kb, nkc, h->ncs, slots,
+            h->csm_max);
+        k_build_grfrag<__bf16><<<blocks_for(nr), 256, 0, h->stream>>>(
+            (const float2 *)h->ybuf.p, (uint4 *)h->grfrag.p, n_st, nkc, h->ncs, slots, h->csm_max);
+    }
+    HornerArgs hx{(const double2 *)h->hrec.p, (const int *)h->vrun.p, (const double *)h->beta.p, h->t_index};
+    if (time_it) (void)hipEventRecord(h->ev[0], h->stream);
+    visit_gmixed(h, [&](auto k) {
+        k<<<sc.n_chunks, 256, kGradLds, h->stream>>>(
+            (const double *)h->coef.p, h->sk, (const double *)h->basis.p, h->sb, (const uint4 *)h->gzfrag.p,
+            (const uint4 *)h->grfrag.p, (const uint4 *)h->gbfrag.p, (float *)h->part_gm.p, (const int4 *)sc.seg.p,
+            (const int *)sc.chunk_off.p, sc.cpx, hx, slots, h->csm_max, h->ncs);
+    });
+    if (time_it) (void)hipEventRecord(h->ev[1], h->stream);
+    const double scale = -4.0 * M_PI / std::sqrt((double)h->n_vox);
+    k_reduce_gmixed<<<blocks_for(h->n_sam * h->LK), 256, 0, h->stream>>>(
+        (const float *)h->part_gm.p, (const int *)sc.tile_off.p, (const int *)sc.tile_list.p, h->LK, (int)h->n_sam,
+        scale, (const float *)h->gbscale.p, slots, h->csm_max, h->ncs, (double *)h->gk.p);
+    return hipGetLastError();
+}
+
 hipError_t run_grad_contract(Handle *h, bool time_it) {
-    // zbuf must hold csm .* xm, ybuf the residual (coil fastest)
+    if (grad_on_mfma(h)) return run_grad_mfma(h, time_it);
+    // direct kernel: zbuf = csm .* xm, ybuf = the residual (coil fastest)
+    hipError_t e = launch_build_zbuf(h, h->xm.p);
+    if (e != hipSuccess) return e;
     const Sched &sc = h->sched_g;
     const int nb = direct_batches(h);
     dim3 grid(sc.n_chunks, nb);
@@ -1262,6 +1720,18 @@ hipError_t launch_build_zbuf(Handle *h, const void *xm) {
     const int64_t nz = h->nv_pad * h->ncs;
     k_build_z<<<blocks_for(nz), 256, 0, h->stream>>>((const float2 *)h->csm.p, (const float2 *)xm,
                                                       (float2 *)h->zbuf.p, (int)h->nv_pad, h->ncs);
+    return hipGetLastError();
+}
+
+hipError_t launch_grad_chain(Handle *h, double *gtau_dev, bool shift, int m, const double w[4]) {
+    const int ns = (int)h->n_sam;
+    k_grad_chain<<<blocks_for((int64_t)ns * h->L), 256, 0, h->stream>>>(
+        (const double *)h->gk.p, ns, (const int *)h->growmap.p, h->L, shift ? 1 : 0, m, w[0], w[1], w[2], w[3],
+        (float *)h->ghbuf.p);
+    const int nb = 512;
+    k_gtau_partial<<<nb, 256, 0, h->stream>>>((const double *)h->gk.p, (const double *)h->dcoef.p, ns,
+                                              (int)h->active.size() + 1, h->sk, (double *)h->red.p);
+    k_cg_scalar<<<1, 256, 0, h->stream>>>((const double *)h->red.p, nb, gtau_dev, 0, nullptr, 0, 0.0);
     return hipGetLastError();
 }
 

@@ -15,7 +15,7 @@ import oracle as O  # noqa: E402
 import hoe  # noqa: E402
 from hoe import synth  # noqa: E402
 
-BACKENDS = ["direct", "mixed_fp16"]
+BACKENDS = ["direct", "mixed_fp16", "mixed_bf16"]
 
 
 def _small(seed=6):
@@ -75,7 +75,8 @@ def test_gradient_small_vs_oracle_and_fd(backend, gpu_lib):
         assert abs(gtau - fdt) <= 1e-4 * abs(fdt)
 
 
-def test_gradient_workload3_subset(gpu_lib):
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_gradient_workload3_subset(backend, gpu_lib):
     """Workload 3 geometry (200x200, 32 coils) on 512 spiral samples: y from
     perturbed coefficients and a 3 us delay; the gradient at the unperturbed
     coefficients (tau = 0) against the FP64 oracle gradient and central
@@ -89,11 +90,11 @@ def test_gradient_workload3_subset(gpu_lib):
     y = O.forward_f64(O.with_coefficients(mp, O.delayed(mp.k64.T, 3e-6, c["dwell"])), c["image"])
     y32 = y.astype(np.complex64)
     with hoe.HighOrderExplicitOp(c["grid"], c["kspha"], c["times"], fieldmap=c["fieldmap"], csm=c["csm"],
-                                 backend="direct", dwell=c["dwell"]) as op:
+                                 backend=backend, dwell=c["dwell"]) as op:
         g, gtau, loss = op.grad(c["image"], y32)
     g64, loss64, r64 = O.grad_f64(m, c["image"], y32.astype(np.complex128))
     e = np.linalg.norm(g - g64) / np.linalg.norm(g64)
-    print(f"workload3 subset: grad vs FP64 {e:.2e}, loss {loss:.6e} vs {loss64:.6e}")
+    print(f"workload3 subset {backend}: grad vs FP64 {e:.2e}, loss {loss:.6e} vs {loss64:.6e}")
     assert e < 1e-4
     rng = np.random.default_rng(64)
     gmax = np.max(np.abs(g64))
@@ -113,5 +114,5 @@ def test_gradient_workload3_subset(gpu_lib):
         fd = (vals[0] - vals[1]) / (2 * eps)
         err = abs(fd - g[l, s]) / max(abs(fd), 1e-3 * gmax)
         worst = max(worst, err)
-    print(f"workload3 subset: worst FD relative error over 64 coordinates {worst:.2e}")
+    print(f"workload3 subset {backend}: worst FD relative error over 64 coordinates {worst:.2e}")
     assert worst < 1e-4
