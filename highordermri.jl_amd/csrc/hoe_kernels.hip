@@ -339,15 +339,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (
                         ph[j] = fma(t, xd.y, fma(fma(fma(hc[3], xd.x, hc[2]), xd.x, hc[1]), xd.x, hc[0]));
                     }
                 } else {
+                    // a run boundary inside the K step: generic phase from the full records
+                    const double *ap = coef + (size_t)row * sk;
+                    double a[L + 1];
+#pragma unroll
+                    for (int i = 0; i <= L; ++i) a[i] = ap[i];
 #pragma unroll
                     for (int j = 0; j < 8; ++j) {
-                        const int r = hx.vrun[q0 + j];
-                        if (r != cur) {
-                            run_setup<L>(coef + (size_t)row * sk, hx.beta + (size_t)r * L * 4, hc);
-                            cur = r;
-                        }
-                        const double2 xd = hx.hrec[q0 + j];
-                        ph[j] = fma(t, xd.y, fma(fma(fma(hc[3], xd.x, hc[2]), xd.x, hc[1]), xd.x, hc[0]));
+                        const double *sm = basis + (size_t)(q0 + j) * sb;
+                        double p = a[0];
+#pragma unroll
+                        for (int i = 1; i <= L; ++i) p = fma(a[i], sm[i - 1], p);
+                        ph[j] = p;
                     }
                 }
             });
