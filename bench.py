@@ -252,8 +252,7 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f16 hi/lo MFMA (f32 acc), f64 phase, f32 sincos",
         "data": "synthetic",
-        "config": {"workload": f"workload {args.workload}: 2D 200x200 FOV 200 mm, 32 coils, 16000-sample "
-                               f"spiral R=4 at 1 us, 16 solid-harmonic terms + dB0", "n_vox": nv,
+        "config": {"workload": synth.describe(args.workload), "n_vox": nv,
                    "n_sam": ns, "n_cha": nc, "n_term": L, "weights": "unit"},
         "fp32_direct": {"value": round(fp32_value, 3), "unit": "E^H E applies/s",
                         "ms_per_step": round(1e3 * t_direct / steps_d, 4), "steps": steps_d,

@@ -216,6 +216,15 @@ int check_open(Handle *h) {
     return HOE_OK;
 }
 
+// The derivative model (kspha_dt) defines only the forward product.
+int reject_dt(Handle *h) {
+    return h->dt_on ? set_err(h, HOE_EINVAL,
+                              "only the forward product is defined while kspha_dt is set "
+                              "(array_operator.jl:153-172); clear it with hoe_set_kspha_dt(h, NULL)")
+                    : HOE_OK;
+}
+
+
 // Upload normal weights D (already squared) into dvec [ns_pad][ncs].
 int set_weights(Handle *h, const float *D, int64_t n_d) {
     const int64_t ns = h->n_sam, nc = h->n_cha;
@@ -255,7 +264,7 @@ void release(Handle *h) {
                               &h->hrec,     &h->vrun,   &h->beta,   &h->zfrag,   &h->ybuf,   &h->yfrag,  &h->part_f,
                               &h->part_a,   &h->part_g, &h->dvec,   &h->io_x,    &h->io_x2,  &h->io_y,   &h->io_y2,
                               &h->scal,     &h->cg_vec, &h->red,    &h->gk,      &h->cg_log, &h->gzfrag, &h->grfrag,
-                              &h->gbfrag,   &h->gbscale, &h->part_gm, &h->ghbuf, &h->growmap};
+                              &h->gbfrag,   &h->gbscale, &h->part_gm, &h->ghbuf, &h->growmap, &h->dtrec};
     for (auto *b : bufs) dfree(*b);
     hoe::Sched *scheds[] = {&h->sched_f[0], &h->sched_f[1], &h->sched_f[2], &h->sched_a[0],
                             &h->sched_a[1], &h->sched_a[2], &h->sched_g,    &h->sched_gm};
@@ -345,6 +354,7 @@ int hoe_create(const hoe_model_desc *d, hoe_handle **out) {
     h->n_cha = d->n_cha;
     h->n_term_in = n_term;
     h->L = n_term + (d->concomitant ? 4 : 0);
+    h->has_conc = d->concomitant != nullptr;
     h->nv_pad = round_up(h->n_vox, hoe::kRowPad);
     h->ns_pad = round_up(h->n_sam, hoe::kRowPad);
     // coil stride: one 32-coil MFMA group, or whole pairs of groups (the mixed kernels keep two
@@ -515,6 +525,7 @@ int hoe_create(const hoe_model_desc *d, hoe_handle **out) {
         {&h->basis, sizeof(double) * brec.size()},
         {&h->coef, sizeof(double) * (size_t)h->ns_pad * h->sk},
         {&h->dcoef, sizeof(double) * (size_t)h->ns_pad * h->sk},
+        {&h->dtrec, sizeof(double) * (size_t)h->ns_pad * h->sk},
         {&h->ghbuf, sizeof(float) * (size_t)ns * L},
         {&h->growmap, sizeof(int32_t) * (size_t)L},
         {&h->csm, sizeof(float2) * csmv.size()},
@@ -678,6 +689,7 @@ int hoe_normal(hoe_handle *hh, const float *D, int64_t n_d, const void *x, void 
     Handle *h = reinterpret_cast<Handle *>(hh);
     int rc = check_open(h);
     if (rc) return rc;
+    if ((rc = reject_dt(h))) return rc;
     if (!out) return set_err(h, HOE_EINVAL, "output pointer is NULL");
     if (!(D == nullptr && n_d == kCurrentWeights))
         if ((rc = set_weights(h, D, n_d))) return rc;
@@ -706,10 +718,42 @@ int hoe_set_coefficients(hoe_handle *hh, const float *kspha, double tau) {
     return upload_coef(h);
 }
 
+int hoe_set_kspha_dt(hoe_handle *hh, const float *kspha_dt) {
+    Handle *h = reinterpret_cast<Handle *>(hh);
+    int rc = check_open(h);
+    if (rc) return rc;
+    if (!kspha_dt) {
+        h->dt_on = false;
+        return HOE_OK;
+    }
+    if (h->has_conc)
+        return set_err(h, HOE_EINVAL,
+                       "kspha_dt cannot be combined with concomitant terms (the derivative model has none)");
+    const int64_t ns = h->n_sam;
+    const int n_term = h->n_term_in;
+    for (int64_t i = 0; i < ns * n_term; ++i)
+        if (!std::isfinite(kspha_dt[i])) return set_err(h, HOE_EINVAL, "kspha_dt must be finite");
+    // records [kdt_1, kdt of the active terms, 0 (t slot), 0...]: terms whose basis is
+    // identically zero on the masked voxels contribute nothing
+    std::vector<double> rec((size_t)h->ns_pad * h->sk, 0.0);
+    for (int64_t s = 0; s < ns; ++s) {
+        double *r = &rec[(size_t)s * h->sk];
+        const float *k = kspha_dt + (size_t)s * n_term;
+        r[0] = (double)k[0];
+        for (size_t i = 0; i < h->active.size(); ++i) r[1 + i] = (double)k[h->active[i]];
+    }
+    HOE_TRY_HIP(h, hipStreamSynchronize(h->stream));
+    HOE_TRY_HIP(h, hipMemcpy(h->dtrec.p, rec.data(), rec.size() * sizeof(double), hipMemcpyHostToDevice));
+    HOE_TRY_HIP(h, hoe::prepare_backend(h, HOE_BACKEND_DIRECT));
+    h->dt_on = true;
+    return HOE_OK;
+}
+
 int hoe_grad(hoe_handle *hh, const void *x, const void *y, float *g_h, double *g_tau, double *loss, int flags) {
     Handle *h = reinterpret_cast<Handle *>(hh);
     int rc = check_open(h);
     if (rc) return rc;
+    if ((rc = reject_dt(h))) return rc;
     (void)hipEventRecord(h->ev[4], h->stream);
     const void *dx, *dy;
     if ((rc = stage_in(h, flags, x, h->io_x, sizeof(float2) * h->n_grid, &dx))) return rc;
@@ -751,6 +795,7 @@ int hoe_cg(hoe_handle *hh, const void *y, const float *D, int64_t n_d, int32_t i
     Handle *h = reinterpret_cast<Handle *>(hh);
     int rc = check_open(h);
     if (rc) return rc;
+    if ((rc = reject_dt(h))) return rc;
     if (!x_out) return set_err(h, HOE_EINVAL, "output pointer is NULL");
     if (iters < 0) return set_err(h, HOE_EINVAL, "iterations must be nonnegative");
     if (!std::isfinite(lambda) || lambda < 0.0) return set_err(h, HOE_EINVAL, "lambda must be finite and >= 0");

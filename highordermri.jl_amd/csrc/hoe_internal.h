@@ -56,6 +56,8 @@ struct Handle {
     int32_t cu_count = 256;
     bool closed = false;
     bool weights_unit = false;        // dvec holds unit weights
+    bool dt_on = false;               // hoe_forward computes the derivative product B x (kspha_dt set)
+    bool has_conc = false;            // the model carries the 4 concomitant terms
     float csm_max = 0.0f;             // max |Re|,|Im| bound of the masked coil maps: max |csm|
     double dwell = 0.0, tau = 0.0;
     std::string err;
@@ -72,6 +74,7 @@ struct Handle {
     DeviceBuf basis;      // double [nv_pad][sb]: b of the active terms, fieldmap
     DeviceBuf coef;       // double [ns_pad][sk]: k_1, k of the active terms, t
     DeviceBuf dcoef;      // double [ns_pad][sk] d k / d tau (for the delay gradient)
+    DeviceBuf dtrec;      // double [ns_pad][sk] kspha_dt records (derivative product), t slot 0
     DeviceBuf csm;        // float2 [nv_pad][ncs]
     DeviceBuf hrec;       // double2 [nv_pad]: (x, fieldmap) of each voxel, for the Horner forward
     DeviceBuf vrun;       // int [nv_pad]: run of each voxel

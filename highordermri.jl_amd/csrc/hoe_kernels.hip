@@ -162,49 +162,68 @@ struct mfma_t<__bf16> {
 // Each lane keeps CB complex accumulators; the streamed record and the CB
 // complex operands of the streamed row arrive through scalar loads. A K unit
 // is 8 streamed rows; rows at or beyond n_stream (padding) are skipped.
-template <bool FWD, bool HX, int L, int CB>
+//
+// DT (forward only): the trajectory-derivative product B x of
+// array_operator.jl:187-235, e(s,v) replaced by 2 pi i psi(s,v) e(s,v) with
+// psi = sum_l kdt_l(s) b_l(v) from the derivative records dtrec (same layout
+// as coef, t slot 0), evaluated like the phase.
+template <bool FWD, bool HX, int L, int CB, bool DT = false>
 __global__ __launch_bounds__(256) void k_direct(
     const double *__restrict__ coef, int sk, const double *__restrict__ basis, int sb,
     const float2 *__restrict__ src, float2 *__restrict__ part, int ncs, int n_stream,
-    const int4 *__restrict__ segs, const int *__restrict__ chunk_off, int cpx, HornerArgs hx) {
+    const int4 *__restrict__ segs, const int *__restrict__ chunk_off, int cpx, HornerArgs hx,
+    const double *__restrict__ dtrec) {
+    static_assert(FWD || !DT, "the derivative product is a forward product");
     const int chunk = chunk_id(cpx);
     const int cb0 = blockIdx.y * CB;
     const int sg0 = chunk_off[chunk], sg1 = chunk_off[chunk + 1];
     constexpr int NST = HX ? 1 : (FWD ? L + 1 : L);
+    constexpr int NSD = DT ? (HX ? 1 : L + 1) : 1;
     for (int sgi = sg0; sgi < sg1; ++sgi) {
         const int4 d = segs[sgi];
         const int row = d.x * 256 + threadIdx.x;
-        double st[NST];
+        double st[NST], sd[NSD];
         if (HX) {
             st[0] = coef[(size_t)row * sk + hx.t_index];
         } else {
             const double *p = FWD ? coef + (size_t)row * sk : basis + (size_t)row * sb;
 #pragma unroll
             for (int i = 0; i < NST; ++i) st[i] = p[i];
+            if (DT) {
+#pragma unroll
+                for (int i = 0; i < NSD; ++i) sd[i] = dtrec[(size_t)row * sk + i];
+            }
         }
         float ar[CB], ai[CB];
 #pragma unroll
         for (int c = 0; c < CB; ++c) ar[c] = ai[c] = 0.0f;
         const int q0 = d.y * 8;
         const int q1 = min(d.z * 8, n_stream);
-        double hc[4] = {0.0, 0.0, 0.0, 0.0};
+        double hc[4] = {0.0, 0.0, 0.0, 0.0}, hd[4] = {0.0, 0.0, 0.0, 0.0};
         int cur = -1;
         for (int q = q0; q < q1; ++q) {
             const double *sm = FWD ? basis + (size_t)q * sb : coef + (size_t)q * sk;
-            double ph;
+            double ph, psi = 0.0;
             if (HX) {
                 const int r = hx.vrun[q];
                 if (r != cur) {
                     run_setup<L>(coef + (size_t)row * sk, hx.beta + (size_t)r * L * 4, hc);
+                    if (DT) run_setup<L>(dtrec + (size_t)row * sk, hx.beta + (size_t)r * L * 4, hd);
                     cur = r;
                 }
                 const double2 xd = hx.hrec[q];
                 ph = fma(fma(fma(hc[3], xd.x, hc[2]), xd.x, hc[1]), xd.x, hc[0]);
                 ph = fma(st[0], xd.y, ph);
+                if (DT) psi = fma(fma(fma(hd[3], xd.x, hd[2]), xd.x, hd[1]), xd.x, hd[0]);
             } else if (FWD) {
                 ph = st[0];
 #pragma unroll
                 for (int i = 1; i <= L; ++i) ph = fma(st[i], sm[i - 1], ph);
+                if (DT) {
+                    psi = sd[0];
+#pragma unroll
+                    for (int i = 1; i <= L; ++i) psi = fma(sd[i], sm[i - 1], psi);
+                }
             } else {
                 ph = sm[L] * st[L - 1];
 #pragma unroll
@@ -214,6 +233,13 @@ __global__ __launch_bounds__(256) void k_direct(
             float sn, cs;
             sincos_cycles(ph, sn, cs);
             if (!FWD) sn = -sn;   // conj(e)
+            if (DT) {
+                // 2 pi i psi e = 2 pi psi (-sin + i cos)
+                const float w = (float)(6.283185307179586 * psi);
+                const float c2 = -sn * w;
+                sn = cs * w;
+                cs = c2;
+            }
             const float2 *w = src + (size_t)q * ncs + cb0;
 #pragma unroll
             for (int c = 0; c < CB; ++c) {
@@ -1229,6 +1255,9 @@ __global__ void k_copy2(float2 *__restrict__ d, const float2 *__restrict__ s, in
 // ---------------------------------------------------------------------------
 
 static inline unsigned *slot(Handle *h, int i) { return reinterpret_cast<unsigned *>(h->scal.p) + 64 + i; }
+
+// Backend of the forward product: the derivative product (kspha_dt set) runs on the FP32 kernels.
+static inline int fwd_backend(const Handle *h) { return h->dt_on ? HOE_BACKEND_DIRECT : h->backend; }
 static inline double *cgs(Handle *h) { return reinterpret_cast<double *>(h->scal.p); }
 
 static inline int blocks_for(int64_t n, int b = 256) { return (int)((n + b - 1) / b); }
@@ -1250,20 +1279,20 @@ static void zero_slots(Handle *h) { k_zero_u32<<<1, 64, 0, h->stream>>>(slot(h, 
 
 // ---- kernel instances over (L, coil batch): visitors hand the instance to a launcher ----
 
-template <bool FWD, bool HX, int L, class F>
+template <bool FWD, bool HX, int L, bool DT, class F>
 static void visit_direct_cb(int cb, F &&f) {
-    if (cb == 8) f(k_direct<FWD, HX, L, 8>);
-    else if (cb == 16) f(k_direct<FWD, HX, L, 16>);
-    else f(k_direct<FWD, HX, L, 32>);
+    if (cb == 8) f(k_direct<FWD, HX, L, 8, DT>);
+    else if (cb == 16) f(k_direct<FWD, HX, L, 16, DT>);
+    else f(k_direct<FWD, HX, L, 32, DT>);
 }
-template <bool FWD, bool HX, class F>
+template <bool FWD, bool HX, bool DT = false, class F>
 static void visit_direct(int L, int cb, F &&f) {
     switch (L) {
-        case 6: visit_direct_cb<FWD, HX, 6>(cb, f); break;
-        case 10: visit_direct_cb<FWD, HX, 10>(cb, f); break;
-        case 12: visit_direct_cb<FWD, HX, 12>(cb, f); break;
-        case 16: visit_direct_cb<FWD, HX, 16>(cb, f); break;
-        default: visit_direct_cb<FWD, HX, 20>(cb, f); break;
+        case 6: visit_direct_cb<FWD, HX, 6, DT>(cb, f); break;
+        case 10: visit_direct_cb<FWD, HX, 10, DT>(cb, f); break;
+        case 12: visit_direct_cb<FWD, HX, 12, DT>(cb, f); break;
+        case 16: visit_direct_cb<FWD, HX, 16, DT>(cb, f); break;
+        default: visit_direct_cb<FWD, HX, 20, DT>(cb, f); break;
     }
 }
 template <typename T16, bool FWD, bool HX, int L, class F>
@@ -1329,8 +1358,14 @@ static void visit_gmixed(Handle *h, F &&f) {
 template <bool FWD, class F>
 static void visit_contract_direct(Handle *h, F &&f) {
     if constexpr (FWD) {
-        if (h->horner) visit_direct<true, true>(h->LK, direct_cb(h), f);
-        else visit_direct<true, false>(h->LK, direct_cb(h), f);
+        if (h->dt_on) {
+            if (h->horner) visit_direct<true, true, true>(h->LK, direct_cb(h), f);
+            else visit_direct<true, false, true>(h->LK, direct_cb(h), f);
+        } else if (h->horner) {
+            visit_direct<true, true>(h->LK, direct_cb(h), f);
+        } else {
+            visit_direct<true, false>(h->LK, direct_cb(h), f);
+        }
     } else {
         visit_direct<false, false>(h->LK, direct_cb(h), f);
     }
@@ -1513,7 +1548,7 @@ static hipError_t prepare_grad_mfma(Handle *h) {
 
 template <bool FWD>
 static void launch_contract(Handle *h, const void *operand, float2 *part) {
-    const int backend = h->backend;
+    const int backend = FWD ? fwd_backend(h) : h->backend;
     const Sched &sc = FWD ? h->sched_f[backend] : h->sched_a[backend];
     const double *coef = (const double *)h->coef.p, *basis = (const double *)h->basis.p;
     const int4 *segs = (const int4 *)sc.seg.p;
@@ -1524,7 +1559,7 @@ static void launch_contract(Handle *h, const void *operand, float2 *part) {
         const int n_stream = (int)(FWD ? h->n_vox : h->n_sam);
         visit_contract_direct<FWD>(h, [&](auto k) {
             k<<<grid, 256, 0, h->stream>>>(coef, h->sk, basis, h->sb, (const float2 *)operand, part, h->ncs,
-                                           n_stream, segs, coff, sc.cpx, hx);
+                                           n_stream, segs, coff, sc.cpx, hx, (const double *)h->dtrec.p);
         });
     } else {
         dim3 grid(sc.n_chunks, mixed_groups(h));
@@ -1538,7 +1573,7 @@ static void launch_contract(Handle *h, const void *operand, float2 *part) {
 // Builds the forward operand from masked xm (device) and runs the forward
 // contraction into part_f.
 static hipError_t forward_from_xm(Handle *h, const float2 *xm, bool time_it) {
-    if (h->backend == HOE_BACKEND_DIRECT) {
+    if (fwd_backend(h) == HOE_BACKEND_DIRECT) {
         const int64_t nz = h->nv_pad * h->ncs;
         k_build_z<<<blocks_for(nz), 256, 0, h->stream>>>((const float2 *)h->csm.p, xm, (float2 *)h->zbuf.p,
                                                           (int)h->nv_pad, h->ncs);
@@ -1573,11 +1608,12 @@ hipError_t run_forward_contract(Handle *h, const void *x_full, bool time_it) {
 static inline bool mixed(Handle *h) { return h->backend != HOE_BACKEND_DIRECT; }
 
 hipError_t launch_reduce_fwd_user(Handle *h, void *y_out, float scale) {
-    const Sched &sc = h->sched_f[h->backend];
+    const Sched &sc = h->sched_f[fwd_backend(h)];
+    const bool unscale = fwd_backend(h) != HOE_BACKEND_DIRECT;
     dim3 grid((unsigned)(h->ns_pad / 32), (unsigned)(h->ncs / 32));
     k_reduce_fwd<<<grid, 256, 0, h->stream>>>((const float2 *)h->part_f.p, (const int *)sc.tile_off.p,
                                               (const int *)sc.tile_list.p, h->ncs, (int)h->n_sam, (int)h->n_cha,
-                                              scale, mixed(h) ? slot(h, 0) : nullptr, h->csm_max, 0,
+                                              scale, unscale ? slot(h, 0) : nullptr, h->csm_max, 0,
                                               (float2 *)y_out, nullptr, nullptr, nullptr);
     return hipGetLastError();
 }

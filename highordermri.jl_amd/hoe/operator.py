@@ -95,8 +95,6 @@ class HighOrderExplicitOp:
         if implementation == "array":
             raise ArgumentError("implementation=:array (the reference CPU array path) is not provided; "
                                 "use implementation=:kernel")
-        if kspha_dt is not None:
-            raise ArgumentError("kspha_dt requires backend=:legacy")
         if backend not in _lib.BACKENDS:
             raise ArgumentError(f"unknown explicit backend {backend}")
         gpus = [0] if gpus is None else [int(g) for g in gpus]
@@ -143,6 +141,13 @@ class HighOrderExplicitOp:
                 raise DimensionMismatch("csm and grid")
             cs = np.ascontiguousarray(c.reshape(n_grid, n_cha, order="F") if c.ndim > 2 else c)
             cs = np.ascontiguousarray(cs.T)   # [coil][grid]: grid fastest per coil
+        kdt = None
+        if kspha_dt is not None:
+            kdt = _f32(kspha_dt, "kspha_dt")
+            if kdt.shape != kspha.shape:
+                raise AssertionError("kspha_dt must have same size as kspha")
+            # the derivative operator carries no concomitant terms (array_operator.jl:139)
+            concomitant = None
         conc = None
         if concomitant is not None:
             ca = np.asarray(concomitant)
@@ -204,6 +209,9 @@ class HighOrderExplicitOp:
         self.backend = backend
         self.symmetric = self.hermitian = False
         self._weights_owner = None
+        self.kspha_dt = None
+        if kdt is not None:
+            self.set_kspha_dt(kdt)
 
     # ---- LinearOperator protocol ----
     @property
@@ -227,8 +235,25 @@ class HighOrderExplicitOp:
         check(self._lib.hoe_set_backend(self._handle(), _lib.BACKENDS[backend]), self._h)
         self.backend = backend
 
+    def set_kspha_dt(self, kspha_dt):
+        """Set (``(nTerm, nSam)`` array) or clear (``None``) the trajectory derivative:
+        while set, ``op * x`` is the derivative product B x (array_operator.jl:187-235)
+        and ``op' * y`` stays E^H y, as for the reference's ``kspha_dt`` operator."""
+        h = self._handle()
+        if kspha_dt is None:
+            check(self._lib.hoe_set_kspha_dt(h, None), h)
+            self.kspha_dt = None
+            return
+        kdt = _f32(kspha_dt, "kspha_dt")
+        if kdt.shape != (self.nTerm, self.nSam):
+            raise AssertionError("kspha_dt must have same size as kspha")
+        kt = np.ascontiguousarray(kdt.T)          # term fastest
+        check(self._lib.hoe_set_kspha_dt(h, kt.ctypes.data), h)
+        self.kspha_dt = kdt
+
     def prod(self, x, out=None):
-        """``op * x``: x of length prod(matrixSize) -> signal of length nSam*nCha."""
+        """``op * x``: x of length prod(matrixSize) -> signal of length nSam*nCha
+        (B x while ``kspha_dt`` is set)."""
         h = self._handle()
         xv = _Vec(x, self.ncol, "x")
         y = out if out is not None else xv.like(self.nrow)

@@ -194,6 +194,23 @@ def make(config: int, *, n_cha: int | None = None, n_sam: int | None = None, see
     return out
 
 
+DESCRIPTIONS = {
+    0: "2D 64x64 FOV 220 mm, 8 coils, 6000-sample Archimedean spiral at 2 us, 16 solid-harmonic terms + dB0",
+    1: "2D 200x200 FOV 200 mm, 32 coils, 16000-sample spiral R=4 at 1 us, 16 solid-harmonic terms + dB0",
+    2: "2D 200x200 FOV 200 mm off-centre slice, 32 coils, single-shot EPI R=4 (50 x 400 samples at 1 us), "
+       "16 terms + 4 concomitant terms (7 T, 40 mT/m) + dB0",
+    3: "2D 200x200 FOV 200 mm, 32 coils, 16000-sample spiral R=4 at 1 us, 16 terms + dB0; "
+       "data from perturbed coefficients and a 3 us delay",
+    4: "3D 96x96x48 FOV 220x220x110 mm, 32 coils, stack of 48 spirals x 8000 samples at 1 us, "
+       "16 solid-harmonic terms + dB0",
+}
+
+
+def describe(config: int) -> str:
+    """One-line description of workload ``config``."""
+    return f"workload {config}: {DESCRIPTIONS[config]}"
+
+
 def perturb(cfg: dict, seed: int = 7, order1: float = 0.01, order23: float = 0.05):
     """Perturbed coefficients for the differentiation workload: seeded noise of
     ``order1`` relative amplitude on rows 2-4 and ``order23`` on rows 5-16."""

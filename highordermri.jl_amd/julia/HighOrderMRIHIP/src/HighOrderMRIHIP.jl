@@ -45,7 +45,10 @@ end
 
 function check(rc::Integer, h::Ptr{Cvoid}=C_NULL)
     rc == 0 && return nothing
-    msg = unsafe_string(ccall((:hoe_last_error, libhoe), Cstring, (Ptr{Cvoid},), h))
+    throw_status(rc, unsafe_string(ccall((:hoe_last_error, libhoe), Cstring, (Ptr{Cvoid},), h)))
+end
+
+function throw_status(rc::Integer, msg::AbstractString)
     rc == 1 && throw(ArgumentError(msg))
     rc == 2 && throw(DimensionMismatch(msg))
     rc == 3 && throw(AssertionError(msg))
@@ -82,7 +85,7 @@ LinearOperators.storage_type(op::HIPExplicitOp) = typeof(op.Mv)
 handle(op::HIPExplicitOp) = op.handle == C_NULL ? throw(ArgumentError("operator is closed")) : op.handle
 
 """
-    HIPExplicitOp(grid, kspha, times; fieldmap, csm, mask, concomitant,
+    HIPExplicitOp(grid, kspha, times; fieldmap, csm, mask, concomitant, kspha_dt,
                   recon_terms, k_nominal, gpus=[0], backend=:mixed, dwell=0)
 
 Same inputs and conventions as `HighOrderExplicitOp(...; implementation=:kernel)`
@@ -90,12 +93,22 @@ Same inputs and conventions as `HighOrderExplicitOp(...; implementation=:kernel)
 seconds, `fieldmap` in Hz, `csm` `(nX,nY,nZ,nCha)`, `mask` Bool, `concomitant`
 `(4, nSam)`. `backend` is `:direct` (FP32 VALU), `:mixed`/`:mixed_fp16` or
 `:mixed_bf16` (MFMA). `dwell` is the coefficient sampling interval used by delays.
+
+With `kspha_dt` (`size(kspha)`, dk/dt per term) the operator is the derivative
+operator of `explicit_array_derivative` (array_operator.jl:187-235, as built by
+FindDelay.jl:184-187): `op * x` is `B x`, `op' * y` stays `Eᴴ y`, concomitant
+terms are dropped (array_operator.jl:139) and `normalOperator`/`hip_explicit_grad`
+raise `ArgumentError`.
 """
 function HIPExplicitOp(grid::Grid{Float32}, kspha::AbstractMatrix{Float32}, times::AbstractVector{Float32};
                        fieldmap=zeros(Float32, grid.matrixSize...), csm=ones(ComplexF32, grid.matrixSize..., 1),
-                       mask=trues(grid.matrixSize...), concomitant=nothing, recon_terms=nothing,
-                       k_nominal=nothing, gpus=[0], backend::Symbol=:mixed, dwell::Real=0)
+                       mask=trues(grid.matrixSize...), concomitant=nothing, kspha_dt=nothing,
+                       recon_terms=nothing, k_nominal=nothing, gpus=[0], backend::Symbol=:mixed, dwell::Real=0)
     length(gpus) == 1 || throw(ArgumentError("HIPExplicitOp runs on one GPU"))
+    if kspha_dt !== nothing
+        @assert size(kspha_dt) == size(kspha) "kspha_dt must have same size as kspha"
+        concomitant = nothing
+    end
     haskey(BACKENDS, backend) || throw(ArgumentError("unknown explicit backend $backend"))
     nTerm, nSam = size(kspha)
     nCha = size(csm)[end]
@@ -115,6 +128,16 @@ function HIPExplicitOp(grid::Grid{Float32}, kspha::AbstractMatrix{Float32}, time
                       conc === nothing ? C_NULL : pointer(conc), rt === nothing ? C_NULL : pointer(rt),
                       kn === nothing ? C_NULL : pointer(kn), Float64(dwell), BACKENDS[backend], Int32(first(gpus)))
         check(ccall((:hoe_create, libhoe), Cint, (Ref{ModelDesc}, Ref{Ptr{Cvoid}}), d, h))
+    end
+    if kspha_dt !== nothing
+        kdt = Matrix{Float32}(kspha_dt)
+        rc = GC.@preserve kdt ccall((:hoe_set_kspha_dt, libhoe), Cint, (Ptr{Cvoid}, Ptr{Float32}),
+                                    h[], pointer(kdt))
+        if rc != 0
+            msg = unsafe_string(ccall((:hoe_last_error, libhoe), Cstring, (Ptr{Cvoid},), h[]))
+            ccall((:hoe_destroy, libhoe), Cvoid, (Ptr{Cvoid},), h[])
+            throw_status(rc, msg)
+        end
     end
     nTermEff = nTerm + (concomitant === nothing ? 0 : 4)
     op = HIPExplicitOp(nSam * nCha, prod(grid.matrixSize), false, false, nothing, nothing, nothing,

@@ -167,6 +167,18 @@ int hoe_set_weights(hoe_handle *h, const float *D, int64_t n_d);
  * model (FindDelay.jl:179-201, array_operator.jl:187-235). Host pointers. */
 int hoe_set_coefficients(hoe_handle *h, const float *kspha, double tau);
 
+/* Set (kspha_dt != NULL) or clear (NULL) the trajectory-derivative
+ * coefficients dk/dt: n_term x n_sam floats (term fastest, the kspha input
+ * layout, not processed by recon_terms). While set, hoe_forward computes the
+ * derivative product B x = 1/sqrt(nVox) sum_v 2 pi i (sum_l kdt_l(s) b_l(v))
+ * exp(2 pi i phi) csm x (explicit_array_derivative, array_operator.jl:187-235;
+ * delay estimation, FindDelay.jl:139-214) on the FP32 kernels, hoe_adjoint
+ * stays E^H (the reference's ctprod, array_operator.jl:167-171), and
+ * hoe_normal / hoe_cg / hoe_grad return HOE_EINVAL. A model with
+ * concomitant terms returns HOE_EINVAL: the reference builds the derivative
+ * operator without them (array_operator.jl:139). Host pointer. */
+int hoe_set_kspha_dt(hoe_handle *h, const float *kspha_dt);
+
 /* Gradient of L(h,tau) = ||E(h,tau) x - y||^2.
  * x: n_grid complex, y: n_sam*n_cha complex (flags as for hoe_forward).
  * g_h: n_term_eff*n_sam floats, dL/dh (host); g_tau: dL/dtau (seconds^-1);

@@ -176,6 +176,49 @@ void oracle_forward_f64(int64_t nv, int64_t ns, int64_t nc, int L, const double 
     free(z);
 }
 
+/* Derivative product (explicit_array_derivative, array_operator.jl:187-235):
+ * y[c][s] = 1/sqrt(nv) sum_v exp(2 pi i phi) * 2 pi i (sum_l kdt[s][l] b[v][l]) * csm[c][v] x[v]
+ * (the t * fieldmap term of phi is not differentiated). */
+void oracle_forward_dt_f64(int64_t nv, int64_t ns, int64_t nc, int L, const double *basis, const double *coef,
+                           const double *kdt, const double *times, const double *fieldmap, const double *csm,
+                           const double *x, double *y) {
+    const double scale = 1.0 / sqrt((double)nv);
+    double *z = (double *)malloc(sizeof(double) * 2 * nv * nc);
+    for (int64_t c = 0; c < nc; ++c)
+        for (int64_t v = 0; v < nv; ++v) {
+            const double ar = csm[2 * (c * nv + v)], ai = csm[2 * (c * nv + v) + 1];
+            const double br = x[2 * v], bi = x[2 * v + 1];
+            z[2 * (v * nc + c)] = ar * br - ai * bi;
+            z[2 * (v * nc + c) + 1] = ar * bi + ai * br;
+        }
+#pragma omp parallel
+    {
+        double *acc = (double *)malloc(sizeof(double) * 2 * nc);
+#pragma omp for schedule(static)
+        for (int64_t s = 0; s < ns; ++s) {
+            memset(acc, 0, sizeof(double) * 2 * nc);
+            for (int64_t v = 0; v < nv; ++v) {
+                const double ph = phase64(basis + v * L, coef + s * L, L, times[s], fieldmap[v]);
+                const double psi = phase64(basis + v * L, kdt + s * L, L, 0.0, 0.0);
+                const double a = TWO_PI * ph;
+                /* 2 pi i psi (cos a + i sin a) = 2 pi psi (-sin a + i cos a) */
+                const double er = -TWO_PI * psi * sin(a), ei = TWO_PI * psi * cos(a);
+                const double *zv = z + 2 * v * nc;
+                for (int64_t c = 0; c < nc; ++c) {
+                    acc[2 * c] += er * zv[2 * c] - ei * zv[2 * c + 1];
+                    acc[2 * c + 1] += er * zv[2 * c + 1] + ei * zv[2 * c];
+                }
+            }
+            for (int64_t c = 0; c < nc; ++c) {
+                y[2 * (c * ns + s)] = acc[2 * c] * scale;
+                y[2 * (c * ns + s) + 1] = acc[2 * c + 1] * scale;
+            }
+        }
+        free(acc);
+    }
+    free(z);
+}
+
 /* x[v] = 1/sqrt(nv) sum_c conj(csm[c][v]) sum_s conj(exp(2 pi i phi)) y[c][s] */
 void oracle_adjoint_f64(int64_t nv, int64_t ns, int64_t nc, int L, const double *basis, const double *coef,
                         const double *times, const double *fieldmap, const double *csm, const double *y,

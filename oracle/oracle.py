@@ -44,6 +44,7 @@ def lib():
         for n in ("oracle_forward_f64", "oracle_adjoint_f64"):
             getattr(L, n).argtypes = [i64, i64, i64, ci, vp, vp, vp, vp, vp, vp, vp]
         L.oracle_grad_f64.argtypes = [i64, i64, i64, ci, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.oracle_forward_dt_f64.argtypes = [i64, i64, i64, ci, vp, vp, vp, vp, vp, vp, vp, vp]
         for n in ("cpu_ref_forward", "cpu_ref_adjoint"):
             getattr(L, n).argtypes = [i64, i64, i64, ci, vp, vp, vp, vp, vp, vp, vp, i64]
         L.oracle_num_threads.restype = ci
@@ -194,6 +195,21 @@ def forward_f64(m: Model, x_full):
     y = np.empty(m.nc * m.ns, dtype=np.complex128)
     t, fm = _f64(m.times), _f64(m.fieldmap)      # keep the temporaries alive during the call
     lib().oracle_forward_f64(m.nv, m.ns, m.nc, m.L, _p(m.bf64), _p(m.k64), _p(t), _p(fm), _p(cs), _p(xm), _p(y))
+    return y
+
+
+def forward_dt_f64(m: Model, kspha_dt, x_full):
+    """Derivative product B x in FP64 -> [nc*ns] (array_operator.jl:187-235). ``kspha_dt`` is
+    (nTerm, nSam) as given to the operator (no recon_terms processing); the model must
+    carry no concomitant terms (array_operator.jl:139)."""
+    kdt = np.ascontiguousarray(np.asarray(kspha_dt, dtype=np.float32).astype(np.float64).T)
+    assert kdt.shape == (m.ns, m.L), "kspha_dt must match the model's terms and samples"
+    xm = np.ascontiguousarray(gather(m, x_full).astype(np.complex128))
+    cs = np.ascontiguousarray(m.csm.astype(np.complex128))
+    y = np.empty(m.nc * m.ns, dtype=np.complex128)
+    t, fm = _f64(m.times), _f64(m.fieldmap)
+    lib().oracle_forward_dt_f64(m.nv, m.ns, m.nc, m.L, _p(m.bf64), _p(m.k64), _p(kdt), _p(t), _p(fm), _p(cs),
+                                _p(xm), _p(y))
     return y
 
 

@@ -114,8 +114,10 @@ def test_host_mirror_argument_errors():
         hoe.HighOrderExplicitOp(g, k, t, implementation="array")
     with pytest.raises(hoe.ArgumentError):
         hoe.HighOrderExplicitOp(g, k, t, implementation="other")
-    with pytest.raises(hoe.ArgumentError):
-        hoe.HighOrderExplicitOp(g, k, t, kspha_dt=k)
+    with pytest.raises(AssertionError):                 # array_operator.jl:160
+        hoe.HighOrderExplicitOp(g, k, t, kspha_dt=k[:, :-1])
+    with pytest.raises(hoe.ArgumentError):              # test_explicit_interface.jl:71
+        hoe.HighOrderExplicitOp(g, k, t, backend="blocked", kspha_dt=k)
     with pytest.raises(hoe.ArgumentError):
         hoe.HighOrderExplicitOp(g, k, t, backend="invalid")
     with pytest.raises(hoe.ArgumentError):

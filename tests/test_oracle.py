@@ -10,7 +10,8 @@ a wrong sign, a transposed operand, a wrong scale or ordering) fails it:
     adjoint identity, block-count invariance, zeros outside the mask;
   - brute force: the C oracle against an independently coded dense matrix,
     CGNR against a QR least-squares solve (test_ReconstructionStability.jl),
-    the gradient oracle against central finite differences of the forward.
+    the gradient oracle and the derivative product against central finite
+    differences of the forward.
 """
 import numpy as np
 import pytest
@@ -228,6 +229,33 @@ def test_gradient_oracle_matches_finite_differences():
         lm = np.linalg.norm(O.forward_f64(O.with_coefficients(m, km), x) - y) ** 2
         fd = (lp - lm) / (2 * eps)
         assert abs(fd - g[l, s]) <= 1e-6 * max(1.0, abs(fd)), (l, s, fd, g[l, s])
+
+
+def test_derivative_product_is_the_coefficient_derivative_of_the_forward():
+    """B x (array_operator.jl:187-235) is d/de E(k + e kdt) x at e = 0: central
+    differences of the FP64 forward along kdt pin the 2 pi i factor, the sign,
+    the basis contraction and the untouched t * fieldmap term; linearity in kdt."""
+    rng = np.random.default_rng(21)
+    grid = Grid(5, 4, 1, np.float32(0.02), np.float32(0.02), np.float32(0.02))
+    ns, nc = 10, 2
+    kspha = (rng.normal(size=(16, ns)) * 3).astype(np.float32)
+    kdt = (rng.normal(size=(16, ns)) * 2).astype(np.float32)
+    times = np.linspace(0, 2e-3, ns).astype(np.float32)
+    fm = rng.uniform(-80, 80, grid.n).astype(np.float32)
+    csm = (rng.normal(size=(grid.n, nc)) + 1j * rng.normal(size=(grid.n, nc))).astype(np.complex64)
+    x = (rng.normal(size=grid.n) + 1j * rng.normal(size=grid.n)).astype(np.complex64)
+    m = O.make_model(grid, kspha, times, fm, csm, None)
+    bx = O.forward_dt_f64(m, kdt, x)
+    eps = 1e-6
+    k0, d = m.k64.T, kdt.astype(np.float64)
+    fd = (O.forward_f64(O.with_coefficients(m, k0 + eps * d), x)
+          - O.forward_f64(O.with_coefficients(m, k0 - eps * d), x)) / (2 * eps)
+    assert np.linalg.norm(bx - fd) <= 1e-6 * np.linalg.norm(fd)
+    kdt2 = (rng.normal(size=(16, ns))).astype(np.float32)
+    lin = O.forward_dt_f64(m, kdt + 2 * kdt2, x)
+    ref = bx + 2 * O.forward_dt_f64(m, kdt2, x)
+    assert np.linalg.norm(lin - ref) <= 1e-5 * np.linalg.norm(ref)
+    assert np.linalg.norm(O.forward_dt_f64(m, np.zeros_like(kdt), x)) == 0.0
 
 
 def test_catmull_rom_delay():
