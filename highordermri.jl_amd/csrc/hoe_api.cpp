@@ -198,6 +198,26 @@ void build_dcoef_records(const Handle *h, std::vector<double> &rec) {
         }
 }
 
+// prep_kspha (prep_kspha.jl:1-56) into the first n_term rows of the traces: a cleared
+// order-0/2/3 flag zeroes its rows, a cleared order-1 flag replaces rows 2-4 with
+// k_nominal (3 x n_sam, term fastest; NULL keeps kspha's own rows, the reference's
+// default k_nominal = kspha[2:4, :]). Concomitant rows are left as they are.
+void prepare_traces(Handle *h, const float *kspha, const float *k_nominal) {
+    const int L = h->L, n_term = h->n_term_in;
+    const bool *t = h->order;
+    for (int64_t s = 0; s < h->n_sam; ++s) {
+        float *r = &h->traces[(size_t)s * L];
+        for (int l = 0; l < n_term; ++l) r[l] = kspha[(size_t)s * n_term + l];
+        if (!t[0]) r[0] = 0.0f;
+        if (!t[1] && k_nominal)
+            for (int l = 0; l < 3; ++l) r[1 + l] = k_nominal[(size_t)s * 3 + l];
+        if (!t[2])
+            for (int l = 4; l < 9; ++l) r[l] = 0.0f;
+        if (!t[3] && n_term == 16)
+            for (int l = 9; l < 16; ++l) r[l] = 0.0f;
+    }
+}
+
 int upload_coef(Handle *h) {
     std::vector<double> rec, drec;
     build_coef_records(h, rec);
@@ -368,20 +388,12 @@ int hoe_create(const hoe_model_desc *d, hoe_handle **out) {
     // ---- prepared coefficient traces (term fastest), prep_kspha + prep_concomitant ----
     const int L = h->L;
     const int64_t ns = h->n_sam;
+    for (int i = 0; i < 4; ++i) h->order[i] = t[i];
     h->traces.assign((size_t)L * ns, 0.0f);
-    for (int64_t s = 0; s < ns; ++s) {
-        float *r = &h->traces[(size_t)s * L];
-        for (int l = 0; l < n_term; ++l) r[l] = d->kspha[(size_t)s * n_term + l];
-        if (!t[0]) r[0] = 0.0f;
-        if (!t[1] && d->k_nominal)
-            for (int l = 0; l < 3; ++l) r[1 + l] = d->k_nominal[(size_t)s * 3 + l];
-        if (!t[2])
-            for (int l = 4; l < 9; ++l) r[l] = 0.0f;
-        if (!t[3] && n_term == 16)
-            for (int l = 9; l < 16; ++l) r[l] = 0.0f;
-        if (d->concomitant)
-            for (int l = 0; l < 4; ++l) r[n_term + l] = d->concomitant[(size_t)s * 4 + l];
-    }
+    prepare_traces(h, d->kspha, d->k_nominal);
+    if (d->concomitant)
+        for (int64_t s = 0; s < ns; ++s)
+            for (int l = 0; l < 4; ++l) h->traces[(size_t)s * L + n_term + l] = d->concomitant[(size_t)s * 4 + l];
     h->times.assign(d->times, d->times + ns);
 
     // ---- basis columns in FP64; identically zero columns (z terms on a z = 0
@@ -714,6 +726,17 @@ int hoe_set_coefficients(hoe_handle *hh, const float *kspha, double tau) {
     if (tau != 0.0 && !(h->dwell > 0.0)) return set_err(h, HOE_EINVAL, "a delay requires dwell > 0");
     if (kspha) std::memcpy(h->traces.data(), kspha, sizeof(float) * h->traces.size());
     h->tau = tau;
+    HOE_TRY_HIP(h, hipStreamSynchronize(h->stream));
+    return upload_coef(h);
+}
+
+int hoe_set_kspha(hoe_handle *hh, const float *kspha, const float *k_nominal) {
+    Handle *h = reinterpret_cast<Handle *>(hh);
+    int rc = check_open(h);
+    if (rc) return rc;
+    if (!kspha) return set_err(h, HOE_EINVAL, "kspha is required");
+    prepare_traces(h, kspha, k_nominal);
+    h->tau = 0.0;
     HOE_TRY_HIP(h, hipStreamSynchronize(h->stream));
     return upload_coef(h);
 }

@@ -58,6 +58,7 @@ struct Handle {
     bool weights_unit = false;        // dvec holds unit weights
     bool dt_on = false;               // hoe_forward computes the derivative product B x (kspha_dt set)
     bool has_conc = false;            // the model carries the 4 concomitant terms
+    bool order[4] = {true, true, true, true};   // recon_terms flags: orders 0..3 kept
     float csm_max = 0.0f;             // max |Re|,|Im| bound of the masked coil maps: max |csm|
     double dwell = 0.0, tau = 0.0;
     std::string err;

@@ -1,20 +1,23 @@
 """hoe: MI355X (gfx950) Explicit higher-order MRI encoding operator.
 
 Host-side mirror of HighOrderMRI.jl's Explicit operator API
-(HighOrderExplicitOp / normalOperator / explicit_gpu_normal / recon_HOOp)
-over the C ABI of libhoe (include/hoe.h).
+(HighOrderExplicitOp / normalOperator / explicit_gpu_normal / recon_HOOp) and
+its delay estimation (FindDelay / FindDelay_multishot / InterpTrajTime) over the C ABI of libhoe (include/hoe.h).
 """
 from ._lib import (ArgumentError, AssertionFailure, DimensionMismatch, HIPError, LibraryNotBuilt,
                    NoDeviceError, LIB_PATH, load)
 from .grid import Grid, basisfunc_spha
 from .operator import (CGNR, AdjointOp, ComposedOp, HighOrderExplicitOp, L2Regularization, NormalOp,
                        WeightingOp, compose, explicit_gpu_normal, normalOperator, recon_HOOp)
+from .sync import (AkimaMonotonicInterpolation, FindDelay, FindDelay_multishot, InterpTrajTime,
+                   LinearMonotonicInterpolation, kspha_derivative)
 
 __all__ = [
     "ArgumentError", "AssertionFailure", "DimensionMismatch", "HIPError", "LibraryNotBuilt",
     "NoDeviceError", "LIB_PATH", "load", "Grid", "basisfunc_spha", "CGNR", "AdjointOp", "ComposedOp",
     "HighOrderExplicitOp", "L2Regularization", "NormalOp", "WeightingOp", "compose",
-    "explicit_gpu_normal", "normalOperator", "recon_HOOp",
+    "explicit_gpu_normal", "normalOperator", "recon_HOOp", "AkimaMonotonicInterpolation", "FindDelay",
+    "FindDelay_multishot", "InterpTrajTime", "LinearMonotonicInterpolation", "kspha_derivative",
 ]
 
 

@@ -28,7 +28,7 @@ BACKENDS = {
 EXPORTS = (
     "hoe_version", "hoe_last_error", "hoe_device_count", "hoe_create", "hoe_get_info",
     "hoe_set_backend", "hoe_forward", "hoe_adjoint", "hoe_normal", "hoe_set_weights",
-    "hoe_set_coefficients", "hoe_set_kspha_dt", "hoe_grad", "hoe_cg", "hoe_close", "hoe_destroy",
+    "hoe_set_coefficients", "hoe_set_kspha", "hoe_set_kspha_dt", "hoe_grad", "hoe_cg", "hoe_close", "hoe_destroy",
     "hoe_sync", "hoe_last_kernel_ms",
 )
 
@@ -115,6 +115,7 @@ def load(path: str | os.PathLike | None = None):
         "hoe_normal": (C.c_int, [vp, vp, i64, vp, vp, C.c_int]),
         "hoe_set_weights": (C.c_int, [vp, vp, i64]),
         "hoe_set_coefficients": (C.c_int, [vp, vp, dbl]),
+        "hoe_set_kspha": (C.c_int, [vp, vp, vp]),
         "hoe_set_kspha_dt": (C.c_int, [vp, vp]),
         "hoe_grad": (C.c_int, [vp, vp, vp, vp, C.POINTER(dbl), C.POINTER(dbl), C.c_int]),
         "hoe_cg": (C.c_int, [vp, vp, vp, i64, i32, dbl, dbl, vp, vp, C.c_int]),

@@ -201,6 +201,7 @@ class HighOrderExplicitOp:
         self.grid = grid
         self.nSam, self.nCha, self.nGrid = n_sam, n_cha, n_grid
         self.nTerm = n_term + (4 if concomitant is not None else 0)
+        self._n_term_in = n_term
         info = _lib.Info()
         check(lib.hoe_get_info(h, C.byref(info)), h)
         self.nVox = int(info.n_vox)
@@ -245,11 +246,30 @@ class HighOrderExplicitOp:
             self.kspha_dt = None
             return
         kdt = _f32(kspha_dt, "kspha_dt")
-        if kdt.shape != (self.nTerm, self.nSam):
+        if kdt.shape != (self._n_term_in, self.nSam):
             raise AssertionError("kspha_dt must have same size as kspha")
         kt = np.ascontiguousarray(kdt.T)          # term fastest
         check(self._lib.hoe_set_kspha_dt(h, kt.ctypes.data), h)
         self.kspha_dt = kdt
+
+    def set_kspha(self, kspha, k_nominal=None):
+        """Replace ``kspha`` (``(nTerm, nSam)``) and prepare it as the constructor does
+        (``recon_terms``, ``k_nominal``; prep_kspha.jl:1-56). Concomitant terms, maps and
+        times are kept and the delay is reset to 0: the operator equals one built anew
+        with the new coefficients, as FindDelay does for every trial delay
+        (FindDelay.jl:184-187)."""
+        h = self._handle()
+        k = _f32(kspha, "kspha")
+        if k.shape != (self._n_term_in, self.nSam):
+            raise DimensionMismatch(f"kspha must be ({self._n_term_in}, {self.nSam})")
+        kt = np.ascontiguousarray(k.T)
+        knp = None
+        if k_nominal is not None:
+            kn = _f32(k_nominal, "k_nominal")
+            if kn.shape != (3, self.nSam):
+                raise DimensionMismatch("k_nominal must be (3, nSam)")
+            knp = np.ascontiguousarray(kn.T)
+        check(self._lib.hoe_set_kspha(h, kt.ctypes.data, None if knp is None else knp.ctypes.data), h)
 
     def prod(self, x, out=None):
         """``op * x``: x of length prod(matrixSize) -> signal of length nSam*nCha

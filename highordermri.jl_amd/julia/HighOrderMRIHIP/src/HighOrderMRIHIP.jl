@@ -21,7 +21,7 @@ using ChainRulesCore
 import HighOrderMRI
 using HighOrderMRI: Grid, HighOrderExplicitOp
 
-export HIPExplicitOp, hip_explicit_loss
+export HIPExplicitOp, hip_explicit_loss, hip_FindDelay, set_kspha!, set_kspha_dt!
 
 const libhoe = get(ENV, "HOE_LIBRARY", joinpath(@__DIR__, "..", "..", "..", "libhoe.so"))
 
@@ -221,6 +221,102 @@ function ChainRulesCore.rrule(::typeof(hip_explicit_loss), op::HIPExplicitOp, h,
     loss, g, gτ = hip_explicit_grad(op, h, τ, x, y)
     pullback(Δ) = (NoTangent(), NoTangent(), Δ .* g, Δ * gτ, NoTangent(), NoTangent())
     return loss, pullback
+end
+
+"""
+    set_kspha!(op, kspha; k_nominal=nothing)
+
+Replace the operator's coefficients, prepared as at construction (`recon_terms`,
+`k_nominal`; prep_kspha.jl:1-56): the operator then equals one built anew with `kspha`.
+"""
+function set_kspha!(op::HIPExplicitOp, kspha::AbstractMatrix; k_nominal=nothing)
+    k = Matrix{Float32}(kspha)
+    kn = k_nominal === nothing ? nothing : Matrix{Float32}(k_nominal)
+    hd = handle(op)
+    GC.@preserve k kn check(ccall((:hoe_set_kspha, libhoe), Cint, (Ptr{Cvoid}, Ptr{Float32}, Ptr{Float32}),
+                                  hd, pointer(k), kn === nothing ? C_NULL : pointer(kn)), hd)
+    return op
+end
+
+"""
+    set_kspha_dt!(op, kspha_dt)
+
+Switch `op * x` to the derivative product `B x` of array_operator.jl:187-235
+(`kspha_dt` of `size(kspha)`), or back to `E x` with `nothing`.
+"""
+function set_kspha_dt!(op::HIPExplicitOp, kspha_dt)
+    hd = handle(op)
+    if kspha_dt === nothing
+        check(ccall((:hoe_set_kspha_dt, libhoe), Cint, (Ptr{Cvoid}, Ptr{Float32}), hd, C_NULL), hd)
+    else
+        kdt = Matrix{Float32}(kspha_dt)
+        GC.@preserve kdt check(ccall((:hoe_set_kspha_dt, libhoe), Cint, (Ptr{Cvoid}, Ptr{Float32}),
+                                     hd, pointer(kdt)), hd)
+    end
+    return op
+end
+
+"""
+    hip_FindDelay(gridding, data, kspha, datatime, StartTime, dt; JumpFact=3, Δτ_min,
+                  intermode=AkimaMonotonicInterpolation(), fieldmap, csm, recon_terms="111",
+                  solver=CGNR, reg=L2Regularization(zero(T)), iter_max=10, backend=:mixed)
+
+`FindDelay` (FindDelay.jl:139-214) with every operator product and the inner CGNR's
+products on the GPU through one `HIPExplicitOp`: each trial delay replaces the
+coefficients with `set_kspha!` instead of building two operators, and `B x` is taken
+with `set_kspha_dt!`. Resampling and density weights are the reference's own
+`InterpTrajTime` and `samplingDensity`.
+"""
+function hip_FindDelay(gridding::Grid{T}, data::AbstractArray{Complex{T},2}, kspha::AbstractArray{T,2},
+                       datatime::AbstractVector{T}, StartTime::T, dt::T; JumpFact=3, Δτ_min::T=T(0.005),
+                       intermode=HighOrderMRI.AkimaMonotonicInterpolation(),
+                       fieldmap=zeros(T, gridding.nX, gridding.nY),
+                       csm=ones(Complex{T}, gridding.nX, gridding.nY, 1), recon_terms::String="111",
+                       solver=HighOrderMRI.CGNR, reg=HighOrderMRI.L2Regularization(zero(T)),
+                       iter_max::Int=10, backend::Symbol=:mixed) where {T<:AbstractFloat}
+    @assert size(data, 2) == size(csm, 3) "data and csm must have the same number of coil channels"
+    @assert size(data, 1) == size(datatime, 1) "data and datatime must have the same number of spatial points"
+    nSample = size(data, 1)
+    # 5-tap derivative with zeros beyond both ends (conv(kspha, kernel)[3:end-2, :] / dt)
+    c = T.((1/8, 1/4, 0, -1/4, -1/8))
+    n = size(kspha, 1)
+    kz(i, j) = 1 <= i <= n ? kspha[i, j] : zero(T)
+    kspha_dt = [sum(c[m] * kz(i + 3 - m, j) for m in 1:5) / dt for i in 1:n, j in 1:size(kspha, 2)]
+    recParams = Dict{Symbol,Any}(:reconSize => (gridding.nX, gridding.nY), :reg => reg,
+                                 :iterations => iter_max, :solver => solver)
+    τ = 0.0; nIter = 1; Δτ_prev = Δτ = Inf
+    op = nothing
+    try
+        while abs(Δτ) > Δτ_min
+            kspha_τ = T.(HighOrderMRI.InterpTrajTime(kspha, dt, τ + StartTime, datatime;
+                                                     intermode=intermode)[1:nSample, :]')
+            kspha_dt_τ = T.(HighOrderMRI.InterpTrajTime(kspha_dt, dt, τ + StartTime, datatime;
+                                                        intermode=intermode)[1:nSample, :]')
+            weight = HighOrderMRI.samplingDensity(kspha_τ[2:3, :], (gridding.nX, gridding.nY))
+            if op === nothing
+                op = HIPExplicitOp(gridding, kspha_τ, datatime; recon_terms=recon_terms, csm=csm,
+                                   fieldmap=fieldmap, backend=backend)
+            else
+                set_kspha!(op, kspha_τ)
+            end
+            x = HighOrderMRI.recon_HOOp(op, data, weight, recParams)
+            y1 = vec(data) - op * vec(x)          # Y - Aₚxₚ
+            set_kspha_dt!(op, kspha_dt_τ)
+            y2 = op * vec(x)                      # Bₚxₚ
+            set_kspha_dt!(op, nothing)
+            Δτ = JumpFact * real(y2 \ y1)
+            τ += Δτ
+            @info "Iteration $nIter: Δτ = $(round(Δτ/dt, digits=5)) [dt] | τ = $(round(τ/dt, digits=5)) [dt] | JumpFact = $JumpFact"
+            if nIter > 1 && sign(Δτ_prev) != sign(Δτ)
+                JumpFact = max(1, JumpFact / 2)
+            end
+            Δτ_prev = Δτ
+            nIter += 1
+        end
+    finally
+        op === nothing || close(op)
+    end
+    return T(τ - Δτ / JumpFact * (JumpFact - 1))
 end
 
 end # module

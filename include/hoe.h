@@ -167,6 +167,14 @@ int hoe_set_weights(hoe_handle *h, const float *D, int64_t n_d);
  * model (FindDelay.jl:179-201, array_operator.jl:187-235). Host pointers. */
 int hoe_set_coefficients(hoe_handle *h, const float *kspha, double tau);
 
+/* Replace kspha (n_term x n_sam, term fastest, the layout of hoe_model_desc)
+ * and process it exactly as hoe_create does: the handle's recon_terms flags
+ * (prep_kspha.jl:1-56), with k_nominal (3 x n_sam, NULL = kspha's own rows
+ * 2..4, the default of array_operator.jl:92). Concomitant rows, times, maps
+ * and the backend are kept; the delay tau is reset to 0. Replaces building a
+ * new operator for each trial delay (FindDelay.jl:184-187). Host pointers. */
+int hoe_set_kspha(hoe_handle *h, const float *kspha, const float *k_nominal);
+
 /* Set (kspha_dt != NULL) or clear (NULL) the trajectory-derivative
  * coefficients dk/dt: n_term x n_sam floats (term fastest, the kspha input
  * layout, not processed by recon_terms). While set, hoe_forward computes the

@@ -10,6 +10,8 @@ Each function cites the reference code it restates:
   - CPU Array path (cpu_ref_*): array_operator.jl:241-348
   - deterministic fixture: test/test_HighOrderExplicit.jl:3-85
   - normal weights: explicit_gpu_operator.jl:399-416; reconstruction: recon_HOOp.jl:73-81
+  - derivative product B x: array_operator.jl:187-235
+  - delay loop: src/Synchronization/FindDelay.jl:152-214, InterpTraj.jl:18-44
 The heavy loops live in hoe_oracle.c (liboracle.so, built by oracle/Makefile).
 """
 from __future__ import annotations
@@ -339,6 +341,61 @@ def delayed(traces, tau, dwell):
     return out
 
 
+# ---------------------------------------------------------------- delay estimation
+
+def kspha_dt_stencil(kspha, dt):
+    """``conv(kspha, [1/8 1/4 0 -1/4 -1/8]')[3:end-2, :] / dt`` per column of a
+    ``(nFieldSample, nTerm)`` array (FindDelay.jl:178-180), FP64."""
+    k = np.asarray(kspha, dtype=np.float64)
+    ker = np.array([1 / 8, 1 / 4, 0.0, -1 / 4, -1 / 8])
+    cols = [np.convolve(k[:, j], ker, mode="full")[2:-2] for j in range(k.shape[1])]
+    return np.stack(cols, axis=1) / float(dt)
+
+
+def interp_traj_time(traj, dt, delTime, datatime):
+    """InterpTraj.jl:18-44: column j of ``traj`` ``(nFieldSample, nTerm)`` sampled at
+    ``(0:n-1)*dt - delTime``, interpolated at ``datatime`` by Akima's method, zero outside
+    the sampled interval. Uses scipy's Akima1DInterpolator, an implementation of the
+    same published method (Akima 1970) that is independent of the product's."""
+    from scipy.interpolate import Akima1DInterpolator
+    tr = np.asarray(traj, dtype=np.float64)
+    tt = float(dt) * np.arange(tr.shape[0]) - float(delTime)
+    out = Akima1DInterpolator(tt, tr, axis=0)(np.asarray(datatime, dtype=np.float64))
+    return np.nan_to_num(out, nan=0.0)
+
+
+def find_delay_f64(grid, data, kspha, datatime, StartTime, dt, *, JumpFact=3, dtau_min, fieldmap=None,
+                   csm=None, recon_terms=None, iter_max=10, lam=0.0, weight=None, max_updates=50):
+    """FindDelay.jl:152-214 in FP64: per update, resample kspha and its stencil derivative at
+    ``StartTime + τ``, reconstruct x by ``iter_max`` CGNR iterations on the weighted normal,
+    ``Δτ = J * real((B x) \\ (y - E x))``, halve J (not below 1) on a sign change, stop at
+    ``|Δτ| <= dtau_min``; returns ``(τ - Δτ/J*(J-1), [τ after each update])``."""
+    data = np.asarray(data)
+    n_sample = data.shape[0]
+    y = data.reshape(-1, order="F").astype(np.complex128)
+    kdt = kspha_dt_stencil(kspha, dt)
+    tau, J, n_it = 0.0, JumpFact, 1
+    dtau = dtau_prev = np.inf
+    hist = [0.0]
+    while abs(dtau) > dtau_min and n_it <= max_updates:
+        k_tau = interp_traj_time(kspha, dt, tau + StartTime, datatime)[:n_sample].T.astype(np.float32)
+        kdt_tau = interp_traj_time(kdt, dt, tau + StartTime, datatime)[:n_sample].T.astype(np.float32)
+        m = make_model(grid, k_tau, datatime, fieldmap, csm, None, recon_terms=recon_terms)
+        D = np.ones(n_sample) if weight is None else np.abs(np.asarray(weight, dtype=np.complex128)) ** 2
+        rhs = adjoint_f64(m, expand_weights(m, D) * y)
+        x, _ = cgnr(lambda p: normal_f64(m, p, D), rhs, iter_max, lam)
+        y1 = y - forward_f64(m, x)
+        y2 = forward_dt_f64(m, kdt_tau, x)
+        dtau = J * float((np.vdot(y2, y1) / np.vdot(y2, y2)).real)
+        tau += dtau
+        if n_it > 1 and np.sign(dtau_prev) != np.sign(dtau):
+            J = max(1, J / 2)
+        dtau_prev = dtau
+        n_it += 1
+        hist.append(tau)
+    return tau - dtau / J * (J - 1), hist
+
+
 # ---------------------------------------------------------------- reference fixtures
 
 def _jrange(a, b, n, T=np.float32):
@@ -382,6 +439,49 @@ def explicit_highorder_test_problem(n_coil=3, grid_cls=None):
     mask[n - 1] = False
     image = (_jrange(0.1, 1.0, n) + 1j * _jrange(-0.2, 0.15, n)).astype(np.complex64)
     return dict(grid=grid, kspha=k, times=times, fieldmap=fieldmap, csm=csm, mask=mask, image=image)
+
+
+def delay_test_problem(n=16, n_cha=4, n_field=1200, n_adc=1000, dt=2e-6, start=100, tau_samples=3.4,
+                       turns=6, seed=5, grid_cls=None):
+    """A 2-D spiral acquisition whose data are simulated (FP64, noiseless) from field
+    coefficients delayed by ``tau_samples * dt``: ``y = E(k(t + StartTime + τ)) x``.
+    Coefficients ``(n_field, 16)``: a smooth order-0 drift, the spiral in rows 2-3,
+    smooth small orders 2-3. Returns the FindDelay inputs and the planted delay."""
+    import sys
+    if grid_cls is None:
+        sys.path.insert(0, str(HERE.parent / "highordermri.jl_amd"))
+        from hoe.grid import Grid as grid_cls
+    rng = np.random.default_rng(seed)
+    fov = 0.2
+    grid = grid_cls(n, n, 1, np.float32(fov / n), np.float32(fov / n), np.float32(fov / n))
+    u = np.arange(n_field) / (n_field - 1)
+    kmax = n / (2 * fov)
+    r = kmax * u
+    th = 2 * np.pi * turns * u
+    k = np.zeros((n_field, 16))
+    k[:, 0] = 0.05 * np.sin(2 * np.pi * 1.3 * u + rng.uniform(0, 6))
+    k[:, 1], k[:, 2] = r * np.cos(th), r * np.sin(th)
+    for j in range(4, 16):
+        order = 2 if j < 9 else 3
+        a, b, f = rng.normal(size=3)
+        k[:, j] = 0.03 * kmax / (fov / 2) ** (order - 1) * (a * np.sin(2 * np.pi * (1 + abs(f)) * u) + b * u)
+    kspha = k.astype(np.float32)
+    datatime = (dt * np.arange(n_adc)).astype(np.float32)
+    StartTime = np.float32(start * dt)
+    tau = tau_samples * dt
+    x, y = grid.x.astype(np.float64), grid.y.astype(np.float64)
+    csm = np.empty((grid.n, n_cha), np.complex64)
+    for c in range(n_cha):
+        a = 2 * np.pi * c / n_cha
+        csm[:, c] = np.exp(-((x - 0.06 * np.cos(a)) ** 2 + (y - 0.06 * np.sin(a)) ** 2) / (2 * 0.08 ** 2)) * \
+            np.exp(1j * (x * np.cos(a) - y * np.sin(a)) * 10)
+    fm = (40 * np.exp(-((x - 0.02) ** 2 + y ** 2) / (2 * 0.05 ** 2))).astype(np.float32)
+    img = (np.exp(-(x ** 2 + (y / 0.8) ** 2) / (2 * 0.05 ** 2)) * (1 + 0.3 * np.sin(30 * x))).astype(np.complex64)
+    k_true = interp_traj_time(kspha, dt, StartTime + tau, datatime).T.astype(np.float32)
+    m = make_model(grid, k_true, datatime, fm, csm, None)
+    data = forward_f64(m, img).reshape(n_adc, n_cha, order="F").astype(np.complex64)
+    return dict(grid=grid, data=data, kspha=kspha, datatime=datatime, StartTime=StartTime, dt=np.float32(dt),
+                fieldmap=fm, csm=csm, image=img, tau=tau)
 
 
 def relerr(a, b):
