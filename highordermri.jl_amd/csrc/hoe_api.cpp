@@ -308,6 +308,25 @@ void release(Handle *h) {
 
 }  // namespace
 
+namespace {
+template <typename R>
+int apply_girf_entry(const R *g, int64_t n_point, int64_t n_other, const R *hw, int32_t n_freq, int32_t n_out,
+                     double rbw, int32_t device, R *out, int flags) {
+    g_err.clear();
+    if (!g || !hw || !out) return set_err(nullptr, HOE_EINVAL, "gradient, response and output must not be NULL");
+    if (n_point < 1 || n_other < 1) return set_err(nullptr, HOE_EINVAL, "the gradient waveforms are empty");
+    if (n_freq < 1 || n_out < 1) return set_err(nullptr, HOE_EINVAL, "the response is empty");
+    if (!std::isfinite(rbw)) return set_err(nullptr, HOE_EINVAL, "rbw must be finite");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_err(nullptr, HOE_ENODEV, "no HIP device");
+    if (device < 0 || device >= ndev) return set_err(nullptr, HOE_EINVAL, "invalid visible GPU ID");
+    std::string err;
+    const int rc = hoe::girf_run<R>(g, n_point, n_other, hw, n_freq, n_out, rbw, device, out, flags, err);
+    if (rc) return set_err(nullptr, rc, err);
+    return HOE_OK;
+}
+}  // namespace
+
 extern "C" {
 
 const char *hoe_version(void) { return "libhoe 0.1.0 (gfx950)"; }
@@ -316,6 +335,18 @@ const char *hoe_last_error(const hoe_handle *h) {
     if (h) return reinterpret_cast<const Handle *>(h)->err.c_str();
     return g_err.c_str();
 }
+
+int hoe_apply_girf(const float *g, int64_t n_point, int64_t n_other, const float *hw, int32_t n_freq, int32_t n_out,
+                   double rbw, int32_t device, float *out, int flags) {
+    return apply_girf_entry<float>(g, n_point, n_other, hw, n_freq, n_out, rbw, device, out, flags);
+}
+
+int hoe_apply_girf_f64(const double *g, int64_t n_point, int64_t n_other, const double *hw, int32_t n_freq,
+                       int32_t n_out, double rbw, int32_t device, double *out, int flags) {
+    return apply_girf_entry<double>(g, n_point, n_other, hw, n_freq, n_out, rbw, device, out, flags);
+}
+
+int64_t hoe_girf_fft_length(int64_t n) { return hoe::girf_fft_length(n); }
 
 int hoe_device_count(void) {
     int n = 0;

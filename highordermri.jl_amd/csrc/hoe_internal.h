@@ -163,4 +163,11 @@ hipError_t launch_norm2(Handle *h, const void *v, int64_t n, double *out_dev);
 // shift, m, w: the delay resampling k(s) = sum_t w[t] h(clamp(s + m - 1 + t)).
 hipError_t launch_grad_chain(Handle *h, double *gtau_dev, bool shift, int m, const double w[4]);
 
+// GIRF prediction (hoe_girf.hip): FFT length nextprod([2,3,5], n), and the whole
+// prediction of n_other 3-channel waveforms; on failure err holds the message.
+int64_t girf_fft_length(int64_t n);
+template <typename R>
+int girf_run(const R *g, int64_t n_point, int64_t n_other, const R *hw, int32_t n_freq, int32_t n_out, double rbw,
+             int32_t device, R *out, int flags, std::string &err);
+
 }  // namespace hoe

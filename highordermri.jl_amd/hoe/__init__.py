@@ -2,13 +2,15 @@
 
 Host-side mirror of HighOrderMRI.jl's Explicit operator API
 (HighOrderExplicitOp / normalOperator / explicit_gpu_normal / recon_HOOp) and
-its delay estimation (FindDelay / FindDelay_multishot / InterpTrajTime) over the C ABI of libhoe (include/hoe.h).
+its delay estimation (FindDelay / FindDelay_multishot / InterpTrajTime) and field-model
+inputs (apply_girf / grad2traj / traj2grad) over the C ABI of libhoe (include/hoe.h).
 """
 from ._lib import (ArgumentError, AssertionFailure, DimensionMismatch, HIPError, LibraryNotBuilt,
                    NoDeviceError, LIB_PATH, load)
 from .grid import Grid, basisfunc_spha
 from .operator import (CGNR, AdjointOp, ComposedOp, HighOrderExplicitOp, L2Regularization, NormalOp,
                        WeightingOp, compose, explicit_gpu_normal, normalOperator, recon_HOOp)
+from .girf import GIRFModel, apply_girf, grad2traj, traj2grad, unpack_girf_channels
 from .sync import (AkimaMonotonicInterpolation, FindDelay, FindDelay_multishot, InterpTrajTime,
                    LinearMonotonicInterpolation, kspha_derivative)
 
@@ -18,6 +20,7 @@ __all__ = [
     "HighOrderExplicitOp", "L2Regularization", "NormalOp", "WeightingOp", "compose",
     "explicit_gpu_normal", "normalOperator", "recon_HOOp", "AkimaMonotonicInterpolation", "FindDelay",
     "FindDelay_multishot", "InterpTrajTime", "LinearMonotonicInterpolation", "kspha_derivative",
+    "GIRFModel", "apply_girf", "grad2traj", "traj2grad", "unpack_girf_channels",
 ]
 
 

@@ -29,7 +29,7 @@ EXPORTS = (
     "hoe_version", "hoe_last_error", "hoe_device_count", "hoe_create", "hoe_get_info",
     "hoe_set_backend", "hoe_forward", "hoe_adjoint", "hoe_normal", "hoe_set_weights",
     "hoe_set_coefficients", "hoe_set_kspha", "hoe_set_kspha_dt", "hoe_grad", "hoe_cg", "hoe_close", "hoe_destroy",
-    "hoe_sync", "hoe_last_kernel_ms",
+    "hoe_sync", "hoe_last_kernel_ms", "hoe_apply_girf", "hoe_apply_girf_f64", "hoe_girf_fft_length",
 )
 
 
@@ -116,6 +116,9 @@ def load(path: str | os.PathLike | None = None):
         "hoe_set_weights": (C.c_int, [vp, vp, i64]),
         "hoe_set_coefficients": (C.c_int, [vp, vp, dbl]),
         "hoe_set_kspha": (C.c_int, [vp, vp, vp]),
+        "hoe_apply_girf": (C.c_int, [vp, i64, i64, vp, i32, i32, dbl, i32, vp, C.c_int]),
+        "hoe_apply_girf_f64": (C.c_int, [vp, i64, i64, vp, i32, i32, dbl, i32, vp, C.c_int]),
+        "hoe_girf_fft_length": (i64, [i64]),
         "hoe_set_kspha_dt": (C.c_int, [vp, vp]),
         "hoe_grad": (C.c_int, [vp, vp, vp, vp, C.POINTER(dbl), C.POINTER(dbl), C.c_int]),
         "hoe_cg": (C.c_int, [vp, vp, vp, i64, i32, dbl, dbl, vp, vp, C.c_int]),

@@ -21,7 +21,7 @@ using ChainRulesCore
 import HighOrderMRI
 using HighOrderMRI: Grid, HighOrderExplicitOp
 
-export HIPExplicitOp, hip_explicit_loss, hip_FindDelay, set_kspha!, set_kspha_dt!
+export HIPExplicitOp, hip_explicit_loss, hip_FindDelay, hip_apply_girf, set_kspha!, set_kspha_dt!
 
 const libhoe = get(ENV, "HOE_LIBRARY", joinpath(@__DIR__, "..", "..", "..", "libhoe.so"))
 
@@ -318,5 +318,39 @@ function hip_FindDelay(gridding::Grid{T}, data::AbstractArray{Complex{T},2}, ksp
     end
     return T(τ - Δτ / JumpFact * (JumpFact - 1))
 end
+
+"""
+    hip_apply_girf(G_DCS_nom, Hw; dim_spatial=1, dim_time=2, rbw=1.0, device=0)
+
+`apply_girf` (src/GIRF/apply_girf.jl:39-109) on the GPU: hipFFT transforms and HIP
+kernels for the transfer product, batched over every waveform. Same arguments, shapes,
+result and FFT-length / response-expansion / rbw conventions; `Float32` gradients are
+computed in FP32, `Float64` in FP64. Also accepts a `GIRFModel`.
+"""
+function hip_apply_girf(G::AbstractArray{T,N}, Hw::AbstractArray; dim_spatial::Int=1, dim_time::Int=2,
+                        rbw::Real=1.0, device::Integer=0) where {T<:Union{Float32,Float64},N}
+    size(G, dim_spatial) == 3 || throw(ArgumentError("Input gradient must have 3 channels along dim_spatial."))
+    nFreq, nIn, nOut = size(Hw)
+    @assert nIn == 3 "GIRF model must have 3 input channels (X, Y, Z)"
+    nPoint = size(G, dim_time)
+    other = Tuple(setdiff(1:N, (dim_spatial, dim_time)))
+    perm = (dim_spatial, dim_time, other...)
+    g = Array{T}(permutedims(G, perm))                 # (3, nPoint, others...), channel fastest
+    nOther = length(g) ÷ (3 * nPoint)
+    h = Array{Complex{T}}(Hw)
+    out = Array{T}(undef, nOut, nPoint, (size(G, d) for d in other)...)
+    check(girf_ccall(g, nPoint, nOther, h, nFreq, nOut, Float64(rbw), Int32(device), out))
+    return permutedims(out, invperm(perm))
+end
+
+girf_ccall(g::Array{Float32}, nPoint, nOther, h::Array{ComplexF32}, nFreq, nOut, rbw, device, out::Array{Float32}) =
+    ccall((:hoe_apply_girf, libhoe), Cint,
+          (Ptr{Float32}, Int64, Int64, Ptr{ComplexF32}, Int32, Int32, Float64, Int32, Ptr{Float32}, Cint),
+          g, nPoint, nOther, h, nFreq, nOut, rbw, device, out, Cint(0))
+girf_ccall(g::Array{Float64}, nPoint, nOther, h::Array{ComplexF64}, nFreq, nOut, rbw, device, out::Array{Float64}) =
+    ccall((:hoe_apply_girf_f64, libhoe), Cint,
+          (Ptr{Float64}, Int64, Int64, Ptr{ComplexF64}, Int32, Int32, Float64, Int32, Ptr{Float64}, Cint),
+          g, nPoint, nOther, h, nFreq, nOut, rbw, device, out, Cint(0))
+hip_apply_girf(G::AbstractArray, model::HighOrderMRI.GIRFModel; kwargs...) = hip_apply_girf(G, model.Hw; kwargs...)
 
 end # module

@@ -222,6 +222,32 @@ int hoe_sync(hoe_handle *h);
  * 2 = whole call. Returns a negative value when unavailable. */
 double hoe_last_kernel_ms(const hoe_handle *h, int which);
 
+/* ---- field-model inputs upstream of kspha ---- */
+
+/* GIRF prediction: apply the frequency-domain gradient response Hw to n_other
+ * independent 3-channel nominal gradient waveforms (apply_girf,
+ * src/GIRF/apply_girf.jl:39-150, with _apply_girf_core :111-150).
+ * g:   3 x n_point x n_other reals, channel fastest (Julia (3, nPoint, nOther)).
+ * hw:  complex n_freq x 3 x n_out, frequency fastest, DC centred (fftshift
+ *      order), interleaved (re, im).
+ * out: n_out x n_point x n_other reals, channel fastest.
+ * The FFT length is nextprod([2,3,5], max(n_point, n_freq)); waveforms are
+ * zero padded to it (circular convolution); a shorter response is expanded in
+ * the time domain exactly as the reference does (apply_girf.jl:72-90, which
+ * leaves out time sample n_freq/2+1, 1-based); rbw < 1 clears
+ * round(N*(1-rbw)/2) bins at each end of the centred spectrum. FP32
+ * arithmetic (hipFFT C2C) for hoe_apply_girf, FP64 (Z2Z) for _f64. Runs on
+ * the given device; pointers are host memory, or device memory on that device
+ * with HOE_DEVICE_PTRS. No handle: errors are read with hoe_last_error(NULL). */
+int hoe_apply_girf(const float *g, int64_t n_point, int64_t n_other, const float *hw, int32_t n_freq,
+                   int32_t n_out, double rbw, int32_t device, float *out, int flags);
+int hoe_apply_girf_f64(const double *g, int64_t n_point, int64_t n_other, const double *hw, int32_t n_freq,
+                       int32_t n_out, double rbw, int32_t device, double *out, int flags);
+
+/* nextprod([2,3,5], n): the FFT length apply_girf uses for n = max(nPoint, nFreq)
+ * (apply_girf.jl:69). */
+int64_t hoe_girf_fft_length(int64_t n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -12,6 +12,7 @@ Each function cites the reference code it restates:
   - normal weights: explicit_gpu_operator.jl:399-416; reconstruction: recon_HOOp.jl:73-81
   - derivative product B x: array_operator.jl:187-235
   - delay loop: src/Synchronization/FindDelay.jl:152-214, InterpTraj.jl:18-44
+  - GIRF prediction: src/GIRF/apply_girf.jl:39-150; trajectory integration: utils/conversion.jl:14-41
 The heavy loops live in hoe_oracle.c (liboracle.so, built by oracle/Makefile).
 """
 from __future__ import annotations
@@ -394,6 +395,78 @@ def find_delay_f64(grid, data, kspha, datatime, StartTime, dt, *, JumpFact=3, dt
         n_it += 1
         hist.append(tau)
     return tau - dtau / J * (J - 1), hist
+
+
+# ---------------------------------------------------------------- GIRF prediction
+
+def nextprod235(n):
+    """Smallest m >= n whose only prime factors are 2, 3, 5 (``nextprod([2,3,5], n)``)."""
+    m = max(int(n), 1)
+    while True:
+        r = m
+        for f in (2, 3, 5):
+            while r % f == 0:
+                r //= f
+        if r == 1:
+            return m
+        m += 1
+
+
+def girf_expand_response(Hw, N):
+    """apply_girf.jl:72-90 in FP64: time-domain expansion of a DC-centred response
+    ``(nFreq, 3, nOut)`` to N frequencies; time sample nFreq//2 (0-based) is left out."""
+    Hw = np.asarray(Hw, dtype=np.complex128)
+    n_freq = Hw.shape[0]
+    hw_t = np.fft.ifft(np.fft.ifftshift(Hw, axes=0), axis=0)
+    pad = np.zeros((N,) + Hw.shape[1:], dtype=np.complex128)
+    half = n_freq // 2
+    pad[:half] = hw_t[:half]
+    pad[N - (n_freq - half) + 1:] = hw_t[half + 1:]
+    return np.fft.fftshift(np.fft.fft(pad, axis=0), axes=0)
+
+
+def girf_core_f64(G2, Hw_work, n_point, rbw=1.0):
+    """_apply_girf_core (apply_girf.jl:111-150) in FP64 for one (3, nPoint) waveform."""
+    N = Hw_work.shape[0]
+    gp = np.zeros((3, N), dtype=np.complex128)
+    gp[:, :n_point] = G2
+    Gf = np.fft.fftshift(np.fft.fft(gp, axis=1), axes=1)
+    if rbw < 1.0:
+        nz = int(np.round(N * (1.0 - rbw) / 2.0))
+        if nz > 0:
+            Gf[:, :nz] = 0
+            Gf[:, N - nz:] = 0
+    Of = np.einsum("fio,if->of", Hw_work, Gf)
+    return np.real(np.fft.ifft(np.fft.ifftshift(Of, axes=1), axis=1))[:, :n_point]
+
+
+def apply_girf_f64(G, Hw, dim_spatial=1, dim_time=2, rbw=1.0):
+    """apply_girf (apply_girf.jl:39-109) in FP64; dims are 1-based as in the reference."""
+    G = np.asarray(G, dtype=np.float64)
+    Hw = np.asarray(Hw, dtype=np.complex128)
+    ds, dt_ = dim_spatial - 1, dim_time - 1
+    assert G.shape[ds] == 3 and Hw.shape[1] == 3
+    n_point = G.shape[dt_]
+    N = nextprod235(max(n_point, Hw.shape[0]))
+    Hw_work = girf_expand_response(Hw, N) if N > Hw.shape[0] else Hw
+    Gm = np.moveaxis(G, (ds, dt_), (0, 1))
+    others = Gm.shape[2:]
+    G3 = Gm.reshape(3, n_point, -1)
+    out = np.stack([girf_core_f64(G3[:, :, i], Hw_work, n_point, rbw) for i in range(G3.shape[2])], axis=2)
+    out = out.reshape((Hw.shape[2], n_point) + others)
+    return np.moveaxis(out, (0, 1), (ds, dt_))
+
+
+def grad2traj_f64(x, dt, dim=1):
+    """utils/conversion.jl:14-41: zero then the running trapezoid sum (x[j]+x[j+1])*dt/2."""
+    x = np.asarray(x, dtype=np.float64)
+    a = np.moveaxis(x, dim - 1, 0)
+    out = np.zeros_like(a)
+    acc = np.zeros(a.shape[1:])
+    for j in range(1, a.shape[0]):
+        acc = acc + (a[j - 1] + a[j]) * (dt / 2)
+        out[j] = acc
+    return np.moveaxis(out, 0, dim - 1)
 
 
 # ---------------------------------------------------------------- reference fixtures
