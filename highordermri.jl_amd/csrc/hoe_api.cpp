@@ -537,6 +537,14 @@ int hoe_create(const hoe_model_desc *d, hoe_handle **out) {
             }
         }
         h->horner = ok && (double)nv / (double)h->n_runs >= 16.0;
+        // the MFMA adjoint evaluates a wave's 64 voxels from the Horner tables of at most two runs
+        bool two_runs = h->horner;
+        for (int64_t w = 0; w + 63 < h->nv_pad && two_runs; w += 64)
+            two_runs = vrun[(size_t)w + 63] - vrun[(size_t)w] <= 1;
+#ifndef HOE_HORNER_ADJ
+#define HOE_HORNER_ADJ 1
+#endif
+        h->horner_adj = HOE_HORNER_ADJ && two_runs;
     }
 
     hipDeviceProp_t prop;
@@ -636,6 +644,8 @@ int hoe_get_info(const hoe_handle *hh, hoe_info_t *info) {
     info->backend = h->backend;
     info->device = h->device;
     info->closed = h->closed ? 1 : 0;
+    info->horner = (h->horner ? 1 : 0) | (h->horner_adj ? 2 : 0);
+    info->n_runs = h->n_runs;
     return HOE_OK;
 }
 

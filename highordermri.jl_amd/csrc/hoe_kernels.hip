@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <type_traits>
 
 namespace hoe {
 
@@ -292,9 +293,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (
                 acc[a][b][0] = f16x_t{};
                 acc[a][b][1] = f16x_t{};
             }
-        // K loop over this segment; phases_of(ks, ph) fills the phases of the
-        // 8 streamed rows of K step ks.
-        auto kloop = [&](auto phases_of) {
+        // K loop over this segment; phases_of(ks, ph) fills 8 phases per lane:
+        // with swap, those of the lane's own row for the 8 streamed rows of K
+        // step ks (moved into the A layout by permlane32 swaps); without, the
+        // A layout directly (rows lane%32 and 32 + lane%32, streamed rows
+        // 4*(lane/32)..+3 each).
+        auto kloop = [&](auto phases_of, auto swap) {
             for (int ks = d.y; ks < d.z; ++ks) {
                 uint4 bfr[NB][4];
 #pragma unroll
@@ -312,14 +316,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (
                     sincos_cycles(ph[j], sn, cs);
                     split2<T16>(cs, sn, hp[j], lp[j]);
                 }
+                if constexpr (decltype(swap)::value) {
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    auto a = __builtin_amdgcn_permlane32_swap(hp[j], hp[j + 4], false, false);
-                    hp[j] = a[0];
-                    hp[j + 4] = a[1];
-                    auto b = __builtin_amdgcn_permlane32_swap(lp[j], lp[j + 4], false, false);
-                    lp[j] = b[0];
-                    lp[j + 4] = b[1];
+                    for (int j = 0; j < 4; ++j) {
+                        auto a = __builtin_amdgcn_permlane32_swap(hp[j], hp[j + 4], false, false);
+                        hp[j] = a[0];
+                        hp[j + 4] = a[1];
+                        auto b = __builtin_amdgcn_permlane32_swap(lp[j], lp[j + 4], false, false);
+                        lp[j] = b[0];
+                        lp[j + 4] = b[1];
+                    }
                 }
                 vec_t ahi[2], alo[2];
                 ahi[0] = __builtin_bit_cast(vec_t, make_uint4(hp[0], hp[1], hp[2], hp[3]));
@@ -379,7 +385,107 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (
                         ph[j] = p;
                     }
                 }
-            });
+            }, std::true_type{});
+        } else if constexpr (HX && !FWD) {
+            // Horner adjoint. The wave's 64 voxels lie in at most two runs (r_lo, r_hi;
+            // checked on the host). For each pair of K steps one FP64 matrix product
+            // gives every coefficient the pair needs: rows = its 16 streamed samples,
+            // columns 0..7 = c_j(s, run) = sum_i A_s[i] beta[run][i][j] for (run, j),
+            // column 8 = t_s. The 16x16 result goes through a wave-private LDS slot;
+            // each lane reads the (c0..c3, t) of its samples and runs and spends 4 FP64
+            // FMAs per phase on its two voxel rows, already in the A-operand layout.
+            constexpr int NM = (L + 4) / 4;   // k blocks of 4 record terms covering 0..L (t included)
+            typedef double d4v __attribute__((ext_vector_type(4)));
+            __shared__ double hl[4][288];     // per wave, 2 slots: c [2 steps][2 runs][2 halves][4][4], t [2][2][4]
+            __shared__ double hb[4][NM][64];  // per wave: B operands of the setup product (constant per segment)
+            const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+            double *cl = hl[wv];
+            const int base = d.x * 256 + wv * 64;
+            const int r_lo = hx.vrun[base], r_hi = hx.vrun[base + 63];
+            const bool two = r_hi != r_lo;
+            const int v0 = base + (lane & 31), v1 = v0 + 32;
+            const double2 h0 = hx.hrec[v0], h1 = hx.hrec[v1];
+            const int rs0 = hx.vrun[v0] - r_lo, rs1 = hx.vrun[v1] - r_lo;
+            const int half = lane >> 5;
+            const int col = lane & 15, kq = lane >> 4;
+            double *bop = hb[wv][0];
+#pragma unroll
+            for (int m = 0; m < NM; ++m) {
+                const int term = 4 * m + kq;
+                double b = 0.0;
+                if (col < 8) {
+                    const int rr = min(r_lo + (col >> 2), r_hi);
+                    if (term < L) b = hx.beta[((size_t)rr * L + term) * 4 + (col & 3)];
+                } else if (col == 8) {
+                    b = term == hx.t_index ? 1.0 : 0.0;
+                }
+                bop[m * 64 + lane] = b;
+            }
+            auto horner = [](const double *c, double x) { return fma(fma(fma(c[3], x, c[2]), x, c[1]), x, c[0]); };
+            // A operand of the pair starting at step kp: rows 0..7 = samples of kp, rows
+            // 8..15 = of kp + 1; zero where the step is outside the segment (the address
+            // is then a valid row, so the load is in bounds whatever the compiler does)
+            auto load_a = [&](int kp, double *a) {
+                const int r = lane & 15;
+                const bool valid = kp < d.z && (r < 8 || kp + 1 < d.z);
+                const double *ap = coef + ((size_t)(valid ? kp * 8 + r : d.y * 8)) * sk + kq;
+#pragma unroll
+                for (int m = 0; m < NM; ++m) a[m] = valid ? ap[4 * m] : 0.0;
+            };
+            // setup product of one pair into an LDS slot. v_mfma_f64_16x16x4: register r
+            // of lane l holds row l/16 + 4 r, column l%16; row kq + 4 r is sample kq of
+            // half r&1 of step r>>1 of the pair.
+            auto setup = [&](const double *a, double *slot) {
+                d4v acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int m = 0; m < NM; ++m)
+                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[m], bop[m * 64 + lane], acc, 0, 0, 0);
+                if (col < 8) {
+                    double *w = slot + (col >> 2) * 32 + kq * 4 + (col & 3);
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) w[((r >> 1) * 4 + (r & 1)) * 16] = acc[r];
+                } else if (col == 8) {
+                    double *w = slot + 128 + kq;
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) w[((r >> 1) * 2 + (r & 1)) * 4] = acc[r];
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            };
+            // software pipeline: the slot of pair p + 1 is filled during the second step of
+            // pair p, and the sample records of pair p + 2 are fetched meanwhile
+            double anx[NM];
+            load_a(d.y, anx);
+            if (d.y < d.z) setup(anx, cl);
+            load_a(d.y + 2, anx);
+            kloop([&](int ks, double *ph) {
+                const int rel = ks - d.y, k2 = rel & 1;
+                const double *cur = cl + ((rel >> 1) & 1) * 144;
+                if (k2 == 1 && ks + 1 < d.z) {
+                    setup(anx, cl + (((rel >> 1) & 1) ^ 1) * 144);
+                    load_a(ks + 3, anx);
+                }
+                const double *tp = cur + 128 + (k2 * 2 + half) * 4;
+                const double *c0p = cur + ((k2 * 2 + rs0) * 2 + half) * 16;
+                if (two) {
+                    const double *c1p = cur + ((k2 * 2 + rs1) * 2 + half) * 16;
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const double t = tp[i];
+                        ph[i] = fma(t, h0.y, horner(c0p + 4 * i, h0.x));
+                        ph[4 + i] = fma(t, h1.y, horner(c1p + 4 * i, h1.x));
+                    }
+                } else {
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        const double t = tp[i];
+                        const double *c = c0p + 4 * i;
+                        ph[i] = fma(t, h0.y, horner(c, h0.x));
+                        ph[4 + i] = fma(t, h1.y, horner(c, h1.x));
+                    }
+                }
+            }, std::false_type{});
         } else {
             double st[NST];
             {
@@ -405,7 +511,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (
                     }
                     ph[j] = p;
                 }
-            });
+            }, std::true_type{});
         }
         // D layout of v_mfma_f32_32x32x*: register r of lane l holds
         // row 8*(r/4) + 4*(l/32) + r%4, column l%32.
@@ -1381,8 +1487,13 @@ static void visit_contract_mixed(Handle *h, int backend, F &&f) {
             else visit_mixed<__bf16, true, false>(h->LK, mixed_nb(h), f);
         }
     } else {
-        if (backend == HOE_BACKEND_MIXED_FP16) visit_mixed<_Float16, false, false>(h->LK, mixed_nb(h), f);
-        else visit_mixed<__bf16, false, false>(h->LK, mixed_nb(h), f);
+        if (backend == HOE_BACKEND_MIXED_FP16) {
+            if (h->horner_adj) visit_mixed<_Float16, false, true>(h->LK, mixed_nb(h), f);
+            else visit_mixed<_Float16, false, false>(h->LK, mixed_nb(h), f);
+        } else {
+            if (h->horner_adj) visit_mixed<__bf16, false, true>(h->LK, mixed_nb(h), f);
+            else visit_mixed<__bf16, false, false>(h->LK, mixed_nb(h), f);
+        }
     }
 }
 template <bool FWD, class F>

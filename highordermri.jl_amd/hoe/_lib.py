@@ -87,6 +87,7 @@ class Info(C.Structure):
     _fields_ = [
         ("n_grid", C.c_int64), ("n_vox", C.c_int64), ("n_sam", C.c_int64), ("n_cha", C.c_int64),
         ("n_term", C.c_int32), ("backend", C.c_int32), ("device", C.c_int32), ("closed", C.c_int32),
+        ("horner", C.c_int32), ("n_runs", C.c_int32),
     ]
 
 

@@ -1,0 +1,74 @@
+"""Phase evaluation by Horner runs against the FP64 oracle, in the geometries that select
+each kernel path:
+
+- 3-D 96x12x4: every basis term active (record bucket L = 16), runs of 96 voxels, so
+  many 64-voxel waves hold the ends of two runs -- the MFMA adjoint's two-run branch,
+  with its per-pair FP64 setup product (the Horner forward and MFMA adjoint both on);
+- 2-D 40x40 with a mask: runs of 40 or fewer, waves spanning three runs -- Horner forward
+  only, the MFMA adjoint on its generic phase;
+- 2-D 192x8: runs of exactly three waves, every wave inside one run (the single-run branch).
+
+Every backend's E x, E^H y and E^H E x are compared with the FP64 operator
+(array_operator.jl:241-348 restated in the oracle) at the Float32 bar of
+test_explicit_gpu.jl:51 (3e-5; 1e-4 for BF16).
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+import oracle as O  # noqa: E402
+import hoe  # noqa: E402
+from hoe import synth  # noqa: E402
+
+TOLS = {"direct": 3e-5, "mixed_fp16": 3e-5, "mixed_bf16": 1e-4}
+
+
+def _problem(nx, ny, nz, n_sam=1536, n_cha=6, seed=3, mask=None):
+    rng = np.random.default_rng(seed)
+    fov = (0.22, 0.22 * ny / nx, 0.11 if nz > 1 else 0.22 / nx)
+    grid = hoe.Grid(nx, ny, nz, np.float32(fov[0] / nx), np.float32(fov[1] / ny), np.float32(fov[2] / nz))
+    kmax = nx / (2 * fov[0])
+    parts = 4 if nz > 1 else 1
+    per = n_sam // parts
+    kx, ky = synth.spiral(per, kmax, 6)
+    kz = (np.arange(parts) - parts // 2) / fov[2] if nz > 1 else np.zeros(parts)
+    k1 = np.stack([np.tile(kx, parts), np.tile(ky, parts), np.repeat(kz, per)])
+    kspha = synth.coefficient_rows(rng, k1, fov[0] / 2, k1.shape[1]).astype(np.float32)
+    times = np.tile(np.arange(per) * 2e-6, parts).astype(np.float32)
+    fm = synth.gaussian_b0(grid, rng, 120.0)
+    csm = synth.gaussian_coils(grid, n_cha, rng, sigma=fov[0] / 3, ring_radius=0.3 * fov[0],
+                               z_spread=fov[2] if nz > 1 else 0.0)
+    x = synth.phantom_image(grid, rng, fov)
+    return grid, kspha, times, fm, csm, x, mask
+
+
+@pytest.mark.parametrize("shape,mask_kind,expect_adj", [
+    ((96, 12, 4), None, True),
+    ((40, 40, 1), "disc", False),
+    ((192, 8, 1), None, True),
+])
+@pytest.mark.parametrize("backend", ["direct", "mixed_fp16", "mixed_bf16"])
+def test_horner_paths_vs_fp64(shape, mask_kind, expect_adj, backend, gpu_lib):
+    nx, ny, nz = shape
+    mask = None
+    if mask_kind == "disc":
+        i, j = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+        mask = ((i - nx / 2 + 0.5) ** 2 + (j - ny / 2 + 0.5) ** 2 <= (0.45 * nx) ** 2).reshape(nx, ny, 1)
+    grid, kspha, times, fm, csm, x, mask = _problem(nx, ny, nz, mask=mask)
+    m = O.make_model(grid, kspha, times, fm, csm, mask)
+    y_ref = O.forward_f64(m, x)
+    probe = (np.random.default_rng(1).normal(size=y_ref.size)
+             + 1j * np.random.default_rng(2).normal(size=y_ref.size)).astype(np.complex64)
+    xa_ref = O.adjoint_f64(m, probe)
+    n_ref = O.normal_f64(m, x)
+    tol = TOLS[backend]
+    with hoe.HighOrderExplicitOp(grid, kspha, times, fieldmap=fm, csm=csm, mask=mask, backend=backend) as op:
+        assert op.horner & 1, "runs long enough for the Horner forward"
+        if backend != "direct":
+            assert bool(op.horner & 2) == expect_adj
+        e_f = O.relerr(op * x, y_ref)
+        e_a = O.relerr(op.H * probe, xa_ref)
+        e_n = O.relerr(hoe.normalOperator(op) * x, n_ref)
+    print(f"{shape} mask={mask_kind} {backend} (horner {op.horner}): E {e_f:.2e}  E^H {e_a:.2e}  E^H E {e_n:.2e}")
+    assert e_f < tol and e_a < tol and e_n < tol
