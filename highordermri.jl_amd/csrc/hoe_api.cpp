@@ -750,9 +750,8 @@ int hoe_normal(hoe_handle *hh, const float *D, int64_t n_d, const void *x, void 
     const void *dx;
     if ((rc = stage_in(h, flags, x, h->io_x, sizeof(float2) * h->n_grid, &dx))) return rc;
     void *dout = (flags & HOE_DEVICE_PTRS) ? out : h->io_x2.p;
-    HOE_TRY_HIP(h, hoe::launch_gather(h, dx, h->xm.p));
     float2 *tmp = reinterpret_cast<float2 *>(h->cg_vec.p);   // masked output
-    HOE_TRY_HIP(h, hoe::run_normal_masked(h, h->xm.p, tmp, (float)(1.0 / (double)h->n_vox), 0.0, true));
+    HOE_TRY_HIP(h, hoe::run_normal_full(h, dx, tmp, (float)(1.0 / (double)h->n_vox), true));
     HOE_TRY_HIP(h, hoe::launch_scatter(h, tmp, dout, 1.0f));
     if ((rc = finish(h, flags, out, dout, sizeof(float2) * h->n_grid))) return rc;
     if (!(flags & HOE_ASYNC)) record_times(h, true, true);

@@ -149,8 +149,12 @@ hipError_t launch_load_ybuf(Handle *h, const void *y_user, bool weights);
 // Adjoint contraction from ybuf -> part_a, then coil reduction into xm (masked) with scale.
 hipError_t run_adjoint_contract(Handle *h, float scale, void *xm_out, bool time_it);
 // Normal operator on masked vectors: out_m = scale * E^H D E x_m (+ lambda x_m).
+// have_max: slot 0 already holds max|xm_in| and the other slots are clear (launch from a
+// gather that computed it).
 hipError_t run_normal_masked(Handle *h, const void *xm_in, void *xm_out, float scale, double lambda,
-                             bool time_it);
+                             bool time_it, bool have_max = false);
+// Gather of a full-grid x (with max|x| for the fragment scale) and the normal operator.
+hipError_t run_normal_full(Handle *h, const void *x_full, void *xm_out, float scale, bool time_it);
 // Gradient partial kernel: uses zbuf (csm .* x) and ybuf (residual, coil fastest) -> gk.
 hipError_t run_grad_contract(Handle *h, bool time_it);
 // zbuf = csm .* xm

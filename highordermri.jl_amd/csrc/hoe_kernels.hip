@@ -1368,20 +1368,30 @@ static inline double *cgs(Handle *h) { return reinterpret_cast<double *>(h->scal
 
 static inline int blocks_for(int64_t n, int b = 256) { return (int)((n + b - 1) / b); }
 
+static void zero_slots(Handle *h) { k_zero_u32<<<1, 64, 0, h->stream>>>(slot(h, 0), 64); }
+
 hipError_t launch_gather(Handle *h, const void *x_full, void *xm) {
     k_gather<<<blocks_for(h->nv_pad), 256, 0, h->stream>>>((const float2 *)x_full, (const int *)h->mask_idx.p,
                                                            (float2 *)xm, (int)h->n_vox, (int)h->nv_pad, nullptr);
     return hipGetLastError();
 }
 
+// Gather that also leaves max|xm| in slot 0 (all slots cleared first): the
+// fragment scale of the mixed forward, without a separate pass over xm.
+static hipError_t launch_gather_max(Handle *h, const void *x_full, void *xm) {
+    zero_slots(h);
+    k_gather<<<blocks_for(h->nv_pad), 256, 0, h->stream>>>((const float2 *)x_full, (const int *)h->mask_idx.p,
+                                                           (float2 *)xm, (int)h->n_vox, (int)h->nv_pad, slot(h, 0));
+    return hipGetLastError();
+}
+
 hipError_t launch_scatter(Handle *h, const void *xm, void *x_full, float scale) {
-    k_fill0<<<blocks_for(h->n_grid), 256, 0, h->stream>>>((float2 *)x_full, h->n_grid);
+    // without a mask every grid entry is written below
+    if (h->n_vox != h->n_grid) k_fill0<<<blocks_for(h->n_grid), 256, 0, h->stream>>>((float2 *)x_full, h->n_grid);
     k_scatter<<<blocks_for(h->n_vox), 256, 0, h->stream>>>((const float2 *)xm, (const int *)h->mask_idx.p,
                                                            (float2 *)x_full, (int)h->n_vox, scale);
     return hipGetLastError();
 }
-
-static void zero_slots(Handle *h) { k_zero_u32<<<1, 64, 0, h->stream>>>(slot(h, 0), 64); }
 
 // ---- kernel instances over (L, coil batch): visitors hand the instance to a launcher ----
 
@@ -1682,8 +1692,8 @@ static void launch_contract(Handle *h, const void *operand, float2 *part) {
 }
 
 // Builds the forward operand from masked xm (device) and runs the forward
-// contraction into part_f.
-static hipError_t forward_from_xm(Handle *h, const float2 *xm, bool time_it) {
+// contraction into part_f. have_max: slot 0 already holds max|xm| (gathered).
+static hipError_t forward_from_xm(Handle *h, const float2 *xm, bool time_it, bool have_max = false) {
     if (fwd_backend(h) == HOE_BACKEND_DIRECT) {
         const int64_t nz = h->nv_pad * h->ncs;
         k_build_z<<<blocks_for(nz), 256, 0, h->stream>>>((const float2 *)h->csm.p, xm, (float2 *)h->zbuf.p,
@@ -1694,8 +1704,10 @@ static hipError_t forward_from_xm(Handle *h, const float2 *xm, bool time_it) {
     } else {
         const int n_ks = (int)(h->nv_pad / 8);
         const int n_gb = frag_groups(h);
-        zero_slots(h);
-        k_absmax2<<<blocks_for(h->nv_pad), 256, 0, h->stream>>>(xm, (int)h->nv_pad, slot(h, 0));
+        if (!have_max) {
+            zero_slots(h);
+            k_absmax2<<<blocks_for(h->nv_pad), 256, 0, h->stream>>>(xm, (int)h->nv_pad, slot(h, 0));
+        }
         const int64_t nt = (int64_t)n_gb * n_ks * 64;
         if (h->backend == HOE_BACKEND_MIXED_FP16)
             k_build_zfrag<_Float16><<<blocks_for(nt), 256, 0, h->stream>>>(
@@ -1711,9 +1723,15 @@ static hipError_t forward_from_xm(Handle *h, const float2 *xm, bool time_it) {
 }
 
 hipError_t run_forward_contract(Handle *h, const void *x_full, bool time_it) {
-    hipError_t e = launch_gather(h, x_full, h->xm.p);
+    hipError_t e = launch_gather_max(h, x_full, h->xm.p);
     if (e != hipSuccess) return e;
-    return forward_from_xm(h, (const float2 *)h->xm.p, time_it);
+    return forward_from_xm(h, (const float2 *)h->xm.p, time_it, true);
+}
+
+hipError_t run_normal_full(Handle *h, const void *x_full, void *xm_out, float scale, bool time_it) {
+    hipError_t e = launch_gather_max(h, x_full, h->xm.p);
+    if (e != hipSuccess) return e;
+    return run_normal_masked(h, h->xm.p, xm_out, scale, 0.0, time_it, true);
 }
 
 static inline bool mixed(Handle *h) { return h->backend != HOE_BACKEND_DIRECT; }
@@ -1791,10 +1809,10 @@ hipError_t run_adjoint_contract(Handle *h, float scale, void *xm_out, bool time_
 }
 
 hipError_t run_normal_masked(Handle *h, const void *xm_in, void *xm_out, float scale, double lambda,
-                             bool time_it) {
+                             bool time_it, bool have_max) {
     // slot 0 / slot 1 hold the fragment scales of the mixed path
-    zero_slots(h);
-    hipError_t e = forward_from_xm(h, (const float2 *)xm_in, time_it);
+    if (!have_max) zero_slots(h);
+    hipError_t e = forward_from_xm(h, (const float2 *)xm_in, time_it, have_max);
     if (e != hipSuccess) return e;
     e = launch_reduce_fwd_ybuf(h, 1.0f, true, nullptr);
     if (e != hipSuccess) return e;
