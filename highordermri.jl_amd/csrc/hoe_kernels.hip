@@ -895,9 +895,15 @@ __global__ void k_zero_u32(unsigned *p, int n) {
     if (i < n) p[i] = 0u;
 }
 
+// The slot is cleared before the launch and only grows during it, so a value
+// read from it never exceeds the true maximum: when it already covers m the
+// atomic is skipped (most waves, once a large value has landed), which keeps
+// thousands of waves from serialising on one address.
 __device__ __forceinline__ void wave_absmax_to(unsigned *dst, float m) {
     for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
-    if ((threadIdx.x & 63) == 0 && m > 0.0f) atomicMax(dst, __float_as_uint(m));
+    if ((threadIdx.x & 63) == 0 && m > 0.0f &&
+        __float_as_uint(m) > __hip_atomic_load(dst, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        atomicMax(dst, __float_as_uint(m));
 }
 
 // xm[v] = x_full[mask_idx[v]] (zero for padded voxels); optional |.|max.
