@@ -116,6 +116,44 @@ def cpu_baseline(cfg, n_sub, threads):
     return t * ns_full / n_sub, m
 
 
+def real_data(steps):
+    """E^H E throughput and the 20-iteration recon_HOOp time on the reference's 7 T spiral
+    demo data (29 000 samples, 32 coils, 16 measured field terms; in-tree fixture written
+    by tests/golden/make_7t_fixture.py), with the reconstruction's correlation to the
+    reference's own image of the same data. None when the fixture is absent."""
+    import torch
+    import hoe
+    from hoe import _lib
+    sys.path.insert(0, str(ROOT / "tests" / "golden"))
+    import fixture7t
+    p = fixture7t.load("spiral", "measured")
+    if p is None:
+        return None
+    n = p["grid"].nX
+    dev = torch.device("cuda:0")
+    x = torch.zeros(p["grid"].n, dtype=torch.complex64, device=dev)
+    x[torch.from_numpy(p["mask"]).to(dev)] = 1.0
+    out = torch.empty_like(x)
+    rp = {"reconSize": (n, p["grid"].nY), "reg": hoe.L2Regularization(1e-9), "iterations": 20, "solver": hoe.CGNR}
+    res = {"dataset": "7T_2D_Spiral_1p0_200_r4 (reference demo data), measured field, recon terms 0111",
+           "n_sam": int(p["kspha"].shape[1]), "n_vox": int(p["mask"].sum()), "n_cha": int(p["csm"].shape[1])}
+    for backend in ("mixed_fp16", "direct"):
+        with hoe.HighOrderExplicitOp(p["grid"], p["kspha"], p["times"], fieldmap=p["fieldmap"], csm=p["csm"],
+                                     mask=p["mask"], recon_terms=p["recon_terms"], backend=backend) as op:
+            k = steps if backend == "mixed_fp16" else max(3, steps // 5)
+            t, _, _ = time_normal(op, x, out, k, 2)
+            img = np.asarray(hoe.recon_HOOp(op, p["data"], p["weight"], rp))
+            ts = []
+            for _ in range(2):
+                t0 = time.perf_counter()
+                img = np.asarray(hoe.recon_HOOp(op, p["data"], p["weight"], rp))
+                ts.append(time.perf_counter() - t0)
+            res[backend] = {"normal_applies_per_s": round(k / t, 3), "recon20_ms": round(1e3 * min(ts), 3),
+                            "corr_with_reference_image": round(fixture7t.corr(fixture7t.rendered(img, n),
+                                                                              p["golden"]), 5)}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -127,6 +165,7 @@ def main():
     ap.add_argument("--no-direct", action="store_true")
     ap.add_argument("--no-cg", action="store_true")
     ap.add_argument("--cpu-samples", type=int, default=0)
+    ap.add_argument("--no-realdata", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
@@ -232,6 +271,9 @@ def main():
         grad_ms, grad_kernel_ms = grad["mixed_fp16"]["ms"], grad["mixed_fp16"]["kernel_ms"]
         log(f"grad mixed vs fp32: {grad['mixed_vs_fp32']:.2e}")
     op.close()
+    realdata = None if args.no_realdata else real_data(args.steps)
+    if realdata:
+        log(f"7T spiral demo data: {realdata}")
 
     value = args.steps / t_mixed
     fp32_value = steps_d / t_direct
@@ -273,6 +315,7 @@ def main():
                    "forward_vs_fp64_sampled": {k: float(v) for k, v in fwd_err.items()}},
         "grad_ms": round(grad_ms, 3), "grad_kernel_ms": round(grad_kernel_ms, 3), "grad": grad,
         "cg30": cg,
+        "realdata": realdata,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

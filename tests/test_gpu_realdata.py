@@ -18,59 +18,34 @@ tests/golden/make_7t_fixture.py. Checks:
   its own reference image better than the other one by 0.02;
 - the mixed and FP32 backends give the same reconstruction to 1e-3.
 """
+import sys
 from pathlib import Path
 
 import numpy as np
 import pytest
 
 import hoe
-from hoe import fieldcam
 import oracle as O
 
 GOLDEN = Path(__file__).parent / "golden"
+if str(GOLDEN) not in sys.path:
+    sys.path.insert(0, str(GOLDEN))
+import fixture7t  # noqa: E402
+
 pytestmark = pytest.mark.gpu
+rendered, corr = fixture7t.rendered, fixture7t.corr
 
 
 def load(name, trajectory):
-    path = GOLDEN / f"7t_{name}.npz"
-    if not path.exists():
-        pytest.skip(f"{path.name} not generated: tests/golden/make_7t_fixture.py needs the reference's demo data")
-    z = np.load(path)
-    nX, nY, nZ = (int(v) for v in z["matrix"])
-    res = z["resolution"]
-    grid = hoe.Grid(nX, nY, nZ, res[0], res[1], res[2])
-    mask = z["mask"]
-    csm = np.zeros((grid.n, z["csm_masked"].shape[1]), dtype=np.complex64)
-    csm[mask] = z["csm_masked"]
-    if trajectory == "measured":
-        kspha = z["kspha_measured"]
-        data = fieldcam.demodulate(z["kdata"], z["k0_adc"], kspha[0]).astype(np.complex64)
-        weight, terms, golden = z["weight_measured"], "0111", z["golden_measured"]
-    else:
-        kspha = z["kspha_nominal"]
-        data = z["kdata"]
-        weight, terms, golden = z["weight_nominal"], "010", z["golden_nominal"]
-    return dict(grid=grid, kspha=kspha, times=z["times"], csm=csm, fieldmap=z["fieldmap"], mask=mask,
-                data=data, weight=weight, recon_terms=terms, golden=golden,
-                golden_other=z["golden_nominal" if trajectory == "measured" else "golden_measured"])
+    p = fixture7t.load(name, trajectory)
+    if p is None:
+        pytest.skip(f"7t_{name}.npz not generated: tests/golden/make_7t_fixture.py needs the reference's demo data")
+    return p
 
 
 def make_op(p, backend):
     return hoe.HighOrderExplicitOp(p["grid"], p["kspha"], p["times"], fieldmap=p["fieldmap"], csm=p["csm"],
                                    mask=p["mask"], recon_terms=p["recon_terms"], backend=backend)
-
-
-def rendered(img, n):
-    """plt_image's gray levels of |img| after rotl90 (plt_image.jl:36-48)."""
-    a = np.abs(fieldcam.rotl90(np.asarray(img).reshape(n, n, order="F")))
-    lo, hi = a.min(), np.quantile(a, 0.999)
-    return np.clip((a - lo) / (hi - lo), 0.0, 1.0)
-
-
-def corr(a, b):
-    a = a.ravel() - a.mean()
-    b = b.ravel() - b.mean()
-    return float(a @ b / np.sqrt((a @ a) * (b @ b)))
 
 
 CASES = [("spiral", "measured"), ("spiral", "nominal"), ("epi", "measured"), ("epi", "nominal")]
