@@ -3,8 +3,10 @@
 them (demo/result/*_{Measured,Nominal}.png), for the real-data GPU tests.
 
 The arrays are the operator inputs prepared by hoe.fieldcam.prepare_demo (coil maps
-stored on the masked voxels only), the raw k-space data with its k0_adc, both field
-coefficient sets, their density weights, and the two rendered reconstructions of the
+stored on the masked voxels only: they are zero elsewhere), the raw k-space data with
+its k0_adc, both field coefficient sets at the ADC times and as measured (with their
+start times, sampling intervals and the delays stored with the data), their density
+weights, and the two rendered reconstructions of the
 reference reduced to the 200 x 200 image grid: the plot area of each PNG (below the
 title) box-averaged onto 200 x 200 pixels, gray levels in [0, 1]. The files are large
 (about 16 MB each) and are generated, not committed. Run from the repository root:
@@ -52,6 +54,10 @@ def main(ref):
             weight_measured=meas["weight"], weight_nominal=nom["weight"],
             tau_measured=float(raw["tauMeasured"]), tau_nominal=float(raw["tauNominal"]),
             dt_adc=float(raw["dt_adc"]),
+            field_measured=np.asarray(raw["ksphaMeasured"], dtype=np.float32),
+            field_nominal=np.asarray(raw["ksphaNominal"], dtype=np.float32),
+            start_measured=float(raw["startMeasured"]), start_nominal=float(raw["startNominal"]),
+            dt_measured=float(raw["dt_Measured"]), dt_nominal=float(raw["dt_Nominal"]),
             golden_measured=rendered_image(demo / "result" / f"{stem}_Measured.png", g.nX),
             golden_nominal=rendered_image(demo / "result" / f"{stem}_Nominal.png", g.nX),
         )

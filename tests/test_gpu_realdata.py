@@ -16,7 +16,8 @@ tests/golden/make_7t_fixture.py. Checks:
   not exactly: the bars (0.99 spiral, 0.98 EPI) sit above what a one-pixel shift of the
   same image or the other trajectory's reference image scores, and each image must match
   its own reference image better than the other one by 0.02;
-- the mixed and FP32 backends give the same reconstruction to 1e-3.
+- the mixed and FP32 backends give the same reconstruction to 1e-3;
+- FindDelay on the raw field samples recovers the delays stored with the data.
 """
 import sys
 from pathlib import Path
@@ -90,3 +91,38 @@ def test_recon_matches_reference_image(name, trajectory, gpu_lib):
     # shift of the same image scores 0.94-0.985, the other trajectory's image 0.93-0.96
     assert c_same > (0.99 if name == "spiral" else 0.98), c_same
     assert c_same > c_other + 0.02, (c_same, c_other)
+
+
+@pytest.mark.parametrize("name,trajectory", CASES)
+def test_find_delay_recovers_the_delays_stored_with_the_data(name, trajectory, gpu_lib):
+    """FindDelay (FindDelay.jl:139-214) on the raw field samples and data of the demo,
+    against tauMeasured / tauNominal stored in the same file (the offsets the demo adds
+    to the start times, DemoCommon.jl:48-49). Measured coefficients: data divided by
+    exp(2 pi i k0_adc) only (the zeroth-order phase depends on the delay, so it stays in
+    the model: recon terms "1111"); nominal: raw data, recon terms "010". Density weights
+    of each trial delay from hoe.fieldcam. Measured on one box: spiral -0.687 / 4.902
+    against -0.568 / 5.008 dwells, EPI -0.398 / 1.648 against -0.408 / 1.675. The
+    reference's weights come from NFFT.jl (not vendored); the spiral's strongly
+    non-uniform density makes its estimate the more weight-dependent, hence the wider
+    bar there."""
+    p = load(name, trajectory)
+    z = np.load(GOLDEN / f"7t_{name}.npz")
+    nX, nY = p["grid"].nX, p["grid"].nY
+    csm = p["csm"].reshape(nX, nY, -1, order="F")
+    b0 = p["fieldmap"].reshape(nX, nY, order="F")
+    dt_adc = float(z["dt_adc"])
+    if trajectory == "measured":
+        data = (z["kdata"] / np.exp(2j * np.pi * z["k0_adc"])[:, None]).astype(np.complex64)
+        field, start, dt, terms, tau_ref = (z["field_measured"], z["start_measured"], z["dt_measured"], "1111",
+                                            float(z["tau_measured"]))
+    else:
+        data = z["kdata"]
+        field, start, dt, terms, tau_ref = (z["field_nominal"], z["start_nominal"], z["dt_nominal"], "010",
+                                            float(z["tau_nominal"]))
+    weight = lambda k: hoe.samplingDensity(np.asarray(k)[1:3], (nX, nY))
+    tau = hoe.FindDelay(p["grid"], data, field, p["times"], np.float32(start), np.float32(dt),
+                        Δτ_min=np.float32(0.005 * dt_adc), fieldmap=b0, csm=csm, recon_terms=terms, iter_max=10,
+                        weight=weight, max_updates=60)
+    est = float(tau) / dt_adc
+    print(f"{name} {trajectory}: FindDelay {est:.4f} dwells, stored {tau_ref:.4f}", flush=True)
+    assert abs(est - tau_ref) < (0.15 if name == "spiral" else 0.05), (est, tau_ref)
