@@ -265,6 +265,9 @@ int set_weights(Handle *h, const float *D, int64_t n_d) {
             if (D) v = (n_d == ns) ? D[s] : D[c * ns + s];
             w[(size_t)s * h->ncs + c] = v;
         }
+    // work queued with HOE_ASYNC may still read the current weights: the copy (on the
+    // null stream, which a non-blocking stream does not wait for) goes in after it
+    HOE_TRY_HIP(h, hipStreamSynchronize(h->stream));
     HOE_TRY_HIP(h, hipMemcpy(h->dvec.p, w.data(), w.size() * sizeof(float), hipMemcpyHostToDevice));
     h->weights_unit = (D == nullptr);
     return HOE_OK;

@@ -74,3 +74,35 @@ def test_cg_workload0_matches_fp64_cg(backend, gpu_lib):
     print(f"workload0 CG {backend}: image vs FP64 CG {err:.2e}; residual {log[-1]:.3e} vs {log64[-1]:.3e}")
     assert err < 1e-3
     assert abs(log[-1] - log64[-1]) <= 0.05 * log64[-1] + 1e-6
+
+
+@pytest.mark.parametrize("backend", ["direct", "mixed_fp16"])
+def test_new_weights_wait_for_queued_work(backend, gpu_lib):
+    """A CG queued with HOE_ASYNC keeps the weights it was queued with when the host
+    sets new ones right after (the upload waits for the handle's queued work): the
+    result equals a synchronous CG with the first weights, bit for bit."""
+    import ctypes as C
+    import torch
+    from hoe import _lib
+    c = synth.make(0)
+    ns = c["kspha"].shape[1]
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(5)
+    y = torch.from_numpy((rng.standard_normal((8, ns)) + 1j * rng.standard_normal((8, ns))).astype(np.complex64)
+                         .reshape(-1)).to(dev)
+    d1 = np.linspace(0.5, 2.0, ns).astype(np.float32)
+    d2 = np.full(ns, 1e-3, dtype=np.float32)
+    with hoe.HighOrderExplicitOp(c["grid"], c["kspha"], c["times"], fieldmap=c["fieldmap"], csm=c["csm"],
+                                 backend=backend) as op:
+        lib, h = op._lib, op._handle()
+        x_sync = torch.empty(c["grid"].n, dtype=torch.complex64, device=dev)
+        x_async = torch.empty_like(x_sync)
+        flags = _lib.HOE_DEVICE_PTRS
+        _lib.check(lib.hoe_cg(h, y.data_ptr(), d1.ctypes.data, ns, 30, 0.0, 0.0, x_sync.data_ptr(), None, flags), h)
+        torch.cuda.synchronize()
+        _lib.check(lib.hoe_cg(h, y.data_ptr(), d1.ctypes.data, ns, 30, 0.0, 0.0, x_async.data_ptr(), None,
+                              flags | _lib.HOE_ASYNC), h)
+        _lib.check(lib.hoe_set_weights(h, d2.ctypes.data, ns), h)
+        _lib.check(lib.hoe_sync(h), h)
+        torch.cuda.synchronize()
+        assert torch.equal(x_sync, x_async)
