@@ -77,6 +77,35 @@ def _ptr(o):
     return o.data_ptr() if _is_torch(o) else o.ctypes.data
 
 
+def _out(out, vec: _Vec, n: int, name: str = "out"):
+    """The output vector: a new one like ``vec``, or the caller's ``out`` after checking
+    that libhoe can write n complex64 values into it in place (same kind as the input,
+    host or device, contiguous, exact length)."""
+    if out is None:
+        return vec.like(n)
+    if vec.dev:
+        if not (_is_torch(out) and out.is_cuda):
+            raise ArgumentError(f"{name} must be a device tensor when the input is one")
+        if out.device != vec.obj.device:
+            raise ArgumentError(f"{name} must be on the input's device")
+        if out.dtype != torch.complex64:
+            raise ArgumentError(f"{name} must be complex64 (ComplexF32)")
+        if not out.is_contiguous():
+            raise ArgumentError(f"{name} must be contiguous")
+        size = out.numel()
+    else:
+        if not isinstance(out, np.ndarray):
+            raise ArgumentError(f"{name} must be a numpy array when the input is a host array")
+        if out.dtype != np.complex64:
+            raise ArgumentError(f"{name} must be complex64 (ComplexF32)")
+        if not (out.flags.c_contiguous or out.flags.f_contiguous) or not out.flags.writeable:
+            raise ArgumentError(f"{name} must be a contiguous, writeable array")
+        size = out.size
+    if size != n:
+        raise DimensionMismatch(f"{name} has length {size}, expected {n}")
+    return out
+
+
 class HighOrderExplicitOp:
     """Explicit higher-order encoding operator of size ``(nSam*nCha, prod(matrixSize))``.
 
@@ -272,12 +301,19 @@ class HighOrderExplicitOp:
             knp = np.ascontiguousarray(kn.T)
         check(self._lib.hoe_set_kspha(h, kt.ctypes.data, None if knp is None else knp.ctypes.data), h)
 
+    def _on_device(self, *vecs):
+        """Device vectors must live on the operator's GPU (multi_gpu_operator.jl:81-89)."""
+        for v in vecs:
+            if v.dev and v.obj.device.index != self.device:
+                raise ArgumentError(f"device vectors must be on the operator's GPU (device {self.device})")
+
     def prod(self, x, out=None):
         """``op * x``: x of length prod(matrixSize) -> signal of length nSam*nCha
         (B x while ``kspha_dt`` is set)."""
         h = self._handle()
         xv = _Vec(x, self.ncol, "x")
-        y = out if out is not None else xv.like(self.nrow)
+        self._on_device(xv)
+        y = _out(out, xv, self.nrow)
         flags = _lib.HOE_DEVICE_PTRS if xv.dev else 0
         if xv.dev:
             torch.cuda.current_stream(xv.obj.device).synchronize()
@@ -288,7 +324,8 @@ class HighOrderExplicitOp:
         """``adjoint(op) * y``."""
         h = self._handle()
         yv = _Vec(y, self.nrow, "y")
-        x = out if out is not None else yv.like(self.ncol)
+        self._on_device(yv)
+        x = _out(out, yv, self.ncol)
         flags = _lib.HOE_DEVICE_PTRS if yv.dev else 0
         if yv.dev:
             torch.cuda.current_stream(yv.obj.device).synchronize()
@@ -325,6 +362,7 @@ class HighOrderExplicitOp:
         h = self._handle()
         xv = _Vec(x, self.ncol, "x")
         yv = _Vec(y, self.nrow, "y")
+        self._on_device(xv, yv)
         if xv.dev != yv.dev:
             raise ArgumentError("x and y must both be host or both be device arrays")
         flags = _lib.HOE_DEVICE_PTRS if xv.dev else 0
@@ -446,7 +484,8 @@ class NormalOp:
         h = self.op._handle()
         self._sync_weights(h)
         xv = _Vec(x, self.op.ncol, "x")
-        o = out if out is not None else xv.like(self.op.ncol)
+        self.op._on_device(xv)
+        o = _out(out, xv, self.op.ncol)
         flags = _lib.HOE_DEVICE_PTRS if xv.dev else 0
         if xv.dev:
             torch.cuda.current_stream(xv.obj.device).synchronize()
@@ -521,6 +560,7 @@ def recon_HOOp(HOOp: HighOrderExplicitOp, Data, weight, recParams: dict, *, retu
     else:
         d = np.asarray(data)
         yv = _Vec(d.reshape(-1, order="F") if d.ndim == 2 else d, op.nrow, "Data")
+    op._on_device(yv)
     w = weight.detach().cpu().numpy() if _is_torch(weight) else np.asarray(weight)
     w = w.reshape(-1)
     if w.size != op.nSam:

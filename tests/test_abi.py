@@ -166,3 +166,25 @@ def test_product_does_not_reference_the_oracle():
             list(pkg.rglob("*.h")) + list(pkg.rglob("Makefile")):
         text = f.read_text()
         assert "import oracle" not in text and "liboracle" not in text and "hoe_oracle" not in text, f
+
+
+def test_out_vector_checks_on_host_arrays():
+    """The output-vector checks of the operator wrapper (host arrays, no GPU needed)."""
+    import numpy as np
+    from hoe import operator as opm
+    v = opm._Vec(np.zeros(6, np.complex64), 6, "x")
+    assert opm._out(None, v, 4).shape == (4,)
+    ok = np.empty(4, np.complex64)
+    assert opm._out(ok, v, 4) is ok
+    with pytest.raises(hoe.DimensionMismatch):
+        opm._out(np.empty(5, np.complex64), v, 4)
+    with pytest.raises(hoe.ArgumentError):
+        opm._out(np.empty(4, np.complex128), v, 4)
+    with pytest.raises(hoe.ArgumentError):
+        opm._out(np.empty(8, np.complex64)[::2], v, 4)
+    ro = np.empty(4, np.complex64)
+    ro.flags.writeable = False
+    with pytest.raises(hoe.ArgumentError):
+        opm._out(ro, v, 4)
+    with pytest.raises(hoe.ArgumentError):
+        opm._out([0j] * 4, v, 4)

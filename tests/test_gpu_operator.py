@@ -214,3 +214,34 @@ def test_concomitant_planar(backend, nz, zoff, gpu_lib):
         assert O.relerr(op * x, A @ x) < TOL32
         assert O.relerr(op.H * y, A.conj().T @ y) < TOL32
         assert O.relerr(hoe.normalOperator(op) * x, A.conj().T @ (A @ x)) < TOL32
+
+
+def test_out_vectors_are_checked_and_written_in_place(gpu_lib):
+    """out= must match the input's kind (host/device), be complex64, contiguous and of
+    the operator's length; a valid one is written in place."""
+    import torch
+    p = O.explicit_highorder_test_problem()
+    x = p["image"]
+    with _op(p, "mixed_fp16") as op:
+        y_ref = op * x
+        xd = torch.from_numpy(x).to("cuda:0")
+        good = torch.empty(op.nrow, dtype=torch.complex64, device="cuda:0")
+        assert op.prod(xd, out=good) is good
+        torch.cuda.synchronize()
+        assert O.relerr(good.cpu().numpy(), y_ref) < 1e-6
+        host = np.empty(op.nrow, dtype=np.complex64)
+        assert op.prod(x, out=host) is host and O.relerr(host, y_ref) < 1e-6
+        with pytest.raises(hoe.ArgumentError):
+            op.prod(xd, out=np.empty(op.nrow, dtype=np.complex64))             # host out, device input
+        with pytest.raises(hoe.ArgumentError):
+            op.prod(x, out=good)                                               # device out, host input
+        with pytest.raises(hoe.DimensionMismatch):
+            op.prod(xd, out=torch.empty(op.nrow - 1, dtype=torch.complex64, device="cuda:0"))
+        with pytest.raises(hoe.ArgumentError):
+            op.prod(xd, out=torch.empty(op.nrow, dtype=torch.complex128, device="cuda:0"))
+        with pytest.raises(hoe.ArgumentError):
+            op.prod(xd, out=torch.empty(2 * op.nrow, dtype=torch.complex64, device="cuda:0")[::2])
+        with pytest.raises(hoe.ArgumentError):
+            op.ctprod(y_ref, out=np.empty(op.ncol, dtype=np.complex128))
+        with pytest.raises(hoe.ArgumentError):
+            hoe.normalOperator(op).prod(xd, out=np.empty(op.ncol, dtype=np.complex64))
