@@ -72,7 +72,7 @@ def test_fixture_matches_dense(backend, gpu_lib):
 
 @pytest.mark.parametrize("backend", BACKENDS)
 @pytest.mark.parametrize("shape,nt,nc", [((5, 4, 1), 9, 3), ((4, 3, 2), 16, 33), ((4, 5, 1), 16, 5),
-                                         ((3, 4, 2), 9, 33)])
+                                         ((3, 4, 2), 9, 33), ((6, 5, 1), 16, 96), ((3, 3, 3), 16, 130)])
 def test_independent_fp64_reference(backend, shape, nt, nc, gpu_lib):
     TOL32 = TOLS[backend][0]
     c = _random_case(shape, nt, nc, 1234 + nt + nc)
