@@ -540,14 +540,12 @@ int hoe_create(const hoe_model_desc *d, hoe_handle **out) {
             }
         }
         h->horner = ok && (double)nv / (double)h->n_runs >= 16.0;
-        // the MFMA adjoint evaluates a wave's 64 voxels from the Horner tables of at most two runs
-        bool two_runs = h->horner;
-        for (int64_t w = 0; w + 63 < h->nv_pad && two_runs; w += 64)
-            two_runs = vrun[(size_t)w + 63] - vrun[(size_t)w] <= 1;
+        // the MFMA adjoint evaluates a wave's 64 voxels from the Horner tables of at most
+        // two runs; waves spanning more (short rows of a mask) take the generic phase
 #ifndef HOE_HORNER_ADJ
 #define HOE_HORNER_ADJ 1
 #endif
-        h->horner_adj = HOE_HORNER_ADJ && two_runs;
+        h->horner_adj = HOE_HORNER_ADJ && h->horner;
     }
 
     hipDeviceProp_t prop;

@@ -45,7 +45,7 @@ struct Handle {
     int32_t LK = 0;                   // kernel record length: active terms + fieldmap, padded to a bucket
     std::vector<int> active;          // 0-based term indices (>= 1) with a not identically zero basis
     bool horner = false;              // forward kernels evaluate the phase per run by Horner in x
-    bool horner_adj = false;          // MFMA adjoint by Horner: every 64-voxel wave spans <= 2 runs
+    bool horner_adj = false;          // MFMA adjoint by Horner (per wave, when its 64 voxels span <= 2 runs)
     int32_t n_runs = 0;               // runs: consecutive masked voxels sharing (y, z)
     int32_t t_index = 0;              // index of t in the sample record (= number of active terms + 1)
     int32_t n_term_in = 0;            // 9 or 16

@@ -350,6 +350,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (
                 }
             }
         };
+        // generic phase: the lane's full record in registers, the streamed records
+        // from scalar loads, L FP64 FMAs per phase
+        auto generic_path = [&]() {
+            double st[NST];
+            {
+                const double *p = FWD ? coef + (size_t)row * sk : basis + (size_t)row * sb;
+#pragma unroll
+                for (int i = 0; i < NST; ++i) st[i] = p[i];
+            }
+            kloop([&](int ks, double *ph) {
+#pragma unroll
+                for (int j = 0; j < 8; ++j) {
+                    const int q = ks * 8 + j;
+                    const double *sm = FWD ? basis + (size_t)q * sb : coef + (size_t)q * sk;
+                    double p;
+                    if (FWD) {
+                        p = st[0];
+#pragma unroll
+                        for (int i = 1; i <= L; ++i) p = fma(st[i], sm[i - 1], p);
+                    } else {
+                        p = sm[L] * st[L - 1];
+#pragma unroll
+                        for (int i = 1; i < L; ++i) p = fma(sm[i], st[i - 1], p);
+                        p += sm[0];
+                    }
+                    ph[j] = p;
+                }
+            }, std::true_type{});
+        };
         if constexpr (HX && FWD) {
             // Horner forward: c_j set up per run, 4 FP64 FMAs per phase
             // Runs are contiguous, so the 8 rows of a K step share one run when
@@ -387,8 +416,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (
                 }
             }, std::true_type{});
         } else if constexpr (HX && !FWD) {
-            // Horner adjoint. The wave's 64 voxels lie in at most two runs (r_lo, r_hi;
-            // checked on the host). For each pair of K steps one FP64 matrix product
+            // A wave whose 64 voxels span three or more runs (short rows of a mask)
+            // takes the generic phase; the decision is uniform over the wave.
+            const int wv0 = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+            const int base0 = d.x * 256 + wv0 * 64;
+            if (hx.vrun[base0 + 63] - hx.vrun[base0] > 1) {
+                generic_path();
+            } else {
+            // Horner adjoint. The wave's 64 voxels lie in at most two runs (r_lo, r_hi).
+            // For each pair of K steps one FP64 matrix product
             // gives every coefficient the pair needs: rows = its 16 streamed samples,
             // columns 0..7 = c_j(s, run) = sum_i A_s[i] beta[run][i][j] for (run, j),
             // column 8 = t_s. The 16x16 result goes through a wave-private LDS slot;
@@ -486,32 +522,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (
                     }
                 }
             }, std::false_type{});
-        } else {
-            double st[NST];
-            {
-                const double *p = FWD ? coef + (size_t)row * sk : basis + (size_t)row * sb;
-#pragma unroll
-                for (int i = 0; i < NST; ++i) st[i] = p[i];
             }
-            kloop([&](int ks, double *ph) {
-#pragma unroll
-                for (int j = 0; j < 8; ++j) {
-                    const int q = ks * 8 + j;
-                    const double *sm = FWD ? basis + (size_t)q * sb : coef + (size_t)q * sk;
-                    double p;
-                    if (FWD) {
-                        p = st[0];
-#pragma unroll
-                        for (int i = 1; i <= L; ++i) p = fma(st[i], sm[i - 1], p);
-                    } else {
-                        p = sm[L] * st[L - 1];
-#pragma unroll
-                        for (int i = 1; i < L; ++i) p = fma(sm[i], st[i - 1], p);
-                        p += sm[0];
-                    }
-                    ph[j] = p;
-                }
-            }, std::true_type{});
+        } else {
+            generic_path();
         }
         // D layout of v_mfma_f32_32x32x*: register r of lane l holds
         // row 8*(r/4) + 4*(l/32) + r%4, column l%32.
