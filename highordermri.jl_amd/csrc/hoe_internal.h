@@ -125,7 +125,7 @@ int direct_cb(const Handle *h);        // coils per direct-kernel batch: 8, 16 o
 int direct_batches(const Handle *h);
 int mixed_nb(const Handle *h);         // 32-coil groups per wave in the MFMA kernels: 1 or 2
 int mixed_groups(const Handle *h);
-hipError_t build_schedule(Sched &sc, int tiles, int n_ks, int slots);
+hipError_t build_schedule(Sched &sc, int tiles, int n_ks, int slots, int kmax);
 // Build the forward/adjoint schedules of a backend and size the partial buffers.
 hipError_t prepare_backend(Handle *h, int backend);
 hipError_t prepare_grad(Handle *h);

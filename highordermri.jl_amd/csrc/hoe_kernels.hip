@@ -40,6 +40,15 @@ namespace hoe {
 #ifndef HOE_ADJ_LDS
 #define HOE_ADJ_LDS 1
 #endif
+// Longest segment of the MFMA adjoint, in K units of 8 samples. Its FP32 accumulators
+// lose accuracy with chain length (measured on a 3-D workload with 384 000 random
+// samples: 2.8e-5 relative with whole K parts of 6 000 units, 1.05e-5 with 1 024,
+// 7.2e-6 with 512), so longer runs are cut into segments with partial sums of their
+// own. The forward's error on the same workload (whole K parts of 6 912 units) is
+// 9.6e-6 for a random image, so its segments stay uncut.
+#ifndef HOE_MFMA_ADJ_KMAX
+#define HOE_MFMA_ADJ_KMAX 1024
+#endif
 
 typedef _Float16 h8_t __attribute__((ext_vector_type(8)));
 typedef __bf16 b8_t __attribute__((ext_vector_type(8)));
@@ -1608,7 +1617,9 @@ static hipError_t upload_ints(DeviceBuf &b, const std::vector<int> &v) {
 
 // Cut tiles x n_ks units into 8 K-parts (one per XCD) and each part into
 // `slots/8` equal chunks of consecutive (tile, k) units; record the segments.
-hipError_t build_schedule(Sched &sc, int tiles, int n_ks, int slots) {
+// kmax > 0 cuts a chunk's run of one tile into near-equal segments of at most
+// kmax units.
+hipError_t build_schedule(Sched &sc, int tiles, int n_ks, int slots, int kmax) {
     const int X = 8;
     sc.tiles = tiles;
     sc.n_ks = n_ks;
@@ -1627,10 +1638,13 @@ hipError_t build_schedule(Sched &sc, int tiles, int n_ks, int slots) {
             while (u < u1) {
                 const int t = (int)(u / Kx), kk = (int)(u % Kx);
                 const int len = (int)std::min<int64_t>(u1 - u, Kx - kk);
-                seg.push_back(t);
-                seg.push_back(K0 + kk);
-                seg.push_back(K0 + kk + len);
-                seg.push_back(0);
+                const int pieces = kmax > 0 ? (len + kmax - 1) / kmax : 1;
+                for (int p = 0; p < pieces; ++p) {
+                    seg.push_back(t);
+                    seg.push_back(K0 + kk + (int)((int64_t)len * p / pieces));
+                    seg.push_back(K0 + kk + (int)((int64_t)len * (p + 1) / pieces));
+                    seg.push_back(0);
+                }
                 u += len;
             }
         }
@@ -1678,8 +1692,10 @@ hipError_t prepare_backend(Handle *h, int backend) {
     const int slots_f = std::max(8, h->cu_count * occ_f / groups);
     const int slots_a = std::max(8, h->cu_count * occ_a / groups);
     hipError_t e;
-    if ((e = build_schedule(sf, (int)(h->ns_pad / 256), (int)(h->nv_pad / 8), slots_f)) != hipSuccess) return e;
-    if ((e = build_schedule(sa, (int)(h->nv_pad / 256), (int)(h->ns_pad / 8), slots_a)) != hipSuccess) return e;
+    const int kmax_a = backend == HOE_BACKEND_DIRECT ? 0 : HOE_MFMA_ADJ_KMAX;
+    if ((e = build_schedule(sf, (int)(h->ns_pad / 256), (int)(h->nv_pad / 8), slots_f, 0)) != hipSuccess) return e;
+    if ((e = build_schedule(sa, (int)(h->nv_pad / 256), (int)(h->ns_pad / 8), slots_a, kmax_a)) != hipSuccess)
+        return e;
     h->occ_f[backend] = occ_f;
     h->occ_a[backend] = occ_a;
     if ((e = grow(h->part_f, (size_t)sf.n_seg * 256 * h->ncs * sizeof(float2))) != hipSuccess) return e;
@@ -1693,7 +1709,7 @@ hipError_t prepare_grad(Handle *h) {
     visit_grad(h->LK, direct_cb(h), [&](auto k) { occ = blocks_per_cu(k); });
     const int nb = direct_batches(h);
     const int slots = std::max(8, h->cu_count * occ / nb);
-    hipError_t e = build_schedule(sg, (int)(h->ns_pad / 256), (int)(h->nv_pad / 8), slots);
+    hipError_t e = build_schedule(sg, (int)(h->ns_pad / 256), (int)(h->nv_pad / 8), slots, 0);
     if (e != hipSuccess) return e;
     return grow(h->part_g, (size_t)sg.n_seg * nb * 256 * h->LK * sizeof(double));
 }
@@ -1712,7 +1728,7 @@ static hipError_t prepare_grad_mfma(Handle *h) {
             if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, 256, kGradLds) == hipSuccess && nb > 0) occ = nb;
         });
         const int slots = std::max(8, h->cu_count * occ);
-        if ((e = build_schedule(sg, (int)(h->ns_pad / 256), (int)(h->nv_pad / 32), slots)) != hipSuccess) return e;
+        if ((e = build_schedule(sg, (int)(h->ns_pad / 256), (int)(h->nv_pad / 32), slots, 0)) != hipSuccess) return e;
         if ((e = grow(h->part_gm, (size_t)sg.n_seg * 256 * 32 * sizeof(float))) != hipSuccess) return e;
         const size_t nkc = (size_t)h->ncs / 8;
         if ((e = grow(h->gzfrag, (size_t)(h->nv_pad / 32) * nkc * 64 * 64)) != hipSuccess) return e;
