@@ -685,7 +685,9 @@ int stage_in(Handle *h, int flags, const void *src, hoe::DeviceBuf &buf, size_t 
 int finish(Handle *h, int flags, void *dst_host, const void *dsrc, size_t bytes) {
     if (!(flags & HOE_DEVICE_PTRS) && dst_host)
         HOE_TRY_HIP(h, hipMemcpyAsync(dst_host, dsrc, bytes, hipMemcpyDeviceToHost, h->stream));
-    (void)hipEventRecord(h->ev[5], h->stream);
+    // timing events only on synchronous calls: each event on the stream costs a gap of
+    // several microseconds between the kernels it separates
+    if (!(flags & HOE_ASYNC)) (void)hipEventRecord(h->ev[5], h->stream);
     if (!(flags & HOE_ASYNC) || !(flags & HOE_DEVICE_PTRS)) HOE_TRY_HIP(h, hipStreamSynchronize(h->stream));
     HOE_TRY_HIP(h, hipGetLastError());
     return HOE_OK;
@@ -704,11 +706,11 @@ int hoe_forward(hoe_handle *hh, const void *x, void *y, int flags) {
     int rc = check_open(h);
     if (rc) return rc;
     if (!y) return set_err(h, HOE_EINVAL, "output pointer is NULL");
-    (void)hipEventRecord(h->ev[4], h->stream);
+    if (!(flags & HOE_ASYNC)) (void)hipEventRecord(h->ev[4], h->stream);
     const void *dx;
     if ((rc = stage_in(h, flags, x, h->io_x, sizeof(float2) * h->n_grid, &dx))) return rc;
     void *dy = (flags & HOE_DEVICE_PTRS) ? y : h->io_y.p;
-    HOE_TRY_HIP(h, hoe::run_forward_contract(h, dx, true));
+    HOE_TRY_HIP(h, hoe::run_forward_contract(h, dx, !(flags & HOE_ASYNC)));
     HOE_TRY_HIP(h, hoe::launch_reduce_fwd_user(h, dy, (float)(1.0 / std::sqrt((double)h->n_vox))));
     if ((rc = finish(h, flags, y, dy, sizeof(float2) * h->n_sam * h->n_cha))) return rc;
     if (!(flags & HOE_ASYNC)) record_times(h, true, false);
@@ -720,12 +722,13 @@ int hoe_adjoint(hoe_handle *hh, const void *y, void *x, int flags) {
     int rc = check_open(h);
     if (rc) return rc;
     if (!x) return set_err(h, HOE_EINVAL, "output pointer is NULL");
-    (void)hipEventRecord(h->ev[4], h->stream);
+    if (!(flags & HOE_ASYNC)) (void)hipEventRecord(h->ev[4], h->stream);
     const void *dy;
     if ((rc = stage_in(h, flags, y, h->io_y, sizeof(float2) * h->n_sam * h->n_cha, &dy))) return rc;
     void *dx = (flags & HOE_DEVICE_PTRS) ? x : h->io_x.p;
     HOE_TRY_HIP(h, hoe::launch_load_ybuf(h, dy, false));
-    HOE_TRY_HIP(h, hoe::run_adjoint_contract(h, (float)(1.0 / std::sqrt((double)h->n_vox)), h->xm.p, true));
+    HOE_TRY_HIP(h, hoe::run_adjoint_contract(h, (float)(1.0 / std::sqrt((double)h->n_vox)), h->xm.p,
+                                             !(flags & HOE_ASYNC)));
     HOE_TRY_HIP(h, hoe::launch_scatter(h, h->xm.p, dx, 1.0f));
     if ((rc = finish(h, flags, x, dx, sizeof(float2) * h->n_grid))) return rc;
     if (!(flags & HOE_ASYNC)) record_times(h, false, true);
@@ -747,12 +750,12 @@ int hoe_normal(hoe_handle *hh, const float *D, int64_t n_d, const void *x, void 
     if (!out) return set_err(h, HOE_EINVAL, "output pointer is NULL");
     if (!(D == nullptr && n_d == kCurrentWeights))
         if ((rc = set_weights(h, D, n_d))) return rc;
-    (void)hipEventRecord(h->ev[4], h->stream);
+    if (!(flags & HOE_ASYNC)) (void)hipEventRecord(h->ev[4], h->stream);
     const void *dx;
     if ((rc = stage_in(h, flags, x, h->io_x, sizeof(float2) * h->n_grid, &dx))) return rc;
     void *dout = (flags & HOE_DEVICE_PTRS) ? out : h->io_x2.p;
     float2 *tmp = reinterpret_cast<float2 *>(h->cg_vec.p);   // masked output
-    HOE_TRY_HIP(h, hoe::run_normal_full(h, dx, tmp, (float)(1.0 / (double)h->n_vox), true));
+    HOE_TRY_HIP(h, hoe::run_normal_full(h, dx, tmp, (float)(1.0 / (double)h->n_vox), !(flags & HOE_ASYNC)));
     HOE_TRY_HIP(h, hoe::launch_scatter(h, tmp, dout, 1.0f));
     if ((rc = finish(h, flags, out, dout, sizeof(float2) * h->n_grid))) return rc;
     if (!(flags & HOE_ASYNC)) record_times(h, true, true);
@@ -818,7 +821,7 @@ int hoe_grad(hoe_handle *hh, const void *x, const void *y, float *g_h, double *g
     int rc = check_open(h);
     if (rc) return rc;
     if ((rc = reject_dt(h))) return rc;
-    (void)hipEventRecord(h->ev[4], h->stream);
+    (void)hipEventRecord(h->ev[4], h->stream);   // the call always waits for its results
     const void *dx, *dy;
     if ((rc = stage_in(h, flags, x, h->io_x, sizeof(float2) * h->n_grid, &dx))) return rc;
     if ((rc = stage_in(h, flags, y, h->io_y, sizeof(float2) * h->n_sam * h->n_cha, &dy))) return rc;
@@ -870,7 +873,7 @@ int hoe_cg(hoe_handle *hh, const void *y, const float *D, int64_t n_d, int32_t i
         dfree(h->cg_log);
         HOE_TRY_HIP(h, dalloc(h->cg_log, log_bytes));
     }
-    (void)hipEventRecord(h->ev[4], h->stream);
+    if (!(flags & HOE_ASYNC)) (void)hipEventRecord(h->ev[4], h->stream);
     const void *dy;
     if ((rc = stage_in(h, flags, y, h->io_y, sizeof(float2) * h->n_sam * h->n_cha, &dy))) return rc;
     void *dx = (flags & HOE_DEVICE_PTRS) ? x_out : h->io_x.p;

@@ -1147,18 +1147,38 @@ __global__ __launch_bounds__(256) void k_reduce_fwd(
         }
         __syncthreads();
     }
-    // four rows per thread, accumulated together so that 4 loads are in flight
-    float2 rows[4];
+    // four rows per thread and two segments per step, in separate sums, so that 8
+    // loads are in flight
+    float2 rows[4], rows2[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) rows[q] = make_float2(0.0f, 0.0f);
-    for (int j = 0; j < nseg; ++j) {
-        const size_t base = (size_t)list[j] * 256;
+    for (int q = 0; q < 4; ++q) rows[q] = rows2[q] = make_float2(0.0f, 0.0f);
+    int j = 0;
+    for (; j + 2 <= nseg; j += 2) {
+        const size_t b0 = (size_t)list[j] * 256, b1 = (size_t)list[j + 1] * 256;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const float2 t = part[(base + ((s0 + tr + 8 * q) & 255)) * ncs + c0 + tc];
+            const int r = (s0 + tr + 8 * q) & 255;
+            const float2 t0 = part[(b0 + r) * ncs + c0 + tc];
+            const float2 t1 = part[(b1 + r) * ncs + c0 + tc];
+            rows[q].x += t0.x;
+            rows[q].y += t0.y;
+            rows2[q].x += t1.x;
+            rows2[q].y += t1.y;
+        }
+    }
+    if (j < nseg) {
+        const size_t b0 = (size_t)list[j] * 256;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const float2 t = part[(b0 + ((s0 + tr + 8 * q) & 255)) * ncs + c0 + tc];
             rows[q].x += t.x;
             rows[q].y += t.y;
         }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        rows[q].x += rows2[q].x;
+        rows[q].y += rows2[q].y;
     }
     float m = 0.0f;
 #pragma unroll

@@ -218,8 +218,9 @@ void hoe_destroy(hoe_handle *h);
 int hoe_sync(hoe_handle *h);
 
 /* Device timing of the most recent hoe_forward / hoe_adjoint / hoe_normal /
- * hoe_grad call: milliseconds spent in that call's dominant kernel launches
- * (per launch, averaged) measured with HIP events on the handle's stream.
+ * hoe_grad call made without HOE_ASYNC: milliseconds spent in that call's
+ * dominant kernel launches (per launch, averaged) measured with HIP events on
+ * the handle's stream. Calls with HOE_ASYNC record no events.
  * which: 0 = forward contraction kernel, 1 = adjoint contraction kernel,
  * 2 = whole call. Returns a negative value when unavailable. */
 double hoe_last_kernel_ms(const hoe_handle *h, int which);
