@@ -13,7 +13,9 @@ states them for small shapes):
 - the adjoint identity <E x, y> = <x, Eᴴ y>;
 - the normal operator against Eᴴ(E x) from the separate products.
 Bars: 3e-5 relative L2 for the sampled rows and entries (the Float32 GPU-operator
-bar of test/test_explicit_gpu.jl:51), 1e-5 for the identities.
+bar of test/test_explicit_gpu.jl:51) and 1e-5 for the identities; the BF16 hi/lo
+backend (about 16 significant bits per product) is held to the 1e-4 mixed-precision
+target for both.
 """
 import numpy as np
 import pytest
@@ -44,7 +46,10 @@ def _picks(n, seed):
     return np.unique(idx)[:N_PICK]
 
 
-@pytest.mark.parametrize("backend", ["mixed_fp16", "direct"])
+BARS = {"mixed_fp16": (3e-5, 1e-5), "direct": (3e-5, 1e-5), "mixed_bf16": (1e-4, 1e-4)}
+
+
+@pytest.mark.parametrize("backend", ["mixed_fp16", "direct", "mixed_bf16"])
 def test_full_size_products(workload, backend, gpu_lib):
     w, c, x, y, n, ns, nc = workload
     print(f"workload {w} {backend}: {n} voxels, {ns} samples, {nc} coils", flush=True)
@@ -79,8 +84,9 @@ def test_full_size_products(workload, backend, gpu_lib):
     e_norm = O.relerr(nx, ehex)
     print(f"workload {w} {backend}: E x rows vs FP64 {e_fwd:.2e}, Eᴴy entries vs FP64 {e_adj:.2e}, "
           f"adjoint identity {e_dot:.2e}, EᴴE vs Eᴴ(E x) {e_norm:.2e}", flush=True)
+    bar, bar_id = BARS[backend]
     assert np.count_nonzero(ehy[~full]) == 0
-    assert e_fwd < 3e-5, e_fwd
-    assert e_adj < 3e-5, e_adj
-    assert e_dot < 1e-5, e_dot
-    assert e_norm < 1e-5, e_norm
+    assert e_fwd < bar, e_fwd
+    assert e_adj < bar, e_adj
+    assert e_dot < bar_id, e_dot
+    assert e_norm < bar_id, e_norm
