@@ -492,10 +492,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (
             // For each pair of K steps one FP64 matrix product
             // gives every coefficient the pair needs: rows = its 16 streamed samples,
             // columns 0..7 = c_j(s, run) = sum_i A_s[i] beta[run][i][j] for (run, j),
-            // column 8 = t_s. The 16x16 result goes through a wave-private LDS slot;
-            // each lane reads the (c0..c3, t) of its samples and runs and spends 4 FP64
-            // FMAs per phase on its two voxel rows, already in the A-operand layout.
-            constexpr int NM = (L + 4) / 4;   // k blocks of 4 record terms covering 0..L (t included)
+            // column 8 = t_s. The result goes through a wave-private LDS slot; each lane
+            // reads the (c0..c3, t) of its samples and runs and spends 4 FP64 FMAs per
+            // phase on its two voxel rows, already in the A-operand layout.
+            // The product runs over k blocks of 4 record terms. When L is a multiple of 4,
+            // covering t (record entry L at most) takes one more FP64 MFMA (64 cycles on
+            // gfx950) than the coefficients 0..L-1 alone, so t_s is then loaded with the
+            // records and written to the slot directly (measured at 16 terms: adjoint -2.3 %;
+            // at 10 terms, where it saves no MFMA, the extra load costs +3 %).
+            constexpr bool TCOL = (L % 4) != 0;
+            constexpr int NM = TCOL ? (L + 4) / 4 : (L + 3) / 4;
             typedef double d4v __attribute__((ext_vector_type(4)));
             __shared__ double hl[4][288];     // per wave, 2 slots: c [2 steps][2 runs][2 halves][4][4], t [2][2][4]
             __shared__ double hb[4][NM][64];  // per wave: B operands of the setup product (constant per segment)
@@ -517,7 +523,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (
                 if (col < 8) {
                     const int rr = min(r_lo + (col >> 2), r_hi);
                     if (term < L) b = hx.beta[((size_t)rr * L + term) * 4 + (col & 3)];
-                } else if (col == 8) {
+                } else if (TCOL && col == 8) {
                     b = term == hx.t_index ? 1.0 : 0.0;
                 }
                 bop[m * 64 + lane] = b;
@@ -526,17 +532,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (
             // A operand of the pair starting at step kp: rows 0..7 = samples of kp, rows
             // 8..15 = of kp + 1; zero where the step is outside the segment (the address
             // is then a valid row, so the load is in bounds whatever the compiler does)
-            auto load_a = [&](int kp, double *a) {
+            auto load_a = [&](int kp, double *a, double &t) {
                 const int r = lane & 15;
                 const bool valid = kp < d.z && (r < 8 || kp + 1 < d.z);
-                const double *ap = coef + ((size_t)(valid ? kp * 8 + r : d.y * 8)) * sk + kq;
+                const double *ap = coef + ((size_t)(valid ? kp * 8 + r : d.y * 8)) * sk;
 #pragma unroll
-                for (int m = 0; m < NM; ++m) a[m] = valid ? ap[4 * m] : 0.0;
+                for (int m = 0; m < NM; ++m) a[m] = valid ? ap[kq + 4 * m] : 0.0;
+                if constexpr (!TCOL) t = valid ? ap[hx.t_index] : 0.0;
             };
             // setup product of one pair into an LDS slot. v_mfma_f64_16x16x4: register r
             // of lane l holds row l/16 + 4 r, column l%16; row kq + 4 r is sample kq of
             // half r&1 of step r>>1 of the pair.
-            auto setup = [&](const double *a, double *slot) {
+            auto setup = [&](const double *a, double t, double *slot) {
                 d4v acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
                 for (int m = 0; m < NM; ++m)
@@ -545,10 +552,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (
                     double *w = slot + (col >> 2) * 32 + kq * 4 + (col & 3);
 #pragma unroll
                     for (int r = 0; r < 4; ++r) w[((r >> 1) * 4 + (r & 1)) * 16] = acc[r];
-                } else if (col == 8) {
-                    double *w = slot + 128 + kq;
+                }
+                // t of row R (sample 8 kp + R) at slot[128 + R]: step R/8, half (R/4)%2, sample R%4
+                if constexpr (TCOL) {
+                    if (col == 8) {
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) w[((r >> 1) * 2 + (r & 1)) * 4] = acc[r];
+                        for (int r = 0; r < 4; ++r) slot[128 + kq + 4 * r] = acc[r];
+                    }
+                } else if (kq == 0) {
+                    slot[128 + (lane & 15)] = t;
                 }
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
@@ -556,16 +568,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (
             };
             // software pipeline: the slot of pair p + 1 is filled during the second step of
             // pair p, and the sample records of pair p + 2 are fetched meanwhile
-            double anx[NM];
-            load_a(d.y, anx);
-            if (d.y < d.z) setup(anx, cl);
-            load_a(d.y + 2, anx);
+            double anx[NM], tnx = 0.0;
+            load_a(d.y, anx, tnx);
+            if (d.y < d.z) setup(anx, tnx, cl);
+            load_a(d.y + 2, anx, tnx);
             kloop([&](int ks, double *ph) {
                 const int rel = ks - d.y, k2 = rel & 1;
                 const double *cur = cl + ((rel >> 1) & 1) * 144;
                 if (k2 == 1 && ks + 1 < d.z) {
-                    setup(anx, cl + (((rel >> 1) & 1) ^ 1) * 144);
-                    load_a(ks + 3, anx);
+                    setup(anx, tnx, cl + (((rel >> 1) & 1) ^ 1) * 144);
+                    load_a(ks + 3, anx, tnx);
                 }
                 const double *tp = cur + 128 + (k2 * 2 + half) * 4;
                 const double *c0p = cur + ((k2 * 2 + rs0) * 2 + half) * 16;
