@@ -1724,7 +1724,15 @@ hipError_t prepare_backend(Handle *h, int backend) {
     const int slots_f = std::max(8, h->cu_count * occ_f / groups);
     const int slots_a = std::max(8, h->cu_count * occ_a / groups);
     hipError_t e;
-    const int kmax_a = backend == HOE_BACKEND_DIRECT ? 0 : HOE_MFMA_ADJ_KMAX;
+    // The cut adjoint segments cost partial-sum memory (about nVox nSam nCoil / 1024 bytes
+    // with 1 024-unit segments); beyond 8 GiB the cut length doubles until it fits.
+    int kmax_a = 0;
+    if (backend != HOE_BACKEND_DIRECT) {
+        const int64_t tiles = h->nv_pad / 256, kx = (h->ns_pad / 8 + 7) / 8;
+        const int64_t seg_bytes = (int64_t)256 * h->ncs * (int64_t)sizeof(float2);
+        kmax_a = HOE_MFMA_ADJ_KMAX;
+        while (kmax_a < kx && tiles * 8 * ((kx + kmax_a - 1) / kmax_a) * seg_bytes > ((int64_t)8 << 30)) kmax_a *= 2;
+    }
     if ((e = build_schedule(sf, (int)(h->ns_pad / 256), (int)(h->nv_pad / 8), slots_f, 0)) != hipSuccess) return e;
     if ((e = build_schedule(sa, (int)(h->nv_pad / 256), (int)(h->ns_pad / 8), slots_a, kmax_a)) != hipSuccess)
         return e;
