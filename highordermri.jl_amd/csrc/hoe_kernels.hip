@@ -489,22 +489,22 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (
                 generic_path();
             } else {
             // Horner adjoint. The wave's 64 voxels lie in at most two runs (r_lo, r_hi).
-            // For each pair of K steps one FP64 matrix product
-            // gives every coefficient the pair needs: rows = its 16 streamed samples,
-            // columns 0..7 = c_j(s, run) = sum_i A_s[i] beta[run][i][j] for (run, j),
-            // column 8 = t_s. The result goes through a wave-private LDS slot; each lane
-            // reads the (c0..c3, t) of its samples and runs and spends 4 FP64 FMAs per
-            // phase on its two voxel rows, already in the A-operand layout.
-            // The product runs over k blocks of 4 record terms. When L is a multiple of 4,
-            // covering t (record entry L at most) takes one more FP64 MFMA (64 cycles on
-            // gfx950) than the coefficients 0..L-1 alone, so t_s is then loaded with the
-            // records and written to the slot directly (measured at 16 terms: adjoint -2.3 %;
-            // at 10 terms, where it saves no MFMA, the extra load costs +3 %).
-            constexpr bool TCOL = (L % 4) != 0;
-            constexpr int NM = TCOL ? (L + 4) / 4 : (L + 3) / 4;
-            typedef double d4v __attribute__((ext_vector_type(4)));
-            __shared__ double hl[4][288];     // per wave, 2 slots: c [2 steps][2 runs][2 halves][4][4], t [2][2][4]
-            __shared__ double hb[4][NM][64];  // per wave: B operands of the setup product (constant per segment)
+            // For each pair of K steps, FP64 matrix products give every coefficient the
+            // pair needs: rows = its 16 streamed samples, columns = c_j(s, run) =
+            // sum_i A_s[i] beta[run][i][j] for j = 0..3, one product chain per run. The
+            // results and the samples' t_s go through a wave-private LDS slot; each lane
+            // reads the (c0..c3, t) of its samples and runs and spends 4 FP64 FMAs per phase
+            // on its two voxel rows, already in the A-operand layout.
+            // The products are v_mfma_f64_4x4x4f64 (4 blocks of 4 samples x 4 coefficients):
+            // 10-17 cycles each on gfx950 against 70-80 for v_mfma_f64_16x16x4f64, whose
+            // 16 columns this product would fill only half. They run over k blocks of 4
+            // record terms covering the coefficients 0..L-1; when L is not a multiple of 4
+            // that also covers t (record entry L at most), which is then taken from the
+            // loaded records, else t_s is loaded with them.
+            constexpr int NM = (L + 3) / 4;
+            constexpr bool TIN = (L % 4) != 0;
+            __shared__ double hl[4][288];       // per wave, 2 slots: c [2 steps][2 runs][2 halves][4][4], t [16]
+            __shared__ double hb[4][2][NM][16]; // per wave: B operands per run and k block (constant per segment)
             const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
             double *cl = hl[wv];
             const int base = d.x * 256 + wv * 64;
@@ -514,51 +514,53 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (
             const double2 h0 = hx.hrec[v0], h1 = hx.hrec[v1];
             const int rs0 = hx.vrun[v0] - r_lo, rs1 = hx.vrun[v1] - r_lo;
             const int half = lane >> 5;
-            const int col = lane & 15, kq = lane >> 4;
-            double *bop = hb[wv][0];
-#pragma unroll
-            for (int m = 0; m < NM; ++m) {
-                const int term = 4 * m + kq;
-                double b = 0.0;
-                if (col < 8) {
-                    const int rr = min(r_lo + (col >> 2), r_hi);
-                    if (term < L) b = hx.beta[((size_t)rr * L + term) * 4 + (col & 3)];
-                } else if (TCOL && col == 8) {
-                    b = term == hx.t_index ? 1.0 : 0.0;
-                }
-                bop[m * 64 + lane] = b;
+            const int kq = lane >> 4;
+            // B entry (k, j) of run g and k block m = beta[run g][4 m + k][j], the same for
+            // the 4 blocks of a product
+            for (int i = lane; i < 2 * NM * 16; i += 64) {
+                const int g = i / (NM * 16), m = (i / 16) % NM, e = i & 15;
+                const int term = 4 * m + (e >> 2);
+                const int rr = min(r_lo + g, r_hi);
+                hb[wv][g][m][e] = term < L ? hx.beta[((size_t)rr * L + term) * 4 + (e & 3)] : 0.0;
             }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             auto horner = [](const double *c, double x) { return fma(fma(fma(c[3], x, c[2]), x, c[1]), x, c[0]); };
-            // A operand of the pair starting at step kp: rows 0..7 = samples of kp, rows
-            // 8..15 = of kp + 1; zero where the step is outside the segment (the address
-            // is then a valid row, so the load is in bounds whatever the compiler does)
+            // A operand of the pair starting at step kp: lane l holds record entry
+            // l/16 + 4 m of row l%16; rows 0..7 = samples of kp, rows 8..15 = of kp + 1;
+            // zero where the step is outside the segment (the address is then a valid row,
+            // so the load is in bounds whatever the compiler does)
             auto load_a = [&](int kp, double *a, double &t) {
                 const int r = lane & 15;
                 const bool valid = kp < d.z && (r < 8 || kp + 1 < d.z);
                 const double *ap = coef + ((size_t)(valid ? kp * 8 + r : d.y * 8)) * sk;
 #pragma unroll
                 for (int m = 0; m < NM; ++m) a[m] = valid ? ap[kq + 4 * m] : 0.0;
-                if constexpr (!TCOL) t = valid ? ap[hx.t_index] : 0.0;
+                if constexpr (!TIN) t = valid ? ap[hx.t_index] : 0.0;
             };
-            // setup product of one pair into an LDS slot. v_mfma_f64_16x16x4: register r
-            // of lane l holds row l/16 + 4 r, column l%16; row kq + 4 r is sample kq of
-            // half r&1 of step r>>1 of the pair.
+            // setup products of one pair into an LDS slot. v_mfma_f64_4x4x4f64: A lane l =
+            // row l%16, k l/16; B lane l = k l/16, column l%16 (block (l%16)/4, coefficient
+            // l%4); D lane l = row R = 4 ((l%16)/4) + l/16, coefficient l%4. Row R is sample
+            // R%4 of half (R/4)%2 of step R/8 of the pair; t of row R goes to slot[128 + R].
+            const int hbe = (kq << 2) | (lane & 3);
+            const int sidx = ((lane >> 3) & 1) * 64 + ((lane >> 2) & 1) * 16 + 4 * kq + (lane & 3);
             auto setup = [&](const double *a, double t, double *slot) {
-                d4v acc = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-                for (int m = 0; m < NM; ++m)
-                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[m], bop[m * 64 + lane], acc, 0, 0, 0);
-                if (col < 8) {
-                    double *w = slot + (col >> 2) * 32 + kq * 4 + (col & 3);
+                for (int g = 0; g < 2; ++g) {
+                    if (g == 1 && !two) break;
+                    double acc = 0.0;
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) w[((r >> 1) * 4 + (r & 1)) * 16] = acc[r];
+                    for (int m = 0; m < NM; ++m)
+                        acc = __builtin_amdgcn_mfma_f64_4x4x4f64(a[m], hb[wv][g][m][hbe], acc, 0, 0, 0);
+                    slot[sidx + g * 32] = acc;
                 }
-                // t of row R (sample 8 kp + R) at slot[128 + R]: step R/8, half (R/4)%2, sample R%4
-                if constexpr (TCOL) {
-                    if (col == 8) {
+                if constexpr (TIN) {
+                    const int ti = hx.t_index;
+                    double tv = 0.0;
 #pragma unroll
-                        for (int r = 0; r < 4; ++r) slot[128 + kq + 4 * r] = acc[r];
-                    }
+                    for (int m = 0; m < NM; ++m) tv = (m == (ti >> 2)) ? a[m] : tv;
+                    if (kq == (ti & 3)) slot[128 + (lane & 15)] = tv;
                 } else if (kq == 0) {
                     slot[128 + (lane & 15)] = t;
                 }
