@@ -73,3 +73,43 @@ def test_horner_paths_vs_fp64(shape, mask_kind, expect_adj, backend, gpu_lib):
         e_n = O.relerr(hoe.normalOperator(op) * x, n_ref)
     print(f"{shape} mask={mask_kind} {backend} (horner {op.horner}): E {e_f:.2e}  E^H {e_a:.2e}  E^H E {e_n:.2e}")
     assert e_f < tol and e_a < tol and e_n < tol
+
+
+@pytest.mark.parametrize("case", ["terms9_plane", "concomitant_plane", "concomitant_offcentre"])
+@pytest.mark.parametrize("backend", ["direct", "mixed_fp16"])
+def test_horner_record_buckets_vs_fp64(case, backend, gpu_lib):
+    """The other record lengths of the Horner kernels, on a 2-D 192x8 grid (runs of 192
+    voxels, every wave in one run):
+    - 9 terms on the z = 0 plane: x, y, xy, 3z^2-r^2, x^2-y^2 active, + dB0 -> bucket 6
+      (the FP64 setup covers the record's t within its loaded blocks);
+    - 16 terms and concomitant fields on z = 0: 9 terms + x^2+y^2, + dB0 -> bucket 12
+      (t loaded with the records; one k block fewer than a t column would need);
+    - the same on an off-centre plane (z = 0.03): all 15 terms and 4 concomitant
+      columns, + dB0 -> bucket 20.
+    E x, E^H y and E^H E x against the FP64 operator (array_operator.jl:241-348 and
+    prep_concomitant, prep_kspha.jl:78-98, restated in the oracle) at 3e-5."""
+    grid, kspha, times, fm, csm, x, _ = _problem(192, 8, 1)
+    rng = np.random.default_rng(17)
+    conc = None
+    if case == "terms9_plane":
+        kspha = np.ascontiguousarray(kspha[:9])
+    else:
+        conc = (rng.normal(size=(4, kspha.shape[1])) * 20.0).astype(np.float32)
+        if case == "concomitant_offcentre":
+            grid.z[:] = np.float32(0.03)
+    m = O.make_model(grid, kspha, times, fm, csm, None, concomitant=conc)
+    y_ref = O.forward_f64(m, x)
+    probe = (np.random.default_rng(1).normal(size=y_ref.size)
+             + 1j * np.random.default_rng(2).normal(size=y_ref.size)).astype(np.complex64)
+    xa_ref = O.adjoint_f64(m, probe)
+    n_ref = O.normal_f64(m, x)
+    kw = {} if conc is None else {"concomitant": conc}
+    with hoe.HighOrderExplicitOp(grid, kspha, times, fieldmap=fm, csm=csm, backend=backend, **kw) as op:
+        assert op.horner & 1, "runs long enough for the Horner forward"
+        if backend != "direct":
+            assert op.horner & 2, "every wave in one run: Horner MFMA adjoint"
+        e_f = O.relerr(op * x, y_ref)
+        e_a = O.relerr(op.H * probe, xa_ref)
+        e_n = O.relerr(hoe.normalOperator(op) * x, n_ref)
+    print(f"{case} {backend} (horner {op.horner}): E {e_f:.2e}  E^H {e_a:.2e}  E^H E {e_n:.2e}")
+    assert e_f < 3e-5 and e_a < 3e-5 and e_n < 3e-5
