@@ -546,6 +546,13 @@ int hoe_create(const hoe_model_desc *d, hoe_handle **out) {
 #define HOE_HORNER_ADJ 1
 #endif
         h->horner_adj = HOE_HORNER_ADJ && h->horner;
+        // A workgroup of the MFMA adjoint lasts as long as its slowest wave; a tile holding a
+        // wave on the generic phase (three or more runs) costs about 1.3x a Horner tile
+        // (measured per workgroup on the 7 T spiral data), which its stream-K share reflects.
+        h->adj_tile_w.assign((size_t)(h->nv_pad / 256), 1.0);
+        if (h->horner_adj)
+            for (int64_t w = 0; w < h->nv_pad / 64; ++w)
+                if (vrun[(size_t)(64 * w + 63)] - vrun[(size_t)(64 * w)] > 1) h->adj_tile_w[(size_t)(w / 4)] = 1.3;
     }
 
     hipDeviceProp_t prop;

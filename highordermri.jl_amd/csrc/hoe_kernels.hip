@@ -1652,8 +1652,10 @@ static hipError_t upload_ints(DeviceBuf &b, const std::vector<int> &v) {
 // Cut tiles x n_ks units into 8 K-parts (one per XCD) and each part into
 // `slots/8` equal chunks of consecutive (tile, k) units; record the segments.
 // kmax > 0 cuts a chunk's run of one tile into near-equal segments of at most
-// kmax units.
-hipError_t build_schedule(Sched &sc, int tiles, int n_ks, int slots, int kmax) {
+// kmax units. tile_w (optional) weights the units of each tile, and the chunks of a
+// part then hold equal weighted work instead of equal unit counts.
+hipError_t build_schedule(Sched &sc, int tiles, int n_ks, int slots, int kmax,
+                          const std::vector<double> *tile_w) {
     const int X = 8;
     sc.tiles = tiles;
     sc.n_ks = n_ks;
@@ -1664,11 +1666,26 @@ hipError_t build_schedule(Sched &sc, int tiles, int n_ks, int slots, int kmax) {
         const int K0 = (int)((int64_t)x * n_ks / X), K1 = (int)((int64_t)(x + 1) * n_ks / X);
         const int Kx = K1 - K0;
         const int64_t units = (int64_t)tiles * Kx;
-        const int64_t U = (units + sc.cpx - 1) / sc.cpx;
+        // first unit of each chunk (ub[cpx] = units)
+        std::vector<int64_t> ub((size_t)sc.cpx + 1, units);
+        if (tile_w && (int)tile_w->size() == tiles) {
+            std::vector<double> cum((size_t)tiles + 1, 0.0);
+            for (int t = 0; t < tiles; ++t) cum[(size_t)t + 1] = cum[(size_t)t] + (*tile_w)[(size_t)t] * Kx;
+            int t = 0;
+            for (int c = 0; c < sc.cpx; ++c) {
+                const double target = cum[(size_t)tiles] * c / sc.cpx;
+                while (t + 1 < tiles && cum[(size_t)t + 1] <= target) ++t;
+                const int64_t k = std::min<int64_t>(Kx, (int64_t)std::ceil((target - cum[(size_t)t]) / (*tile_w)[(size_t)t]));
+                ub[(size_t)c] = std::max<int64_t>(c ? ub[(size_t)c - 1] : 0, (int64_t)t * Kx + std::max<int64_t>(0, k));
+            }
+        } else {
+            const int64_t U = (units + sc.cpx - 1) / sc.cpx;
+            for (int c = 0; c < sc.cpx; ++c) ub[(size_t)c] = std::min(units, (int64_t)c * U);
+        }
         for (int c = 0; c < sc.cpx; ++c) {
             chunk_off[x * sc.cpx + c] = (int)(seg.size() / 4);
-            int64_t u = (int64_t)c * U;
-            const int64_t u1 = std::min(units, (int64_t)(c + 1) * U);
+            int64_t u = ub[(size_t)c];
+            const int64_t u1 = ub[(size_t)c + 1];
             while (u < u1) {
                 const int t = (int)(u / Kx), kk = (int)(u % Kx);
                 const int len = (int)std::min<int64_t>(u1 - u, Kx - kk);
@@ -1736,7 +1753,10 @@ hipError_t prepare_backend(Handle *h, int backend) {
         while (kmax_a < kx && tiles * 8 * ((kx + kmax_a - 1) / kmax_a) * seg_bytes > ((int64_t)8 << 30)) kmax_a *= 2;
     }
     if ((e = build_schedule(sf, (int)(h->ns_pad / 256), (int)(h->nv_pad / 8), slots_f, 0)) != hipSuccess) return e;
-    if ((e = build_schedule(sa, (int)(h->nv_pad / 256), (int)(h->ns_pad / 8), slots_a, kmax_a)) != hipSuccess)
+    const bool weighted = backend != HOE_BACKEND_DIRECT &&
+                          std::any_of(h->adj_tile_w.begin(), h->adj_tile_w.end(), [](double w) { return w != 1.0; });
+    if ((e = build_schedule(sa, (int)(h->nv_pad / 256), (int)(h->ns_pad / 8), slots_a, kmax_a,
+                            weighted ? &h->adj_tile_w : nullptr)) != hipSuccess)
         return e;
     h->occ_f[backend] = occ_f;
     h->occ_a[backend] = occ_a;
