@@ -542,6 +542,9 @@ int hoe_create(const hoe_model_desc *d, hoe_handle **out) {
         h->horner = ok && (double)nv / (double)h->n_runs >= 16.0;
         // the MFMA adjoint evaluates a wave's 64 voxels from the Horner tables of at most
         // two runs; waves spanning more (short rows of a mask) take the generic phase
+#ifndef HOE_FWD_CROSS_W
+#define HOE_FWD_CROSS_W 2.0
+#endif
 #ifndef HOE_HORNER_ADJ
 #define HOE_HORNER_ADJ 1
 #endif
@@ -553,6 +556,13 @@ int hoe_create(const hoe_model_desc *d, hoe_handle **out) {
         if (h->horner_adj)
             for (int64_t w = 0; w < h->nv_pad / 64; ++w)
                 if (vrun[(size_t)(64 * w + 63)] - vrun[(size_t)(64 * w)] > 1) h->adj_tile_w[(size_t)(w / 4)] = 1.3;
+        // In the Horner forward a K unit whose 8 voxels cross a run boundary takes the generic
+        // phase (L FP64 FMAs per phase and the full records instead of 4 FMAs); its weight in
+        // the stream-K cut of the K range over the XCDs says so.
+        h->fwd_k_w.assign((size_t)(h->nv_pad / 8), 1.0);
+        if (h->horner)
+            for (int64_t k = 0; k < h->nv_pad / 8; ++k)
+                if (vrun[(size_t)(8 * k)] != vrun[(size_t)(8 * k + 7)]) h->fwd_k_w[(size_t)k] = HOE_FWD_CROSS_W;
     }
 
     hipDeviceProp_t prop;

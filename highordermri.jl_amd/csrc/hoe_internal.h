@@ -111,6 +111,7 @@ struct Handle {
     DeviceBuf growmap;    // int [L]: kernel row of each original term, -1 if its basis is zero
     std::vector<float> gbscale_h;
     std::vector<double> adj_tile_w;   // stream-K work weight of each 256-voxel tile in the MFMA adjoint
+    std::vector<double> fwd_k_w;      // stream-K work weight of each K unit (8 voxels) in the Horner forward
     int gb_type = -1;     // backend whose element type gbfrag holds
 
     // schedules per backend (index = hoe_backend) and for the gradient kernel
@@ -127,7 +128,7 @@ int direct_batches(const Handle *h);
 int mixed_nb(const Handle *h);         // 32-coil groups per wave in the MFMA kernels: 1 or 2
 int mixed_groups(const Handle *h);
 hipError_t build_schedule(Sched &sc, int tiles, int n_ks, int slots, int kmax,
-                          const std::vector<double> *tile_w = nullptr);
+                          const std::vector<double> *tile_w = nullptr, const std::vector<double> *k_w = nullptr);
 // Build the forward/adjoint schedules of a backend and size the partial buffers.
 hipError_t prepare_backend(Handle *h, int backend);
 hipError_t prepare_grad(Handle *h);

@@ -1652,18 +1652,33 @@ static hipError_t upload_ints(DeviceBuf &b, const std::vector<int> &v) {
 // Cut tiles x n_ks units into 8 K-parts (one per XCD) and each part into
 // `slots/8` equal chunks of consecutive (tile, k) units; record the segments.
 // kmax > 0 cuts a chunk's run of one tile into near-equal segments of at most
-// kmax units. tile_w (optional) weights the units of each tile, and the chunks of a
-// part then hold equal weighted work instead of equal unit counts.
+// kmax units. tile_w (optional) weights the units of each tile, k_w (optional) those
+// of each K index; the K parts and the chunks of a part then hold equal weighted work
+// instead of equal unit counts.
 hipError_t build_schedule(Sched &sc, int tiles, int n_ks, int slots, int kmax,
-                          const std::vector<double> *tile_w) {
+                          const std::vector<double> *tile_w, const std::vector<double> *k_w) {
     const int X = 8;
     sc.tiles = tiles;
     sc.n_ks = n_ks;
     sc.cpx = std::max(1, slots / X);
     sc.n_chunks = sc.cpx * X;
     std::vector<int> seg, chunk_off(sc.n_chunks + 1, 0);
+    const bool kw = k_w && (int)k_w->size() == n_ks;
+    std::vector<double> kcum;   // prefix sums of the K weights
+    std::vector<int> kb(X + 1);
+    if (kw) {
+        kcum.assign((size_t)n_ks + 1, 0.0);
+        for (int k = 0; k < n_ks; ++k) kcum[(size_t)k + 1] = kcum[(size_t)k] + (*k_w)[(size_t)k];
+        for (int x = 0; x <= X; ++x) {
+            const double target = kcum[(size_t)n_ks] * x / X;
+            const int k = (int)(std::lower_bound(kcum.begin(), kcum.end(), target) - kcum.begin());
+            kb[x] = x == 0 ? 0 : (x == X ? n_ks : std::max(kb[x - 1], std::min(k, n_ks)));
+        }
+    } else {
+        for (int x = 0; x <= X; ++x) kb[x] = (int)((int64_t)x * n_ks / X);
+    }
     for (int x = 0; x < X; ++x) {
-        const int K0 = (int)((int64_t)x * n_ks / X), K1 = (int)((int64_t)(x + 1) * n_ks / X);
+        const int K0 = kb[x], K1 = kb[x + 1];
         const int Kx = K1 - K0;
         const int64_t units = (int64_t)tiles * Kx;
         // first unit of each chunk (ub[cpx] = units)
@@ -1677,6 +1692,15 @@ hipError_t build_schedule(Sched &sc, int tiles, int n_ks, int slots, int kmax,
                 while (t + 1 < tiles && cum[(size_t)t + 1] <= target) ++t;
                 const int64_t k = std::min<int64_t>(Kx, (int64_t)std::ceil((target - cum[(size_t)t]) / (*tile_w)[(size_t)t]));
                 ub[(size_t)c] = std::max<int64_t>(c ? ub[(size_t)c - 1] : 0, (int64_t)t * Kx + std::max<int64_t>(0, k));
+            }
+        } else if (kw && Kx > 0) {
+            const double wt = kcum[(size_t)K1] - kcum[(size_t)K0];   // weighted work of one tile in the part
+            for (int c = 0; c < sc.cpx; ++c) {
+                const double target = wt * tiles * c / sc.cpx;
+                const int t = std::min(tiles - 1, (int)(target / wt));
+                const double rem = kcum[(size_t)K0] + (target - wt * t);
+                const int k = (int)(std::lower_bound(kcum.begin() + K0, kcum.begin() + K1 + 1, rem) - kcum.begin());
+                ub[(size_t)c] = std::max<int64_t>(c ? ub[(size_t)c - 1] : 0, (int64_t)t * Kx + (k - K0));
             }
         } else {
             const int64_t U = (units + sc.cpx - 1) / sc.cpx;
@@ -1752,7 +1776,12 @@ hipError_t prepare_backend(Handle *h, int backend) {
         kmax_a = HOE_MFMA_ADJ_KMAX;
         while (kmax_a < kx && tiles * 8 * ((kx + kmax_a - 1) / kmax_a) * seg_bytes > ((int64_t)8 << 30)) kmax_a *= 2;
     }
-    if ((e = build_schedule(sf, (int)(h->ns_pad / 256), (int)(h->nv_pad / 8), slots_f, 0)) != hipSuccess) return e;
+    // the Horner forward (MFMA and FP32) weighs its run-crossing K units
+    const bool fweighted = h->horner && !h->dt_on &&
+                           std::any_of(h->fwd_k_w.begin(), h->fwd_k_w.end(), [](double w) { return w != 1.0; });
+    if ((e = build_schedule(sf, (int)(h->ns_pad / 256), (int)(h->nv_pad / 8), slots_f, 0, nullptr,
+                            fweighted ? &h->fwd_k_w : nullptr)) != hipSuccess)
+        return e;
     const bool weighted = backend != HOE_BACKEND_DIRECT &&
                           std::any_of(h->adj_tile_w.begin(), h->adj_tile_w.end(), [](double w) { return w != 1.0; });
     if ((e = build_schedule(sa, (int)(h->nv_pad / 256), (int)(h->ns_pad / 8), slots_a, kmax_a,
