@@ -542,6 +542,9 @@ int hoe_create(const hoe_model_desc *d, hoe_handle **out) {
         h->horner = ok && (double)nv / (double)h->n_runs >= 16.0;
         // the MFMA adjoint evaluates a wave's 64 voxels from the Horner tables of at most
         // two runs; waves spanning more (short rows of a mask) take the generic phase
+#ifndef HOE_ADJ_GENERIC_W
+#define HOE_ADJ_GENERIC_W 1.6
+#endif
 #ifndef HOE_FWD_CROSS_W
 #define HOE_FWD_CROSS_W 2.0
 #endif
@@ -550,12 +553,14 @@ int hoe_create(const hoe_model_desc *d, hoe_handle **out) {
 #endif
         h->horner_adj = HOE_HORNER_ADJ && h->horner;
         // A workgroup of the MFMA adjoint lasts as long as its slowest wave; a tile holding a
-        // wave on the generic phase (three or more runs) costs about 1.3x a Horner tile
-        // (measured per workgroup on the 7 T spiral data), which its stream-K share reflects.
+        // wave on the generic phase (three or more runs) costs more than a Horner tile (its
+        // workgroups took 1.3x the median on the 7 T spiral data). Its stream-K units weigh
+        // 1.6, the best of 1.15-2.5 measured on those data.
         h->adj_tile_w.assign((size_t)(h->nv_pad / 256), 1.0);
         if (h->horner_adj)
             for (int64_t w = 0; w < h->nv_pad / 64; ++w)
-                if (vrun[(size_t)(64 * w + 63)] - vrun[(size_t)(64 * w)] > 1) h->adj_tile_w[(size_t)(w / 4)] = 1.3;
+                if (vrun[(size_t)(64 * w + 63)] - vrun[(size_t)(64 * w)] > 1)
+                    h->adj_tile_w[(size_t)(w / 4)] = HOE_ADJ_GENERIC_W;
         // In the Horner forward a K unit whose 8 voxels cross a run boundary takes the generic
         // phase (L FP64 FMAs per phase and the full records instead of 4 FMAs); its weight in
         // the stream-K cut of the K range over the XCDs says so.
