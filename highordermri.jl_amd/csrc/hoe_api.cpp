@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -780,6 +781,118 @@ int hoe_normal(hoe_handle *hh, const float *D, int64_t n_d, const void *x, void 
     HOE_TRY_HIP(h, hoe::run_normal_full(h, dx, tmp, (float)(1.0 / (double)h->n_vox), !(flags & HOE_ASYNC)));
     HOE_TRY_HIP(h, hoe::launch_scatter(h, tmp, dout, 1.0f));
     if ((rc = finish(h, flags, out, dout, sizeof(float2) * h->n_grid))) return rc;
+    if (!(flags & HOE_ASYNC)) record_times(h, true, true);
+    return HOE_OK;
+}
+
+}  // extern "C"
+
+// ---- the normal operator without allocation: workspaces from a static device array ----
+
+// Workspace of hoe_normal_noalloc on every GPU the library runs on: one file-scope array
+// per device, held until the process ends. Its name is a host-side stand-in, not a device
+// address: host code uses only the per-device address from hipGetSymbolAddress.
+static constexpr size_t kScratchBytes = size_t(512) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of this device's scratch array (the device must be current).
+int scratch_on(Handle *h, unsigned char **out) {
+    if (h->device < 0 || h->device >= kMaxDevices) return set_err(h, HOE_EINVAL, "device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[h->device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return set_err(h, HOE_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[h->device] = p;
+    }
+    *out = static_cast<unsigned char *>(g_scratch_ptr[h->device]);
+    return HOE_OK;
+}
+
+// The handle's workspaces that one normal-operator application writes, pointed into the
+// scratch array for the duration of a call and restored afterwards (the kernels take the
+// addresses when they are enqueued).
+struct ScratchSwap {
+    Handle *h;
+    hoe::DeviceBuf *bufs[9];
+    hoe::DeviceBuf saved[9];
+    int n = 0;
+    explicit ScratchSwap(Handle *hh) : h(hh) {}
+    void put(hoe::DeviceBuf &b, unsigned char *p, size_t bytes) {
+        bufs[n] = &b;
+        saved[n] = b;
+        b.p = p;
+        b.bytes = bytes;
+        ++n;
+    }
+    ~ScratchSwap() {
+        for (int i = 0; i < n; ++i) *bufs[i] = saved[i];
+    }
+};
+
+inline size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+// Bytes of every workspace of a normal-operator application on the handle's backend,
+// in the order they are carved out of the scratch array.
+void noalloc_plan(const Handle *h, size_t sizes[9]) {
+    const bool mixed = h->backend != HOE_BACKEND_DIRECT;
+    const size_t n_gb = (size_t)hoe::mixed_groups(h) * hoe::mixed_nb(h);
+    sizes[0] = 4096;                                                              // scal: fragment scales
+    sizes[1] = sizeof(float2) * (size_t)h->nv_pad;                                // xm
+    sizes[2] = sizeof(float2) * (size_t)h->nv_pad;                                // cg_vec: masked output
+    sizes[3] = sizeof(float2) * (size_t)h->ns_pad * h->ncs;                       // ybuf
+    sizes[4] = sizeof(float2) * 256 * (size_t)h->ncs * h->sched_f[h->backend].n_seg;   // part_f
+    sizes[5] = sizeof(float2) * 256 * (size_t)h->ncs * h->sched_a[h->backend].n_seg;   // part_a
+    sizes[6] = mixed ? n_gb * (size_t)(h->nv_pad / 8) * 64 * 64 : 0;             // zfrag
+    sizes[7] = mixed ? n_gb * (size_t)(h->ns_pad / 8) * 64 * 64 : 0;             // yfrag
+    sizes[8] = mixed ? 0 : sizeof(float2) * (size_t)h->nv_pad * h->ncs;          // zbuf
+}
+
+}  // namespace
+
+extern "C" {
+
+int hoe_normal_noalloc(hoe_handle *hh, const void *x, void *out, int flags) {
+    Handle *h = reinterpret_cast<Handle *>(hh);
+    int rc = check_open(h);
+    if (rc) return rc;
+    if ((rc = reject_dt(h))) return rc;
+    if (!x || !out) return set_err(h, HOE_EINVAL, "vector pointer is NULL");
+    if (!(flags & HOE_DEVICE_PTRS))
+        return set_err(h, HOE_EINVAL, "hoe_normal_noalloc takes device vectors (HOE_DEVICE_PTRS)");
+    size_t sizes[9], need = 0;
+    noalloc_plan(h, sizes);
+    for (size_t s : sizes) need += align256(s);
+    if (need > kScratchBytes)
+        return set_err(h, HOE_EINVAL,
+                       "hoe_normal_noalloc: this operator needs " + std::to_string(need >> 20) +
+                           " MiB of workspace, more than the " + std::to_string(kScratchBytes >> 20) +
+                           " MiB static array; use hoe_normal");
+    unsigned char *base = nullptr;
+    if ((rc = scratch_on(h, &base))) return rc;
+    {
+        ScratchSwap sw(h);
+        hoe::DeviceBuf *targets[9] = {&h->scal,   &h->xm,     &h->cg_vec, &h->ybuf, &h->part_f,
+                                      &h->part_a, &h->zfrag, &h->yfrag,  &h->zbuf};
+        size_t off = 0;
+        for (int i = 0; i < 9; ++i) {
+            if (sizes[i] == 0) continue;
+            sw.put(*targets[i], base + off, sizes[i]);
+            off += align256(sizes[i]);
+        }
+        if (!(flags & HOE_ASYNC)) (void)hipEventRecord(h->ev[4], h->stream);
+        float2 *tmp = reinterpret_cast<float2 *>(h->cg_vec.p);
+        HOE_TRY_HIP(h, hoe::run_normal_full(h, x, tmp, (float)(1.0 / (double)h->n_vox), !(flags & HOE_ASYNC)));
+        HOE_TRY_HIP(h, hoe::launch_scatter(h, tmp, out, 1.0f));
+    }
+    if ((rc = finish(h, flags, out, out, sizeof(float2) * h->n_grid))) return rc;
     if (!(flags & HOE_ASYNC)) record_times(h, true, true);
     return HOE_OK;
 }

@@ -30,6 +30,7 @@ EXPORTS = (
     "hoe_set_backend", "hoe_forward", "hoe_adjoint", "hoe_normal", "hoe_set_weights",
     "hoe_set_coefficients", "hoe_set_kspha", "hoe_set_kspha_dt", "hoe_grad", "hoe_cg", "hoe_close", "hoe_destroy",
     "hoe_sync", "hoe_last_kernel_ms", "hoe_apply_girf", "hoe_apply_girf_f64", "hoe_girf_fft_length",
+    "hoe_normal_noalloc",
 )
 
 
@@ -115,6 +116,7 @@ def load(path: str | os.PathLike | None = None):
         "hoe_adjoint": (C.c_int, [vp, vp, vp, C.c_int]),
         "hoe_normal": (C.c_int, [vp, vp, i64, vp, vp, C.c_int]),
         "hoe_set_weights": (C.c_int, [vp, vp, i64]),
+        "hoe_normal_noalloc": (C.c_int, [vp, vp, vp, C.c_int]),
         "hoe_set_coefficients": (C.c_int, [vp, vp, dbl]),
         "hoe_set_kspha": (C.c_int, [vp, vp, vp]),
         "hoe_apply_girf": (C.c_int, [vp, i64, i64, vp, i32, i32, dbl, i32, vp, C.c_int]),

@@ -153,6 +153,21 @@ int hoe_adjoint(hoe_handle *h, const void *y, void *x, int flags);
  * (explicit_gpu_operator.jl:272-297, 399-416). D is always a host pointer. */
 int hoe_normal(hoe_handle *h, const float *D, int64_t n_d, const void *x, void *out, int flags);
 
+/* out = E^H D E x / n_vox with the weights stored by the last hoe_set_weights /
+ * hoe_normal / hoe_cg call, computed like hoe_normal but without allocating: no
+ * hipMalloc, hipFree or other allocating call. Has no counterpart in the
+ * reference; it is for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid allocation). x and out are device
+ * pointers on the handle's GPU (flags must contain HOE_DEVICE_PTRS; HOE_ASYNC as
+ * for hoe_normal). The workspaces (partial sums, MFMA fragments, the weighted
+ * signal, the masked vectors, the fragment scales) come from one static 512 MiB
+ * device array per GPU, which each GPU the library runs on holds until the process
+ * ends. An operator that needs more returns HOE_EINVAL with the required size (the
+ * 200 x 200, 32-coil, 16 000-sample class fits on every backend). Every call on
+ * one GPU shares that GPU's array and leaves its contents behind: calls that overlap
+ * on one GPU (queued on different handles' streams) race. */
+int hoe_normal_noalloc(hoe_handle *h, const void *x, void *out, int flags);
+
 /* Store normal-operator weights (rules as for hoe_normal) for later calls
  * with D == NULL, n_d == HOE_CURRENT_WEIGHTS; D == NULL, n_d == 0 stores unit
  * weights. The stored weights play the role of the diagonal captured by
