@@ -279,11 +279,11 @@ def main():
             gts = []
             for i in range(13):          # 3 warm-up calls, then 10 timed
                 t0 = time.perf_counter()
-                g, gtau, loss = op.grad(x_dev, y_obs)
+                g, gtau, loss = op.grad(x_dev, y_obs)      # device vectors: dL/dh stays on the GPU
                 torch.cuda.synchronize()
                 if i >= 3:
                     gts.append(time.perf_counter() - t0)
-            gres[be] = g
+            gres[be] = g.cpu().numpy()
             grad[be] = {"ms": round(1e3 * float(np.median(gts)), 3), "kernel_ms": round(op.last_kernel_ms(0), 3),
                         "loss": loss, "dL_dtau": gtau}
             log(f"grad {be}: {grad[be]['ms']:.2f} ms (kernel {grad[be]['kernel_ms']:.2f} ms), loss {loss:.6e}, "

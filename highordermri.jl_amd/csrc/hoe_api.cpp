@@ -137,68 +137,6 @@ void catmull_rom(double f, double w[4], double dw[4]) {
     dw[3] = 0.5 * (3.0 * f2 - 2.0 * f);
 }
 
-// Effective coefficient records per sample (FP64) from the traces, delayed by
-// tau (shift u = tau/dwell samples): [k_1, k_{a_1}, ..., k_{a_m}, t, 0...]
-// where a_1..a_m are the active (not identically zero) basis terms.
-double trace_at(const Handle *h, int64_t s, int l, bool shift, int64_t m, const double w[4]) {
-    const int64_t ns = h->n_sam;
-    const int L = h->L;
-    if (!shift) return (double)h->traces[(size_t)s * L + l];
-    double acc = 0.0;
-    for (int t = 0; t < 4; ++t) {
-        int64_t j = s + m - 1 + t;
-        j = std::min<int64_t>(std::max<int64_t>(j, 0), ns - 1);
-        acc += w[t] * (double)h->traces[(size_t)j * L + l];
-    }
-    return acc;
-}
-
-void build_coef_records(const Handle *h, std::vector<double> &rec) {
-    const int64_t ns = h->n_sam;
-    rec.assign((size_t)h->ns_pad * h->sk, 0.0);
-    const bool shift = h->tau != 0.0;
-    int64_t m = 0;
-    double w[4] = {0, 1, 0, 0}, dw[4];
-    if (shift) {
-        const double u = h->tau / h->dwell;
-        m = (int64_t)std::floor(u);
-        catmull_rom(u - (double)m, w, dw);
-    }
-    const int na = (int)h->active.size();
-    for (int64_t s = 0; s < ns; ++s) {
-        double *r = &rec[(size_t)s * h->sk];
-        r[0] = trace_at(h, s, 0, shift, m, w);
-        for (int i = 0; i < na; ++i) r[1 + i] = trace_at(h, s, h->active[i], shift, m, w);
-        r[1 + na] = (double)h->times[s];
-    }
-}
-
-// d k / d tau of the kernel rows (term 1, active terms) per sample, [ns_pad][sk]:
-// the derivative of the Catmull-Rom resampling at s + tau/dwell (at tau = 0 the
-// interpolant through the samples, u = 0). All zero when dwell is unset.
-void build_dcoef_records(const Handle *h, std::vector<double> &rec) {
-    const int64_t ns = h->n_sam;
-    const int L = h->L;
-    rec.assign((size_t)h->ns_pad * h->sk, 0.0);
-    if (!(h->dwell > 0.0)) return;
-    const double u = h->tau / h->dwell;
-    const int64_t m = (int64_t)std::floor(u);
-    double w[4], dw[4];
-    catmull_rom(u - (double)m, w, dw);
-    const int na = (int)h->active.size();
-    for (int64_t s = 0; s < ns; ++s)
-        for (int r = 0; r <= na; ++r) {
-            const int l = r == 0 ? 0 : h->active[r - 1];
-            double acc = 0.0;
-            for (int t = 0; t < 4; ++t) {
-                int64_t j = s + m - 1 + t;
-                j = std::min<int64_t>(std::max<int64_t>(j, 0), ns - 1);
-                acc += dw[t] * (double)h->traces[(size_t)j * L + l];
-            }
-            rec[(size_t)s * h->sk + r] = acc / h->dwell;
-        }
-}
-
 // prep_kspha (prep_kspha.jl:1-56) into the first n_term rows of the traces: a cleared
 // order-0/2/3 flag zeroes its rows, a cleared order-1 flag replaces rows 2-4 with
 // k_nominal (3 x n_sam, term fastest; NULL keeps kspha's own rows, the reference's
@@ -219,12 +157,32 @@ void prepare_traces(Handle *h, const float *kspha, const float *k_nominal) {
     }
 }
 
+// The sample records coef ([k_1, k of the active terms, t, 0...] per sample, FP64) and
+// their tau-derivatives dcoef, built on the device from traces_d: the traces resampled at
+// s + tau/dwell with uniform Catmull-Rom weights (clamped ends; tau = 0 keeps them), and
+// d/dtau of that resampling (all zero when dwell is unset). Queued on the handle's stream
+// behind any work that still reads the previous records.
+int rebuild_records(Handle *h) {
+    const bool shift = h->tau != 0.0;
+    int64_t m = 0;
+    double w[4] = {0.0, 1.0, 0.0, 0.0}, dw[4] = {0.0, 0.0, 0.0, 0.0};
+    if (h->dwell > 0.0) {
+        const double u = h->tau / h->dwell;
+        m = (int64_t)std::floor(u);
+        catmull_rom(u - (double)m, w, dw);
+    }
+    if (m < INT32_MIN / 2 || m > INT32_MAX / 2) return set_err(h, HOE_EINVAL, "delay too large for the readout");
+    HOE_TRY_HIP(h, hoe::launch_build_records(h, shift ? 1 : 0, (int)m, w, dw));
+    return HOE_OK;
+}
+
+// Host traces -> traces_d, then the records; waits for the device.
 int upload_coef(Handle *h) {
-    std::vector<double> rec, drec;
-    build_coef_records(h, rec);
-    build_dcoef_records(h, drec);
-    HOE_TRY_HIP(h, hipMemcpy(h->coef.p, rec.data(), rec.size() * sizeof(double), hipMemcpyHostToDevice));
-    HOE_TRY_HIP(h, hipMemcpy(h->dcoef.p, drec.data(), drec.size() * sizeof(double), hipMemcpyHostToDevice));
+    HOE_TRY_HIP(h, hipMemcpyAsync(h->traces_d.p, h->traces.data(), sizeof(float) * h->traces.size(),
+                                  hipMemcpyHostToDevice, h->stream));
+    h->traces_stale = false;
+    int rc = rebuild_records(h);
+    if (rc) return rc;
     HOE_TRY_HIP(h, hipStreamSynchronize(h->stream));
     return HOE_OK;
 }
@@ -235,6 +193,20 @@ int check_open(Handle *h) {
     hipError_t e = hipSetDevice(h->device);
     if (e != hipSuccess) return set_err(h, HOE_EHIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
     return HOE_OK;
+}
+
+// Rebuild the FP32 backend's schedules after dt_on changed (the forward kernel instance,
+// hence its occupancy and the stream-K slot count, depends on it); on failure dt_on goes
+// back to prev_dt and the schedules are rebuilt for it.
+int reschedule_direct(Handle *h, bool prev_dt) {
+    h->sched_f[HOE_BACKEND_DIRECT].built = false;
+    hipError_t e = hoe::prepare_backend(h, HOE_BACKEND_DIRECT);
+    if (e == hipSuccess) return HOE_OK;
+    const std::string msg = std::string("schedule setup failed: ") + hipGetErrorString(e);
+    h->dt_on = prev_dt;
+    h->sched_f[HOE_BACKEND_DIRECT].built = false;
+    (void)hoe::prepare_backend(h, HOE_BACKEND_DIRECT);
+    return set_err(h, e == hipErrorOutOfMemory ? HOE_ENOMEM : HOE_EHIP, msg);
 }
 
 // The derivative model (kspha_dt) defines only the forward product.
@@ -288,7 +260,8 @@ void release(Handle *h) {
                               &h->hrec,     &h->vrun,   &h->beta,   &h->zfrag,   &h->ybuf,   &h->yfrag,  &h->part_f,
                               &h->part_a,   &h->part_g, &h->dvec,   &h->io_x,    &h->io_x2,  &h->io_y,   &h->io_y2,
                               &h->scal,     &h->cg_vec, &h->red,    &h->gk,      &h->cg_log, &h->gzfrag, &h->grfrag,
-                              &h->gbfrag,   &h->gbscale, &h->part_gm, &h->ghbuf, &h->growmap, &h->dtrec};
+                              &h->gbfrag,   &h->gbscale, &h->part_gm, &h->ghbuf, &h->growmap, &h->dtrec,
+                              &h->traces_d, &h->times_d,  &h->act_d};
     for (auto *b : bufs) dfree(*b);
     hoe::Sched *scheds[] = {&h->sched_f[0], &h->sched_f[1], &h->sched_f[2], &h->sched_a[0],
                             &h->sched_a[1], &h->sched_a[2], &h->sched_g,    &h->sched_gm};
@@ -603,6 +576,9 @@ int hoe_create(const hoe_model_desc *d, hoe_handle **out) {
         {&h->dtrec, sizeof(double) * (size_t)h->ns_pad * h->sk},
         {&h->ghbuf, sizeof(float) * (size_t)ns * L},
         {&h->growmap, sizeof(int32_t) * (size_t)L},
+        {&h->traces_d, sizeof(float) * (size_t)L * ns},
+        {&h->times_d, sizeof(float) * (size_t)ns},
+        {&h->act_d, sizeof(int32_t) * std::max<size_t>(h->active.size(), 1)},
         {&h->csm, sizeof(float2) * csmv.size()},
         {&h->hrec, sizeof(double) * hrec.size()},
         {&h->vrun, sizeof(int32_t) * vrun.size()},
@@ -635,6 +611,10 @@ int hoe_create(const hoe_model_desc *d, hoe_handle **out) {
     if (e == hipSuccess) e = hipMemcpy(h->vrun.p, vrun.data(), sizeof(int32_t) * vrun.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess && !beta.empty())
         e = hipMemcpy(h->beta.p, beta.data(), sizeof(double) * beta.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(h->times_d.p, h->times.data(), sizeof(float) * (size_t)ns, hipMemcpyHostToDevice);
+    if (e == hipSuccess && !h->active.empty())
+        e = hipMemcpy(h->act_d.p, h->active.data(), sizeof(int32_t) * h->active.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) {
         // kernel row of each original term for the gradient (-1: identically zero basis)
         std::vector<int32_t> rowmap((size_t)L, -1);
@@ -897,16 +877,32 @@ int hoe_normal_noalloc(hoe_handle *hh, const void *x, void *out, int flags) {
     return HOE_OK;
 }
 
-int hoe_set_coefficients(hoe_handle *hh, const float *kspha, double tau) {
+int hoe_set_coefficients(hoe_handle *hh, const float *kspha, double tau, int flags) {
     Handle *h = reinterpret_cast<Handle *>(hh);
     int rc = check_open(h);
     if (rc) return rc;
     if (!std::isfinite(tau)) return set_err(h, HOE_EINVAL, "tau must be finite");
     if (tau != 0.0 && !(h->dwell > 0.0)) return set_err(h, HOE_EINVAL, "a delay requires dwell > 0");
-    if (kspha) std::memcpy(h->traces.data(), kspha, sizeof(float) * h->traces.size());
+    if (kspha && (flags & HOE_DEVICE_PTRS)) {
+        // device traces go straight to traces_d, behind the work queued on the stream
+        HOE_TRY_HIP(h, hipMemcpyAsync(h->traces_d.p, kspha, sizeof(float) * h->traces.size(),
+                                      hipMemcpyDeviceToDevice, h->stream));
+        h->traces_stale = true;
+    } else if (kspha) {
+        HOE_TRY_HIP(h, hipStreamSynchronize(h->stream));
+        std::memcpy(h->traces.data(), kspha, sizeof(float) * h->traces.size());
+        HOE_TRY_HIP(h, hipMemcpyAsync(h->traces_d.p, h->traces.data(), sizeof(float) * h->traces.size(),
+                                      hipMemcpyHostToDevice, h->stream));
+        h->traces_stale = false;
+    }
+    const double tau_prev = h->tau;
     h->tau = tau;
-    HOE_TRY_HIP(h, hipStreamSynchronize(h->stream));
-    return upload_coef(h);
+    if ((rc = rebuild_records(h))) {
+        h->tau = tau_prev;
+        return rc;
+    }
+    if (!(flags & HOE_ASYNC) || !(flags & HOE_DEVICE_PTRS)) HOE_TRY_HIP(h, hipStreamSynchronize(h->stream));
+    return HOE_OK;
 }
 
 int hoe_set_kspha(hoe_handle *hh, const float *kspha, const float *k_nominal) {
@@ -914,6 +910,13 @@ int hoe_set_kspha(hoe_handle *hh, const float *kspha, const float *k_nominal) {
     int rc = check_open(h);
     if (rc) return rc;
     if (!kspha) return set_err(h, HOE_EINVAL, "kspha is required");
+    HOE_TRY_HIP(h, hipStreamSynchronize(h->stream));
+    if (h->traces_stale) {
+        // the concomitant rows are kept from the traces last set from device memory
+        HOE_TRY_HIP(h, hipMemcpy(h->traces.data(), h->traces_d.p, sizeof(float) * h->traces.size(),
+                                 hipMemcpyDeviceToHost));
+        h->traces_stale = false;
+    }
     prepare_traces(h, kspha, k_nominal);
     h->tau = 0.0;
     HOE_TRY_HIP(h, hipStreamSynchronize(h->stream));
@@ -925,8 +928,10 @@ int hoe_set_kspha_dt(hoe_handle *hh, const float *kspha_dt) {
     int rc = check_open(h);
     if (rc) return rc;
     if (!kspha_dt) {
+        if (!h->dt_on) return HOE_OK;
+        HOE_TRY_HIP(h, hipStreamSynchronize(h->stream));
         h->dt_on = false;
-        return HOE_OK;
+        return reschedule_direct(h, true);
     }
     if (h->has_conc)
         return set_err(h, HOE_EINVAL,
@@ -946,9 +951,11 @@ int hoe_set_kspha_dt(hoe_handle *hh, const float *kspha_dt) {
     }
     HOE_TRY_HIP(h, hipStreamSynchronize(h->stream));
     HOE_TRY_HIP(h, hipMemcpy(h->dtrec.p, rec.data(), rec.size() * sizeof(double), hipMemcpyHostToDevice));
-    HOE_TRY_HIP(h, hoe::prepare_backend(h, HOE_BACKEND_DIRECT));
+    if (h->dt_on) return HOE_OK;
+    // the FP32 forward's schedule is sized by the occupancy of the kernel instance that will
+    // run it: the derivative one from now on
     h->dt_on = true;
-    return HOE_OK;
+    return reschedule_direct(h, false);
 }
 
 int hoe_grad(hoe_handle *hh, const void *x, const void *y, float *g_h, double *g_tau, double *loss, int flags) {
@@ -976,10 +983,11 @@ int hoe_grad(hoe_handle *hh, const void *x, const void *y, float *g_h, double *g
         m = (int64_t)std::floor(u);
         catmull_rom(u - (double)m, w, dw);
     }
-    HOE_TRY_HIP(h, hoe::launch_grad_chain(h, gtau_dev, shift, (int)m, w));
+    const bool dev_out = (flags & HOE_DEVICE_PTRS) && g_h;
+    HOE_TRY_HIP(h, hoe::launch_grad_chain(h, gtau_dev, shift, (int)m, w, dev_out ? g_h : nullptr));
     (void)hipEventRecord(h->ev[5], h->stream);
     double host_sc[2] = {0.0, 0.0};
-    if (g_h)
+    if (g_h && !dev_out)
         HOE_TRY_HIP(h, hipMemcpyAsync(g_h, h->ghbuf.p, sizeof(float) * (size_t)h->n_sam * h->L,
                                       hipMemcpyDeviceToHost, h->stream));
     HOE_TRY_HIP(h, hipMemcpyAsync(host_sc, loss_dev, 2 * sizeof(double), hipMemcpyDeviceToHost, h->stream));

@@ -66,6 +66,7 @@ struct Handle {
 
     // ---- host copies needed to rebuild coefficients ----
     std::vector<float> traces;        // L x n_sam, term fastest (prepared coefficients)
+    bool traces_stale = false;        // traces_d was written from device memory; the host copy is older
     std::vector<float> times;         // n_sam
     std::vector<int32_t> mask_idx_h;  // n_vox
 
@@ -109,6 +110,9 @@ struct Handle {
     DeviceBuf part_gm;    // float [n_seg][256][32] gradient partials
     DeviceBuf ghbuf;      // float [n_sam][L] dL/dh (term fastest)
     DeviceBuf growmap;    // int [L]: kernel row of each original term, -1 if its basis is zero
+    DeviceBuf traces_d;   // float [n_sam][L]: the coefficient traces (term fastest), source of coef/dcoef
+    DeviceBuf times_d;    // float [n_sam]
+    DeviceBuf act_d;      // int [active terms]: 0-based term index of each kernel record entry 1..
     std::vector<float> gbscale_h;
     std::vector<double> adj_tile_w;   // stream-K work weight of each 256-voxel tile in the MFMA adjoint
     std::vector<double> fwd_k_w;      // stream-K work weight of each K unit (8 voxels) in the Horner forward
@@ -169,7 +173,11 @@ hipError_t cg_run(Handle *h, const void *rhs_m, int iters, double lambda, double
 hipError_t launch_norm2(Handle *h, const void *v, int64_t n, double *out_dev);
 // dL/dh into ghbuf and dL/dtau into *gtau_dev from gk (after run_grad_contract);
 // shift, m, w: the delay resampling k(s) = sum_t w[t] h(clamp(s + m - 1 + t)).
-hipError_t launch_grad_chain(Handle *h, double *gtau_dev, bool shift, int m, const double w[4]);
+// gh_out: device array for dL/dh ([n_sam][L]); nullptr writes the handle's ghbuf.
+hipError_t launch_grad_chain(Handle *h, double *gtau_dev, bool shift, int m, const double w[4], float *gh_out);
+// coef and dcoef from traces_d (k_build_records): resampling shift m and Catmull-Rom weights
+// w, dw (shift == 0: the traces unchanged, dcoef from dw at offset 0).
+hipError_t launch_build_records(Handle *h, int shift, int m, const double w[4], const double dw[4]);
 
 // GIRF prediction (hoe_girf.hip): FFT length nextprod([2,3,5], n), and the whole
 // prediction of n_other 3-channel waveforms; on failure err holds the message.

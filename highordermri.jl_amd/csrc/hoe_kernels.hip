@@ -1314,6 +1314,43 @@ __global__ void k_reduce_grad(const double *__restrict__ part, const int *__rest
     gk[(size_t)l * ns + s] = acc * scale;
 }
 
+// ---- coefficient records from the traces ----
+
+// coef[s][r] and dcoef[s][r] (r < sk) for every padded sample s from the traces [ns][L]
+// (term fastest) resampled at s + tau/dwell with uniform Catmull-Rom weights w over taps
+// clamp(s + m - 1 + t), t = 0..3 (shift == 0: the traces unchanged):
+//   r = 0: term 1; r = 1..na: active term act[r-1]; r = na + 1: t_s (coef) or 0 (dcoef);
+//   beyond, and for padded samples: 0.
+// dcoef holds d k / d tau = sum_t dw[t] h(...) / dwell (all zero when dwell is unset).
+__global__ void k_build_records(const float *__restrict__ traces, int ns, int ns_pad, int L,
+                                const int *__restrict__ act, int na, const float *__restrict__ times, int sk,
+                                int shift, int m, double4 w, double4 dw, double inv_dwell, double *__restrict__ coef,
+                                double *__restrict__ dcoef) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)ns_pad * sk) return;
+    const int s = (int)(i / sk), r = (int)(i % sk);
+    double c = 0.0, dc = 0.0;
+    if (s < ns && r <= na + 1) {
+        if (r == na + 1) {
+            c = (double)times[s];
+        } else {
+            const int l = r == 0 ? 0 : act[r - 1];
+            const double wt[4] = {w.x, w.y, w.z, w.w}, dwt[4] = {dw.x, dw.y, dw.z, dw.w};
+            double a = 0.0, da = 0.0;
+            for (int t = 0; t < 4; ++t) {
+                const int j = min(max(s + m - 1 + t, 0), ns - 1);
+                const double v = (double)traces[(size_t)j * L + l];
+                a = fma(wt[t], v, a);
+                da = fma(dwt[t], v, da);
+            }
+            c = shift ? a : (double)traces[(size_t)s * L + l];
+            dc = da * inv_dwell;
+        }
+    }
+    coef[i] = c;
+    dcoef[i] = dc;
+}
+
 // ---- gradient chain rule through the delay interpolation ----
 
 // gh[j][l] = sum_t w[t] sum_{s : clamp(s+m-1+t) = j} gk[row(l)][s]   (shift != 0)
@@ -2073,11 +2110,22 @@ hipError_t launch_build_zbuf(Handle *h, const void *xm) {
     return hipGetLastError();
 }
 
-hipError_t launch_grad_chain(Handle *h, double *gtau_dev, bool shift, int m, const double w[4]) {
+hipError_t launch_build_records(Handle *h, int shift, int m, const double w[4],
+                                const double dw[4]) {
+    const int64_t n = h->ns_pad * h->sk;
+    const double inv = h->dwell > 0.0 ? 1.0 / h->dwell : 0.0;
+    k_build_records<<<blocks_for(n), 256, 0, h->stream>>>(
+        (const float *)h->traces_d.p, (int)h->n_sam, (int)h->ns_pad, h->L, (const int *)h->act_d.p, (int)h->active.size(),
+        (const float *)h->times_d.p, h->sk, shift, m, make_double4(w[0], w[1], w[2], w[3]),
+        make_double4(dw[0], dw[1], dw[2], dw[3]), inv, (double *)h->coef.p, (double *)h->dcoef.p);
+    return hipGetLastError();
+}
+
+hipError_t launch_grad_chain(Handle *h, double *gtau_dev, bool shift, int m, const double w[4], float *gh_out) {
     const int ns = (int)h->n_sam;
     k_grad_chain<<<blocks_for((int64_t)ns * h->L), 256, 0, h->stream>>>(
         (const double *)h->gk.p, ns, (const int *)h->growmap.p, h->L, shift ? 1 : 0, m, w[0], w[1], w[2], w[3],
-        (float *)h->ghbuf.p);
+        gh_out ? gh_out : (float *)h->ghbuf.p);
     const int nb = 512;
     k_gtau_partial<<<nb, 256, 0, h->stream>>>((const double *)h->gk.p, (const double *)h->dcoef.p, ns,
                                               (int)h->active.size() + 1, h->sk, (double *)h->red.p);

@@ -117,7 +117,7 @@ def load(path: str | os.PathLike | None = None):
         "hoe_normal": (C.c_int, [vp, vp, i64, vp, vp, C.c_int]),
         "hoe_set_weights": (C.c_int, [vp, vp, i64]),
         "hoe_normal_noalloc": (C.c_int, [vp, vp, vp, C.c_int]),
-        "hoe_set_coefficients": (C.c_int, [vp, vp, dbl]),
+        "hoe_set_coefficients": (C.c_int, [vp, vp, dbl, C.c_int]),
         "hoe_set_kspha": (C.c_int, [vp, vp, vp]),
         "hoe_apply_girf": (C.c_int, [vp, i64, i64, vp, i32, i32, dbl, i32, vp, C.c_int]),
         "hoe_apply_girf_f64": (C.c_int, [vp, i64, i64, vp, i32, i32, dbl, i32, vp, C.c_int]),

@@ -346,19 +346,33 @@ class HighOrderExplicitOp:
         return AdjointOp(self)
 
     def set_coefficients(self, kspha=None, tau: float = 0.0):
-        """Replace the coefficient traces ``(nTerm_eff, nSam)`` and the delay tau (s)."""
+        """Replace the coefficient traces ``(nTerm_eff, nSam)`` and the delay tau (s).
+        ``kspha`` may be a host array or a float32 tensor on the operator's GPU (no host
+        round trip: the records are rebuilt on the device)."""
         h = self._handle()
-        ptr = None
-        if kspha is not None:
-            k = _f32(kspha, "kspha")
+        ptr, flags, keep = None, 0, None
+        if kspha is not None and _is_torch(kspha) and kspha.is_cuda:
+            if kspha.dtype != torch.float32:
+                raise ArgumentError("kspha must be float32")
+            if tuple(kspha.shape) != (self.nTerm, self.nSam):
+                raise DimensionMismatch(f"kspha must be ({self.nTerm}, {self.nSam})")
+            if kspha.device.index != self.device:
+                raise ArgumentError(f"device arrays must be on the operator's GPU (device {self.device})")
+            keep = kspha.t().contiguous()           # term fastest
+            torch.cuda.current_stream(keep.device).synchronize()
+            ptr, flags = keep.data_ptr(), _lib.HOE_DEVICE_PTRS
+        elif kspha is not None:
+            k = _f32(kspha.detach().cpu().numpy() if _is_torch(kspha) else kspha, "kspha")
             if k.shape != (self.nTerm, self.nSam):
                 raise DimensionMismatch(f"kspha must be ({self.nTerm}, {self.nSam})")
-            kt = np.ascontiguousarray(k.T)
-            ptr = kt.ctypes.data
-        check(self._lib.hoe_set_coefficients(h, ptr, float(tau)), h)
+            keep = np.ascontiguousarray(k.T)
+            ptr = keep.ctypes.data
+        check(self._lib.hoe_set_coefficients(h, ptr, float(tau), flags), h)
 
     def grad(self, x, y):
-        """Gradient of ``||E(h,tau) x - y||^2``: returns (g_h (nTerm, nSam), g_tau, loss)."""
+        """Gradient of ``||E(h,tau) x - y||^2``: returns (g_h (nTerm, nSam), g_tau, loss).
+        With device vectors g_h is a float32 tensor on the operator's GPU (a transposed
+        view of the term-fastest buffer libhoe writes), else a numpy array."""
         h = self._handle()
         xv = _Vec(x, self.ncol, "x")
         yv = _Vec(y, self.nrow, "y")
@@ -366,10 +380,13 @@ class HighOrderExplicitOp:
         if xv.dev != yv.dev:
             raise ArgumentError("x and y must both be host or both be device arrays")
         flags = _lib.HOE_DEVICE_PTRS if xv.dev else 0
+        gt, loss = C.c_double(), C.c_double()
         if xv.dev:
             torch.cuda.current_stream(xv.obj.device).synchronize()
+            g = torch.empty((self.nSam, self.nTerm), dtype=torch.float32, device=xv.obj.device)
+            check(self._lib.hoe_grad(h, xv.ptr, yv.ptr, g.data_ptr(), C.byref(gt), C.byref(loss), flags), h)
+            return g.t(), gt.value, loss.value
         g = np.empty((self.nSam, self.nTerm), dtype=np.float32)
-        gt, loss = C.c_double(), C.c_double()
         check(self._lib.hoe_grad(h, xv.ptr, yv.ptr, g.ctypes.data, C.byref(gt), C.byref(loss), flags), h)
         return np.ascontiguousarray(g.T), gt.value, loss.value
 

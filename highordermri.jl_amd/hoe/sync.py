@@ -18,10 +18,13 @@ four neighbouring interval slopes, two extra slopes at each end by linear
 extrapolation of the interval slopes, and the mean of the two middle slopes
 where both weights vanish; cubic Hermite pieces between nodes.
 
-The reference computes its density weights with ``samplingDensity`` (NFFT
-``sdc``), which is not provided here: pass ``weight`` (an array of one weight
-per sample, or a callable of the resampled ``(nTerm, nSample)`` coefficients);
-the default is unit weights.
+Density weights: the reference recomputes ``weight = samplingDensity(kspha_τ[2:3,:],
+(nX, nY))`` from the resampled first-order rows at every update (FindDelay.jl:183,
+FindDelay_multishot.jl:120), and so does the default here, with
+``hoe.samplingDensity``. That function runs the published Pipe-Menon iteration on its
+own Kaiser-Bessel grid; its parity with NFFT.jl's ``sdc`` (which the reference calls,
+not vendored) is unpinned. ``weight`` overrides it: an array of one weight per sample,
+or a callable of the resampled ``(nTerm, nSample)`` coefficients.
 """
 from __future__ import annotations
 
@@ -165,13 +168,17 @@ def kspha_derivative(kspha, dt):
     return (out / float(T(dt))).astype(T)
 
 
-def _unit_weights(n):
-    return np.ones(n, dtype=np.float32)
+def density_weight(gridding):
+    """The reference's per-update weights: ``samplingDensity`` of the resampled first-order
+    rows 2:3 (FindDelay.jl:183, FindDelay_multishot.jl:120)."""
+    from .fieldcam import samplingDensity
+
+    def weight(k_tau):
+        return samplingDensity(np.asarray(k_tau)[1:3], (gridding.nX, gridding.nY))
+    return weight
 
 
 def _weights(weight, k_tau, n):
-    if weight is None:
-        return _unit_weights(n)
     w = weight(k_tau) if callable(weight) else weight
     w = np.asarray(w).reshape(-1)
     if w.size != n:
@@ -236,7 +243,8 @@ def FindDelay(gridding, data, kspha, *args, JumpFact=3, Δτ_min=0.005, intermod
     ``Δτ = JumpFact * real((B x) \\ (y - E x))``; ``JumpFact`` halves (not below 1)
     when the update changes sign; the loop stops when ``|Δτ| <= Δτ_min`` and returns
     ``τ - Δτ/J*(J-1)``. ``nBlock`` and ``arrayType`` are accepted and unused.
-    Beyond the reference: ``weight`` (see the module docstring), ``backend``,
+    Beyond the reference: ``weight`` (default: the reference's density weights, see the
+    module docstring), ``backend``,
     ``max_updates`` (None: no limit, as in the reference) and ``return_history``
     (also return the list of τ after each update).
     """
@@ -265,6 +273,7 @@ def FindDelay(gridding, data, kspha, *args, JumpFact=3, Δτ_min=0.005, intermod
     if fieldmap is None:
         fieldmap = np.zeros((gridding.nX, gridding.nY), dtype=np.float32)
     reg = L2Regularization(0.0) if reg is None else reg
+    weight = density_weight(gridding) if weight is None else weight
     kspha = np.asarray(kspha)
     kspha_dt = kspha_derivative(kspha, dt)
     rec = _rec_params(gridding, reg, iter_max, solver)
@@ -339,6 +348,7 @@ def FindDelay_multishot(gridding, data, kspha, *args, JumpFact=3, Δτ_min=0.005
     if fieldmap is None:
         fieldmap = np.zeros((gridding.nX, gridding.nY), dtype=np.float32)
     reg = L2Regularization(0.0) if reg is None else reg
+    weight = density_weight(gridding) if weight is None else weight
     StartTime = np.asarray(StartTime, dtype=np.float64).reshape(-1)
     kspha_dt = kspha_derivative(kspha, dt)
     rec = _rec_params(gridding, reg, iter_max, solver)

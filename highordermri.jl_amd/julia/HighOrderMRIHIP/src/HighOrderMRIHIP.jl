@@ -18,10 +18,12 @@ using LinearAlgebra
 using LinearOperators
 using LinearOperatorCollection
 using ChainRulesCore
+using AMDGPU
 import HighOrderMRI
 using HighOrderMRI: Grid, HighOrderExplicitOp
 
-export HIPExplicitOp, hip_explicit_loss, hip_FindDelay, hip_apply_girf, set_kspha!, set_kspha_dt!
+export HIPExplicitOp, hip_explicit_loss, hip_explicit_grad, hip_FindDelay, hip_apply_girf, set_kspha!,
+       set_kspha_dt!, explicit_gpu_normal, route!
 
 const libhoe = get(ENV, "HOE_LIBRARY", joinpath(@__DIR__, "..", "..", "..", "libhoe.so"))
 
@@ -59,6 +61,18 @@ end
 
 ondevice(x) = !(x isa Array)
 flags(x) = ondevice(x) ? HOE_DEVICE_PTRS : Cint(0)
+
+# libhoe reads device vectors on its own stream: work queued on the caller's AMDGPU
+# stream that writes them must be finished first.
+function sync_for(xs...)
+    any(ondevice, xs) && AMDGPU.synchronize()
+    return nothing
+end
+
+# Identity of a weights vector handed to libhoe (objectid can be reused by a new array
+# at the address of a collected one, a counter cannot).
+const WEIGHTS_GEN = Threads.Atomic{UInt}(0)
+next_weights_id() = Threads.atomic_add!(WEIGHTS_GEN, UInt(1)) + UInt(1)
 
 mutable struct HIPExplicitOp{F,G} <: HighOrderExplicitOp{ComplexF32}
     nrow::Int
@@ -143,17 +157,60 @@ function HIPExplicitOp(grid::Grid{Float32}, kspha::AbstractMatrix{Float32}, time
     op = HIPExplicitOp(nSam * nCha, prod(grid.matrixSize), false, false, nothing, nothing, nothing,
                        0, 0, 0, ComplexF32[], ComplexF32[], h[], nSam, nCha, nTermEff, UInt(0))
     op.prod! = (res, xv) -> begin
+        sync_for(xv, res)
         check(ccall((:hoe_forward, libhoe), Cint, (Ptr{Cvoid}, Ptr{ComplexF32}, Ptr{ComplexF32}, Cint),
                     handle(op), pointer(xv), pointer(res), flags(xv)), op.handle)
         res
     end
     op.ctprod! = (res, yv) -> begin
+        sync_for(yv, res)
         check(ccall((:hoe_adjoint, libhoe), Cint, (Ptr{Cvoid}, Ptr{ComplexF32}, Ptr{ComplexF32}, Cint),
                     handle(op), pointer(yv), pointer(res), flags(yv)), op.handle)
         res
     end
     finalizer(close, op)
     return op
+end
+
+# ---- the reference's family constructor (HighOrderExplicitOp.jl:44-51) ----
+
+# Implementations of HighOrderExplicitOp(...; implementation) that build a HIPExplicitOp
+# besides :hip (route!).
+const ROUTED = Dict{Symbol,Bool}(:kernel => false, :array => false)
+const ROUTE_BACKEND = Ref(:mixed)
+
+"""
+    route!(; kernel=true, array=false, backend=:mixed)
+
+Make the reference's `HighOrderExplicitOp(grid, kspha, times; implementation, ...)` build a
+`HIPExplicitOp` (with `backend`) for `implementation=:kernel` and/or `:array`, so callers
+that construct operators themselves (`FindDelay` with its default `:array`,
+demo/Recon_Benchmark.jl and demo/CompareMeasuredNominal.jl with `:kernel`) run on the GPU
+unchanged. `implementation=:hip` always does.
+"""
+function route!(; kernel::Bool=true, array::Bool=false, backend::Symbol=:mixed)
+    haskey(BACKENDS, backend) || throw(ArgumentError("unknown explicit backend $backend"))
+    ROUTED[:kernel] = kernel
+    ROUTED[:array] = array
+    ROUTE_BACKEND[] = backend
+    return nothing
+end
+
+# Keywords of HIPExplicitOp; the reference's execution keywords (nBlock, arrayType,
+# verbose, block sizes, phase/GEMM options) have no meaning here and are dropped.
+const HIP_KEYWORDS = (:fieldmap, :csm, :mask, :concomitant, :kspha_dt, :recon_terms, :k_nominal, :gpus, :dwell)
+
+function HighOrderMRI.HighOrderExplicitOp(grid::Grid{T}, kspha::Matrix{T}, times::Vector{T};
+                                          implementation::Symbol=:array, kwargs...) where {T<:Union{Float32,Float64}}
+    if implementation === :hip || get(ROUTED, implementation, false)
+        kw = Dict{Symbol,Any}(k => v for (k, v) in kwargs if k in HIP_KEYWORDS)
+        b = get(kwargs, :backend, ROUTE_BACKEND[])
+        kw[:backend] = haskey(BACKENDS, b) ? b : ROUTE_BACKEND[]
+        return HIPExplicitOp(grid, kspha, times; kw...)
+    end
+    implementation === :array && return HighOrderMRI.ArrayExplicitOp(grid, kspha, times; kwargs...)
+    implementation === :kernel && return HighOrderMRI.CUDAExplicitOp(grid, kspha, times; kwargs...)
+    throw(ArgumentError("implementation must be :array, :kernel or :hip"))
 end
 
 function Base.adjoint(op::HIPExplicitOp)
@@ -167,17 +224,28 @@ function Base.close(op::HIPExplicitOp)
     return nothing
 end
 
-# normalOperator(W ∘ op) supplies WᴴW; normalOperator(op, D) takes the squared diagonal
-# (explicit_gpu_operator.jl:399-416).
+# normalOperator(W ∘ op) supplies WᴴW; normalOperator(op, D) takes the squared diagonal,
+# real within 10 eps, nonnegative and finite (explicit_gpu_operator.jl:399-416).
+function normal_weights(op::HIPExplicitOp, raw)
+    raw === nothing && return nothing
+    A = Array(raw)
+    isempty(A) && throw(DimensionMismatch("normal weights"))
+    all(isfinite, A) || throw(ArgumentError("normal weights must be finite"))
+    if eltype(A) <: Complex
+        scale = max(1, maximum(abs, A))
+        maximum(abs ∘ imag, A) <= 10 * eps(Float32) * scale ||
+            throw(ArgumentError("normal weights must already be real nonnegative WᴴW"))
+    end
+    D = Vector{Float32}(real.(vec(A)))
+    minimum(D) >= 0 || throw(ArgumentError("normal weights must be nonnegative"))
+    length(D) in (op.nSam, op.nrow) || throw(DimensionMismatch("normal weights"))
+    return D
+end
+
 function LinearOperatorCollection.normalOperator(op::HIPExplicitOp, weights=nothing; kwargs...)
     raw = weights isa LinearOperatorCollection.WeightingOp ? weights.weights : weights
-    D = raw === nothing ? nothing : Vector{Float32}(real.(Array(raw)))
-    if D !== nothing
-        all(isfinite, D) || throw(ArgumentError("normal weights must be finite"))
-        minimum(D) >= 0 || throw(ArgumentError("normal weights must be nonnegative"))
-        length(D) in (op.nSam, op.nrow) || throw(DimensionMismatch("normal weights"))
-    end
-    id = objectid(D)
+    D = normal_weights(op, raw)
+    id = next_weights_id()
     normal = (res, xv) -> begin
         h = handle(op)
         if op.weights_id != id
@@ -185,12 +253,22 @@ function LinearOperatorCollection.normalOperator(op::HIPExplicitOp, weights=noth
                                        D === nothing ? C_NULL : pointer(D), D === nothing ? 0 : length(D)), h)
             op.weights_id = id
         end
+        sync_for(xv, res)
         check(ccall((:hoe_normal, libhoe), Cint, (Ptr{Cvoid}, Ptr{Float32}, Int64, Ptr{ComplexF32}, Ptr{ComplexF32}, Cint),
                     h, C_NULL, HOE_CURRENT_WEIGHTS, pointer(xv), pointer(res), flags(xv)), h)
         res
     end
     return LinearOperator{ComplexF32}(op.ncol, op.ncol, false, true, normal, nothing, normal)
 end
+
+"""
+    explicit_gpu_normal(op::HIPExplicitOp; weights=nothing)
+
+Normal operator from raw weights `W`: `abs2.(weights)` is squared once here
+(explicit_gpu_operator.jl:418-433).
+"""
+explicit_gpu_normal(op::HIPExplicitOp; weights=nothing) =
+    LinearOperatorCollection.normalOperator(op, weights === nothing ? nothing : abs2.(Array(weights)))
 
 """
     hip_explicit_loss(op, h, τ, x, y)
@@ -203,13 +281,23 @@ function hip_explicit_loss(op::HIPExplicitOp, h::AbstractMatrix{Float32}, τ::Re
     return loss
 end
 
+"""
+    hip_explicit_grad(op, h, τ, x, y) -> (loss, g_h, g_τ)
+
+Sets the traces `h` `(nTerm_eff, nSam)` and the delay `τ`, and returns `‖E(h, τ) x − y‖²`,
+its gradient with respect to `h` and its derivative with respect to `τ`. `x` and `y` are
+both host `Array`s or both device arrays; with device arrays `h` may be a device array
+too and `g_h` is one (no host round trip).
+"""
 function hip_explicit_grad(op::HIPExplicitOp, h::AbstractMatrix{Float32}, τ::Real, x, y)
-    H = Matrix{Float32}(h)
-    size(H) == (op.nTerm, op.nSam) || throw(DimensionMismatch("coefficients"))
+    ondevice(x) == ondevice(y) || throw(ArgumentError("x and y must both be host or both be device arrays"))
+    size(h) == (op.nTerm, op.nSam) || throw(DimensionMismatch("coefficients"))
     hd = handle(op)
-    GC.@preserve H check(ccall((:hoe_set_coefficients, libhoe), Cint, (Ptr{Cvoid}, Ptr{Float32}, Float64),
-                               hd, pointer(H), Float64(τ)), hd)
-    g = similar(H)
+    H = ondevice(h) ? h : Matrix{Float32}(h)
+    sync_for(H, x, y)
+    GC.@preserve H check(ccall((:hoe_set_coefficients, libhoe), Cint, (Ptr{Cvoid}, Ptr{Float32}, Float64, Cint),
+                               hd, pointer(H), Float64(τ), flags(H)), hd)
+    g = ondevice(x) ? ROCArray{Float32}(undef, op.nTerm, op.nSam) : Matrix{Float32}(undef, op.nTerm, op.nSam)
     gτ, loss = Ref{Float64}(0), Ref{Float64}(0)
     check(ccall((:hoe_grad, libhoe), Cint,
                 (Ptr{Cvoid}, Ptr{ComplexF32}, Ptr{ComplexF32}, Ptr{Float32}, Ref{Float64}, Ref{Float64}, Cint),

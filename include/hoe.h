@@ -181,8 +181,11 @@ int hoe_set_weights(hoe_handle *h, const float *D, int64_t n_d);
  * sample index s + tau/dwell with uniform Catmull-Rom interpolation (clamped
  * ends); tau = 0 uses them unchanged. kspha == NULL keeps the current
  * traces. The t*fieldmap term is not delayed, as in the reference's delay
- * model (FindDelay.jl:179-201, array_operator.jl:187-235). Host pointers. */
-int hoe_set_coefficients(hoe_handle *h, const float *kspha, double tau);
+ * model (FindDelay.jl:179-201, array_operator.jl:187-235). kspha is a host
+ * pointer, or a device pointer on the handle's GPU with HOE_DEVICE_PTRS (then
+ * HOE_ASYNC returns without waiting); the records are rebuilt on the device,
+ * behind the work already queued on the handle. */
+int hoe_set_coefficients(hoe_handle *h, const float *kspha, double tau, int flags);
 
 /* Replace kspha (n_term x n_sam, term fastest, the layout of hoe_model_desc)
  * and process it exactly as hoe_create does: the handle's recon_terms flags
@@ -206,10 +209,13 @@ int hoe_set_kspha_dt(hoe_handle *h, const float *kspha_dt);
 
 /* Gradient of L(h,tau) = ||E(h,tau) x - y||^2.
  * x: n_grid complex, y: n_sam*n_cha complex (flags as for hoe_forward).
- * g_h: n_term_eff*n_sam floats, dL/dh (host); g_tau: dL/dtau (seconds^-1);
- * loss: L. Reductions over voxels, samples and coils run in FP64. Any of
- * g_h, g_tau, loss may be NULL. No reference counterpart exists; the nearest
- * reference code is the tau-sensitivity product (array_operator.jl:187-235). */
+ * g_h: n_term_eff*n_sam floats (term fastest), dL/dh: host memory, or device
+ * memory on the handle's GPU with HOE_DEVICE_PTRS (written in place, no host
+ * copy); g_tau: dL/dtau (seconds^-1) and loss: L, host scalars. Reductions over
+ * voxels, samples and coils run in FP64. Any of g_h, g_tau, loss may be NULL.
+ * The call returns when the results are written. No reference counterpart
+ * exists; the nearest reference code is the tau-sensitivity product
+ * (array_operator.jl:187-235). */
 int hoe_grad(hoe_handle *h, const void *x, const void *y, float *g_h, double *g_tau,
              double *loss, int flags);
 

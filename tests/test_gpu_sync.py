@@ -40,6 +40,31 @@ def test_set_kspha_equals_a_new_operator(gpu_lib):
             assert O.relerr(y, O.forward_f64(m, x)) < 3e-5
 
 
+# unit weights: the oracle loop's default (the delay-update rule is what is compared)
+UNIT = lambda k: np.ones(np.asarray(k).shape[1], np.float32)  # noqa: E731
+
+
+def test_find_delay_default_weights_are_the_density(gpu_lib):
+    """Without ``weight`` the estimators recompute the reference's density weights,
+    samplingDensity of the resampled rows 2:3, at every update (FindDelay.jl:183,
+    FindDelay_multishot.jl:120): the same updates as passing that weight explicitly,
+    and not those of unit weights."""
+    p = O.delay_test_problem()
+    dt = float(p["dt"])
+    args = (p["grid"], p["data"], p["kspha"], p["datatime"], p["StartTime"], p["dt"])
+    kw = dict(Δτ_min=np.float32(1e-3 * dt), fieldmap=p["fieldmap"], csm=p["csm"], recon_terms="1111",
+              iter_max=10, max_updates=4, return_history=True)
+    _, h_def = hoe.FindDelay(*args, **kw)
+    dens = lambda k: hoe.samplingDensity(np.asarray(k)[1:3], (p["grid"].nX, p["grid"].nY))  # noqa: E731
+    _, h_exp = hoe.FindDelay(*args, weight=dens, **kw)
+    _, h_unit = hoe.FindDelay(*args, weight=UNIT, **kw)
+    assert h_def == h_exp
+    assert h_def != h_unit
+    _, hm = hoe.FindDelay_multishot(p["grid"], p["data"], p["kspha"][None], p["datatime"][None],
+                                    np.array([p["StartTime"]]), p["dt"], **kw)
+    np.testing.assert_allclose(np.array(hm) / dt, np.array(h_def) / dt, atol=1e-5)
+
+
 @pytest.mark.parametrize("backend", ["direct", "mixed_fp16"])
 def test_find_delay_follows_the_oracle(backend, gpu_lib):
     """10 inner CGNR iterations: there FP32 and FP64 CG give delay iterates within
@@ -52,7 +77,7 @@ def test_find_delay_follows_the_oracle(backend, gpu_lib):
     t0 = time.perf_counter()
     tau, hist = hoe.FindDelay(*args, Δτ_min=np.float32(1e-3 * dt), fieldmap=p["fieldmap"], csm=p["csm"],
                               recon_terms="1111", iter_max=10, backend=backend, max_updates=30,
-                              return_history=True)
+                              return_history=True, weight=UNIT)
     wall = time.perf_counter() - t0
     tau_o, hist_o = O.find_delay_f64(p["grid"], p["data"], p["kspha"], p["datatime"], float(p["StartTime"]), dt,
                                      dtau_min=1e-3 * dt, fieldmap=p["fieldmap"], csm=p["csm"],
@@ -70,7 +95,7 @@ def test_find_delay_multishot(gpu_lib):
     p = O.delay_test_problem()
     dt = float(p["dt"])
     common = dict(Δτ_min=np.float32(1e-3 * dt), fieldmap=p["fieldmap"], csm=p["csm"], recon_terms="1111",
-                  iter_max=20, max_updates=30, return_history=True)
+                  iter_max=20, max_updates=30, return_history=True, weight=UNIT)
     tau1, h1 = hoe.FindDelay(p["grid"], p["data"], p["kspha"], p["datatime"], p["StartTime"], p["dt"], **common)
     tau_m, hm = hoe.FindDelay_multishot(p["grid"], p["data"], p["kspha"][None], p["datatime"][None],
                                         np.array([p["StartTime"]]), p["dt"], **common)
@@ -111,7 +136,7 @@ def test_find_delay_at_200x200_32_coils(gpu_lib):
                                  backend="direct") as op:
         data = (op * c["image"]).reshape(ns, -1, order="F")
     kw = dict(Δτ_min=np.float32(2e-3 * dt), fieldmap=c["fieldmap"], csm=c["csm"], recon_terms="1111",
-              iter_max=10, backend="mixed_fp16", return_history=True)
+              iter_max=10, backend="mixed_fp16", return_history=True, weight=UNIT)
     t0 = time.perf_counter()
     tau, hist = hoe.FindDelay(c["grid"], data, kspha, datatime, start, np.float32(dt), max_updates=12, **kw)
     wall = time.perf_counter() - t0
