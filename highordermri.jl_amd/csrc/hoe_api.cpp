@@ -19,6 +19,7 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "hoe_internal.h"
@@ -141,20 +142,41 @@ void catmull_rom(double f, double w[4], double dw[4]) {
 // order-0/2/3 flag zeroes its rows, a cleared order-1 flag replaces rows 2-4 with
 // k_nominal (3 x n_sam, term fastest; NULL keeps kspha's own rows, the reference's
 // default k_nominal = kspha[2:4, :]). Concomitant rows are left as they are.
-void prepare_traces(Handle *h, const float *kspha, const float *k_nominal) {
+template <typename T>
+void prepare_traces_t(const Handle *h, std::vector<T> &traces, const T *kspha, const T *k_nominal) {
     const int L = h->L, n_term = h->n_term_in;
     const bool *t = h->order;
     for (int64_t s = 0; s < h->n_sam; ++s) {
-        float *r = &h->traces[(size_t)s * L];
+        T *r = &traces[(size_t)s * L];
         for (int l = 0; l < n_term; ++l) r[l] = kspha[(size_t)s * n_term + l];
-        if (!t[0]) r[0] = 0.0f;
+        if (!t[0]) r[0] = T(0);
         if (!t[1] && k_nominal)
             for (int l = 0; l < 3; ++l) r[1 + l] = k_nominal[(size_t)s * 3 + l];
         if (!t[2])
-            for (int l = 4; l < 9; ++l) r[l] = 0.0f;
+            for (int l = 4; l < 9; ++l) r[l] = T(0);
         if (!t[3] && n_term == 16)
-            for (int l = 9; l < 16; ++l) r[l] = 0.0f;
+            for (int l = 9; l < 16; ++l) r[l] = T(0);
     }
+}
+
+void prepare_traces(Handle *h, const float *kspha, const float *k_nominal) {
+    prepare_traces_t(h, h->traces, kspha, k_nominal);
+}
+
+// Sample records of the FP64 backend from its traces (no delay): [k_1, k of the active
+// terms, t, 0...], uploaded to coef; dcoef stays zero.
+int upload_coef_f64(Handle *h, const std::vector<double> &times) {
+    std::vector<double> rec((size_t)h->ns_pad * h->sk, 0.0);
+    const int L = h->L;
+    for (int64_t s = 0; s < h->n_sam; ++s) {
+        double *r = &rec[(size_t)s * h->sk];
+        const double *k = &h->traces64[(size_t)s * L];
+        r[0] = k[0];
+        for (size_t i = 0; i < h->active.size(); ++i) r[1 + i] = k[h->active[i]];
+        r[1 + h->active.size()] = times[(size_t)s];
+    }
+    HOE_TRY_HIP(h, hipMemcpy(h->coef.p, rec.data(), rec.size() * sizeof(double), hipMemcpyHostToDevice));
+    return HOE_OK;
 }
 
 // The sample records coef ([k_1, k of the active terms, t, 0...] per sample, FP64) and
@@ -218,8 +240,10 @@ int reject_dt(Handle *h) {
 }
 
 
-// Upload normal weights D (already squared) into dvec [ns_pad][ncs].
-int set_weights(Handle *h, const float *D, int64_t n_d) {
+// Upload normal weights D (already squared) into dvec [ns_pad][ncs], in the handle's
+// precision (FP32, or FP64 for the FP64 backend).
+template <typename T>
+int set_weights_t(Handle *h, const T *D, int64_t n_d) {
     const int64_t ns = h->n_sam, nc = h->n_cha;
     if (D == nullptr && n_d == 0) {
         if (h->weights_unit) return HOE_OK;
@@ -228,22 +252,39 @@ int set_weights(Handle *h, const float *D, int64_t n_d) {
         if (n_d != ns && n_d != ns * nc) return set_err(h, HOE_EDIM, "normal weights");
         for (int64_t i = 0; i < n_d; ++i) {
             if (!std::isfinite(D[i])) return set_err(h, HOE_EINVAL, "normal weights must be finite");
-            if (D[i] < 0.0f) return set_err(h, HOE_EINVAL, "normal weights must be nonnegative");
+            if (D[i] < T(0)) return set_err(h, HOE_EINVAL, "normal weights must be nonnegative");
         }
     }
-    std::vector<float> w((size_t)h->ns_pad * h->ncs, 0.0f);
-    for (int64_t c = 0; c < nc; ++c)
-        for (int64_t s = 0; s < ns; ++s) {
-            float v = 1.0f;
-            if (D) v = (n_d == ns) ? D[s] : D[c * ns + s];
-            w[(size_t)s * h->ncs + c] = v;
-        }
-    // work queued with HOE_ASYNC may still read the current weights: the copy (on the
-    // null stream, which a non-blocking stream does not wait for) goes in after it
-    HOE_TRY_HIP(h, hipStreamSynchronize(h->stream));
-    HOE_TRY_HIP(h, hipMemcpy(h->dvec.p, w.data(), w.size() * sizeof(float), hipMemcpyHostToDevice));
+    auto fill = [&](auto zero) -> int {
+        using W = decltype(zero);
+        std::vector<W> w((size_t)h->ns_pad * h->ncs, W(0));
+        for (int64_t c = 0; c < nc; ++c)
+            for (int64_t s = 0; s < ns; ++s) {
+                W v = W(1);
+                if (D) v = (W)((n_d == ns) ? D[s] : D[c * ns + s]);
+                w[(size_t)s * h->ncs + c] = v;
+            }
+        // work queued with HOE_ASYNC may still read the current weights: the copy (on the
+        // null stream, which a non-blocking stream does not wait for) goes in after it
+        HOE_TRY_HIP(h, hipStreamSynchronize(h->stream));
+        HOE_TRY_HIP(h, hipMemcpy(h->dvec.p, w.data(), w.size() * sizeof(W), hipMemcpyHostToDevice));
+        return HOE_OK;
+    };
+    const int rc = h->f64 ? fill(0.0) : fill(0.0f);
+    if (rc) return rc;
     h->weights_unit = (D == nullptr);
     return HOE_OK;
+}
+
+int set_weights(Handle *h, const float *D, int64_t n_d) { return set_weights_t(h, D, n_d); }
+
+// Scale factors enter the FP32 kernels as float (rounded once, as before), the FP64 ones
+// unrounded.
+inline double prec_scale(const Handle *h, double v) { return h->f64 ? v : (double)(float)v; }
+
+// Entry points the FP64 backend does not provide.
+int reject_f64(Handle *h, const char *what) {
+    return h->f64 ? set_err(h, HOE_EINVAL, std::string(what) + " is not provided on the FP64 backend") : HOE_OK;
 }
 
 double ev_ms(hipEvent_t a, hipEvent_t b) {
@@ -331,7 +372,15 @@ int hoe_device_count(void) {
     return n;
 }
 
-int hoe_create(const hoe_model_desc *d, hoe_handle **out) {
+}  // extern "C"
+
+namespace {
+
+// hoe_create / hoe_create_f64: Desc is hoe_model_desc (FP32 data) or hoe_model_desc_f64.
+template <typename Desc>
+int create_impl(const Desc *d, hoe_handle **out) {
+    constexpr bool F64 = std::is_same<Desc, hoe_model_desc_f64>::value;
+    using T = typename std::conditional<F64, double, float>::type;   // element type of the inputs
     g_err.clear();
     if (!out) return set_err(nullptr, HOE_EINVAL, "out pointer is NULL");
     *out = nullptr;
@@ -342,9 +391,15 @@ int hoe_create(const hoe_model_desc *d, hoe_handle **out) {
     if (d->n_sam < 1 || d->n_cha < 1) return set_err(nullptr, HOE_EINVAL, "samples and coils must be positive");
     if (!d->kspha || !d->times) return set_err(nullptr, HOE_EINVAL, "kspha and times are required");
     if (!d->csm && d->n_cha != 1) return set_err(nullptr, HOE_EDIM, "csm and grid");
-    if (d->backend < HOE_BACKEND_DIRECT || d->backend > HOE_BACKEND_MIXED_BF16)
+    int32_t backend = HOE_BACKEND_FP64;
+    double dwell = 0.0;
+    if constexpr (!F64) {
+        backend = d->backend;
+        dwell = d->dwell;
+    }
+    if (!F64 && (backend < HOE_BACKEND_DIRECT || backend > HOE_BACKEND_MIXED_BF16))
         return set_err(nullptr, HOE_EINVAL, "unknown explicit backend");
-    if (!(d->dwell >= 0.0) || !std::isfinite(d->dwell)) return set_err(nullptr, HOE_EINVAL, "dwell must be >= 0");
+    if (!(dwell >= 0.0) || !std::isfinite(dwell)) return set_err(nullptr, HOE_EINVAL, "dwell must be >= 0");
     const int n_term = d->n_term;
     // order flags (prep_kspha.jl:8-33)
     bool t[4] = {true, true, true, n_term == 16};
@@ -388,21 +443,28 @@ int hoe_create(const hoe_model_desc *d, hoe_handle **out) {
     // coil stride: one 32-coil MFMA group, or whole pairs of groups (the mixed kernels keep two
     // groups per wave when there are more than 32 coils)
     h->ncs = (int32_t)(h->n_cha <= 32 ? 32 : round_up(h->n_cha, 64));
-    h->backend = d->backend;
+    h->backend = backend;
+    h->f64 = F64;
+    h->cbytes = F64 ? 16 : 8;
     h->device = d->device;
-    h->dwell = d->dwell;
+    h->dwell = dwell;
     h->mask_idx_h = idx;
 
     // ---- prepared coefficient traces (term fastest), prep_kspha + prep_concomitant ----
     const int L = h->L;
     const int64_t ns = h->n_sam;
     for (int i = 0; i < 4; ++i) h->order[i] = t[i];
-    h->traces.assign((size_t)L * ns, 0.0f);
-    prepare_traces(h, d->kspha, d->k_nominal);
+    std::vector<T> &traces = [&]() -> std::vector<T> & {
+        if constexpr (F64) return h->traces64;
+        else return h->traces;
+    }();
+    traces.assign((size_t)L * ns, T(0));
+    prepare_traces_t<T>(h, traces, d->kspha, d->k_nominal);
     if (d->concomitant)
         for (int64_t s = 0; s < ns; ++s)
-            for (int l = 0; l < 4; ++l) h->traces[(size_t)s * L + n_term + l] = d->concomitant[(size_t)s * 4 + l];
-    h->times.assign(d->times, d->times + ns);
+            for (int l = 0; l < 4; ++l) traces[(size_t)s * L + n_term + l] = d->concomitant[(size_t)s * 4 + l];
+    if constexpr (!F64) h->times.assign(d->times, d->times + ns);
+    const std::vector<double> times64(d->times, d->times + ns);
 
     // ---- basis columns in FP64; identically zero columns (z terms on a z = 0
     // plane, ...) contribute nothing and are dropped from the kernel records ----
@@ -442,7 +504,8 @@ int hoe_create(const hoe_model_desc *d, hoe_handle **out) {
     // voxel records [b_{a_1}..b_{a_m}, fieldmap, 0...]
     std::vector<double> brec((size_t)h->nv_pad * h->sb, 0.0);
     float csm_max = 0.0f;
-    std::vector<float2> csmv((size_t)h->nv_pad * h->ncs, make_float2(0.0f, 0.0f));
+    using C = typename std::conditional<F64, double2, float2>::type;
+    std::vector<C> csmv((size_t)h->nv_pad * h->ncs, C{T(0), T(0)});
     const int na = (int)h->active.size();
     for (int64_t v = 0; v < nv; ++v) {
         const int64_t g = idx[(size_t)v];
@@ -450,10 +513,10 @@ int hoe_create(const hoe_model_desc *d, hoe_handle **out) {
         for (int i = 0; i < na; ++i) r[i] = bfull[(size_t)v * L + h->active[i]];
         r[na] = d->fieldmap ? (double)d->fieldmap[g] : 0.0;
         for (int64_t c = 0; c < h->n_cha; ++c) {
-            float2 val = make_float2(1.0f, 0.0f);
-            if (d->csm) val = make_float2(d->csm[2 * (c * n_grid + g)], d->csm[2 * (c * n_grid + g) + 1]);
+            C val{T(1), T(0)};
+            if (d->csm) val = C{d->csm[2 * (c * n_grid + g)], d->csm[2 * (c * n_grid + g) + 1]};
             csmv[(size_t)v * h->ncs + c] = val;
-            csm_max = std::max(csm_max, std::sqrt(val.x * val.x + val.y * val.y));
+            csm_max = std::max(csm_max, (float)std::sqrt(val.x * val.x + val.y * val.y));
         }
     }
     h->csm_max = csm_max;
@@ -481,10 +544,10 @@ int hoe_create(const hoe_model_desc *d, hoe_handle **out) {
     std::vector<double> beta;
     {
         int run = -1;
-        float py = 0.0f, pz = 0.0f;
+        T py = T(0), pz = T(0);
         for (int64_t v = 0; v < nv; ++v) {
             const int64_t g = idx[(size_t)v];
-            const float yv = d->y[g], zv = d->z[g];
+            const T yv = d->y[g], zv = d->z[g];
             if (run < 0 || yv != py || zv != pz) {
                 ++run;
                 py = yv;
@@ -563,6 +626,7 @@ int hoe_create(const hoe_model_desc *d, hoe_handle **out) {
     for (auto &ev : h->ev)
         if ((e = hipEventCreate(&ev)) != hipSuccess) return fail(set_err(nullptr, HOE_EHIP, "hipEventCreate failed"));
     const int n_gb = h->ncs / 32;
+    const size_t cb = (size_t)h->cbytes;
     const int nbw = n_gb >= 2 ? 2 : 1;
     const int64_t n_gb_pad = ((n_gb + nbw - 1) / nbw) * nbw;
     struct {
@@ -576,25 +640,25 @@ int hoe_create(const hoe_model_desc *d, hoe_handle **out) {
         {&h->dtrec, sizeof(double) * (size_t)h->ns_pad * h->sk},
         {&h->ghbuf, sizeof(float) * (size_t)ns * L},
         {&h->growmap, sizeof(int32_t) * (size_t)L},
-        {&h->traces_d, sizeof(float) * (size_t)L * ns},
-        {&h->times_d, sizeof(float) * (size_t)ns},
+        {&h->traces_d, F64 ? 0 : sizeof(float) * (size_t)L * ns},
+        {&h->times_d, F64 ? 0 : sizeof(float) * (size_t)ns},
         {&h->act_d, sizeof(int32_t) * std::max<size_t>(h->active.size(), 1)},
-        {&h->csm, sizeof(float2) * csmv.size()},
+        {&h->csm, sizeof(C) * csmv.size()},
         {&h->hrec, sizeof(double) * hrec.size()},
         {&h->vrun, sizeof(int32_t) * vrun.size()},
         {&h->beta, sizeof(double) * std::max<size_t>(beta.size(), 1)},
-        {&h->xm, sizeof(float2) * (size_t)h->nv_pad},
-        {&h->zbuf, sizeof(float2) * (size_t)h->nv_pad * h->ncs},
-        {&h->zfrag, (size_t)n_gb_pad * (size_t)(h->nv_pad / 8) * 64 * 64},
-        {&h->ybuf, sizeof(float2) * (size_t)h->ns_pad * h->ncs},
-        {&h->yfrag, (size_t)n_gb_pad * (size_t)(h->ns_pad / 8) * 64 * 64},
-        {&h->dvec, sizeof(float) * (size_t)h->ns_pad * h->ncs},
-        {&h->io_x, sizeof(float2) * (size_t)n_grid},
-        {&h->io_x2, sizeof(float2) * (size_t)n_grid},
-        {&h->io_y, sizeof(float2) * (size_t)(ns * h->n_cha)},
-        {&h->io_y2, sizeof(float2) * (size_t)(ns * h->n_cha)},
+        {&h->xm, cb * (size_t)h->nv_pad},
+        {&h->zbuf, cb * (size_t)h->nv_pad * h->ncs},
+        {&h->zfrag, F64 ? 0 : (size_t)n_gb_pad * (size_t)(h->nv_pad / 8) * 64 * 64},
+        {&h->ybuf, cb * (size_t)h->ns_pad * h->ncs},
+        {&h->yfrag, F64 ? 0 : (size_t)n_gb_pad * (size_t)(h->ns_pad / 8) * 64 * 64},
+        {&h->dvec, (cb / 2) * (size_t)h->ns_pad * h->ncs},
+        {&h->io_x, cb * (size_t)n_grid},
+        {&h->io_x2, cb * (size_t)n_grid},
+        {&h->io_y, cb * (size_t)(ns * h->n_cha)},
+        {&h->io_y2, cb * (size_t)(ns * h->n_cha)},
         {&h->scal, 4096},
-        {&h->cg_vec, sizeof(float2) * (size_t)h->nv_pad * 4},
+        {&h->cg_vec, cb * (size_t)h->nv_pad * 4},
         {&h->red, sizeof(double) * 4096},
         {&h->gk, sizeof(double) * (size_t)h->LK * ns},
     };
@@ -606,12 +670,12 @@ int hoe_create(const hoe_model_desc *d, hoe_handle **out) {
     }
     e = hipMemcpy(h->mask_idx.p, idx.data(), sizeof(int32_t) * nv, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(h->basis.p, brec.data(), sizeof(double) * brec.size(), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(h->csm.p, csmv.data(), sizeof(float2) * csmv.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(h->csm.p, csmv.data(), sizeof(C) * csmv.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(h->hrec.p, hrec.data(), sizeof(double) * hrec.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(h->vrun.p, vrun.data(), sizeof(int32_t) * vrun.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess && !beta.empty())
         e = hipMemcpy(h->beta.p, beta.data(), sizeof(double) * beta.size(), hipMemcpyHostToDevice);
-    if (e == hipSuccess)
+    if (e == hipSuccess && !F64)
         e = hipMemcpy(h->times_d.p, h->times.data(), sizeof(float) * (size_t)ns, hipMemcpyHostToDevice);
     if (e == hipSuccess && !h->active.empty())
         e = hipMemcpy(h->act_d.p, h->active.data(), sizeof(int32_t) * h->active.size(), hipMemcpyHostToDevice);
@@ -624,11 +688,11 @@ int hoe_create(const hoe_model_desc *d, hoe_handle **out) {
     }
     if (e != hipSuccess) return fail(set_err(nullptr, HOE_EHIP, std::string("upload failed: ") + hipGetErrorString(e)));
     e = hoe::prepare_backend(h, h->backend);
-    if (e == hipSuccess) e = hoe::prepare_grad(h);
+    if (e == hipSuccess && !F64) e = hoe::prepare_grad(h);
     if (e != hipSuccess)
         return fail(set_err(nullptr, e == hipErrorOutOfMemory ? HOE_ENOMEM : HOE_EHIP,
                             std::string("schedule setup failed: ") + hipGetErrorString(e)));
-    int rc = upload_coef(h);
+    int rc = F64 ? upload_coef_f64(h, times64) : upload_coef(h);
     if (rc != HOE_OK) return fail(set_err(nullptr, rc, h->err));
     h->weights_unit = false;
     rc = set_weights(h, nullptr, 0);
@@ -636,6 +700,14 @@ int hoe_create(const hoe_model_desc *d, hoe_handle **out) {
     *out = reinterpret_cast<hoe_handle *>(h);
     return HOE_OK;
 }
+
+}  // namespace
+
+extern "C" {
+
+int hoe_create(const hoe_model_desc *d, hoe_handle **out) { return create_impl(d, out); }
+
+int hoe_create_f64(const hoe_model_desc_f64 *d, hoe_handle **out) { return create_impl(d, out); }
 
 int hoe_get_info(const hoe_handle *hh, hoe_info_t *info) {
     const Handle *h = reinterpret_cast<const Handle *>(hh);
@@ -657,8 +729,12 @@ int hoe_set_backend(hoe_handle *hh, int32_t backend) {
     Handle *h = reinterpret_cast<Handle *>(hh);
     int rc = check_open(h);
     if (rc) return rc;
-    if (backend < HOE_BACKEND_DIRECT || backend > HOE_BACKEND_MIXED_BF16)
+    if (backend < HOE_BACKEND_DIRECT || backend > HOE_BACKEND_FP64)
         return set_err(h, HOE_EINVAL, "unknown explicit backend");
+    if ((backend == HOE_BACKEND_FP64) != h->f64)
+        return set_err(h, HOE_EINVAL,
+                       "the FP64 backend takes ComplexF64 vectors: create it with hoe_create_f64; other "
+                       "backends take ComplexF32 ones");
     HOE_TRY_HIP(h, hipStreamSynchronize(h->stream));
     HOE_TRY_HIP(h, hoe::prepare_backend(h, backend));
     h->backend = backend;
@@ -711,11 +787,11 @@ int hoe_forward(hoe_handle *hh, const void *x, void *y, int flags) {
     if (!y) return set_err(h, HOE_EINVAL, "output pointer is NULL");
     if (!(flags & HOE_ASYNC)) (void)hipEventRecord(h->ev[4], h->stream);
     const void *dx;
-    if ((rc = stage_in(h, flags, x, h->io_x, sizeof(float2) * h->n_grid, &dx))) return rc;
+    if ((rc = stage_in(h, flags, x, h->io_x, (size_t)h->cbytes * h->n_grid, &dx))) return rc;
     void *dy = (flags & HOE_DEVICE_PTRS) ? y : h->io_y.p;
     HOE_TRY_HIP(h, hoe::run_forward_contract(h, dx, !(flags & HOE_ASYNC)));
-    HOE_TRY_HIP(h, hoe::launch_reduce_fwd_user(h, dy, (float)(1.0 / std::sqrt((double)h->n_vox))));
-    if ((rc = finish(h, flags, y, dy, sizeof(float2) * h->n_sam * h->n_cha))) return rc;
+    HOE_TRY_HIP(h, hoe::launch_reduce_fwd_user(h, dy, prec_scale(h, 1.0 / std::sqrt((double)h->n_vox))));
+    if ((rc = finish(h, flags, y, dy, (size_t)h->cbytes * h->n_sam * h->n_cha))) return rc;
     if (!(flags & HOE_ASYNC)) record_times(h, true, false);
     return HOE_OK;
 }
@@ -727,13 +803,13 @@ int hoe_adjoint(hoe_handle *hh, const void *y, void *x, int flags) {
     if (!x) return set_err(h, HOE_EINVAL, "output pointer is NULL");
     if (!(flags & HOE_ASYNC)) (void)hipEventRecord(h->ev[4], h->stream);
     const void *dy;
-    if ((rc = stage_in(h, flags, y, h->io_y, sizeof(float2) * h->n_sam * h->n_cha, &dy))) return rc;
+    if ((rc = stage_in(h, flags, y, h->io_y, (size_t)h->cbytes * h->n_sam * h->n_cha, &dy))) return rc;
     void *dx = (flags & HOE_DEVICE_PTRS) ? x : h->io_x.p;
     HOE_TRY_HIP(h, hoe::launch_load_ybuf(h, dy, false));
-    HOE_TRY_HIP(h, hoe::run_adjoint_contract(h, (float)(1.0 / std::sqrt((double)h->n_vox)), h->xm.p,
+    HOE_TRY_HIP(h, hoe::run_adjoint_contract(h, prec_scale(h, 1.0 / std::sqrt((double)h->n_vox)), h->xm.p,
                                              !(flags & HOE_ASYNC)));
-    HOE_TRY_HIP(h, hoe::launch_scatter(h, h->xm.p, dx, 1.0f));
-    if ((rc = finish(h, flags, x, dx, sizeof(float2) * h->n_grid))) return rc;
+    HOE_TRY_HIP(h, hoe::launch_scatter(h, h->xm.p, dx, 1.0));
+    if ((rc = finish(h, flags, x, dx, (size_t)h->cbytes * h->n_grid))) return rc;
     if (!(flags & HOE_ASYNC)) record_times(h, false, true);
     return HOE_OK;
 }
@@ -743,6 +819,13 @@ int hoe_set_weights(hoe_handle *hh, const float *D, int64_t n_d) {
     int rc = check_open(h);
     if (rc) return rc;
     return set_weights(h, D, n_d);
+}
+
+int hoe_set_weights_f64(hoe_handle *hh, const double *D, int64_t n_d) {
+    Handle *h = reinterpret_cast<Handle *>(hh);
+    int rc = check_open(h);
+    if (rc) return rc;
+    return set_weights_t(h, D, n_d);
 }
 
 int hoe_normal(hoe_handle *hh, const float *D, int64_t n_d, const void *x, void *out, int flags) {
@@ -755,12 +838,12 @@ int hoe_normal(hoe_handle *hh, const float *D, int64_t n_d, const void *x, void 
         if ((rc = set_weights(h, D, n_d))) return rc;
     if (!(flags & HOE_ASYNC)) (void)hipEventRecord(h->ev[4], h->stream);
     const void *dx;
-    if ((rc = stage_in(h, flags, x, h->io_x, sizeof(float2) * h->n_grid, &dx))) return rc;
+    if ((rc = stage_in(h, flags, x, h->io_x, (size_t)h->cbytes * h->n_grid, &dx))) return rc;
     void *dout = (flags & HOE_DEVICE_PTRS) ? out : h->io_x2.p;
-    float2 *tmp = reinterpret_cast<float2 *>(h->cg_vec.p);   // masked output
-    HOE_TRY_HIP(h, hoe::run_normal_full(h, dx, tmp, (float)(1.0 / (double)h->n_vox), !(flags & HOE_ASYNC)));
-    HOE_TRY_HIP(h, hoe::launch_scatter(h, tmp, dout, 1.0f));
-    if ((rc = finish(h, flags, out, dout, sizeof(float2) * h->n_grid))) return rc;
+    void *tmp = h->cg_vec.p;   // masked output
+    HOE_TRY_HIP(h, hoe::run_normal_full(h, dx, tmp, prec_scale(h, 1.0 / (double)h->n_vox), !(flags & HOE_ASYNC)));
+    HOE_TRY_HIP(h, hoe::launch_scatter(h, tmp, dout, 1.0));
+    if ((rc = finish(h, flags, out, dout, (size_t)h->cbytes * h->n_grid))) return rc;
     if (!(flags & HOE_ASYNC)) record_times(h, true, true);
     return HOE_OK;
 }
@@ -844,6 +927,7 @@ int hoe_normal_noalloc(hoe_handle *hh, const void *x, void *out, int flags) {
     int rc = check_open(h);
     if (rc) return rc;
     if ((rc = reject_dt(h))) return rc;
+    if ((rc = reject_f64(h, "hoe_normal_noalloc"))) return rc;
     if (!x || !out) return set_err(h, HOE_EINVAL, "vector pointer is NULL");
     if (!(flags & HOE_DEVICE_PTRS))
         return set_err(h, HOE_EINVAL, "hoe_normal_noalloc takes device vectors (HOE_DEVICE_PTRS)");
@@ -868,9 +952,9 @@ int hoe_normal_noalloc(hoe_handle *hh, const void *x, void *out, int flags) {
             off += align256(sizes[i]);
         }
         if (!(flags & HOE_ASYNC)) (void)hipEventRecord(h->ev[4], h->stream);
-        float2 *tmp = reinterpret_cast<float2 *>(h->cg_vec.p);
-        HOE_TRY_HIP(h, hoe::run_normal_full(h, x, tmp, (float)(1.0 / (double)h->n_vox), !(flags & HOE_ASYNC)));
-        HOE_TRY_HIP(h, hoe::launch_scatter(h, tmp, out, 1.0f));
+        void *tmp = h->cg_vec.p;
+        HOE_TRY_HIP(h, hoe::run_normal_full(h, x, tmp, prec_scale(h, 1.0 / (double)h->n_vox), !(flags & HOE_ASYNC)));
+        HOE_TRY_HIP(h, hoe::launch_scatter(h, tmp, out, 1.0));
     }
     if ((rc = finish(h, flags, out, out, sizeof(float2) * h->n_grid))) return rc;
     if (!(flags & HOE_ASYNC)) record_times(h, true, true);
@@ -881,6 +965,7 @@ int hoe_set_coefficients(hoe_handle *hh, const float *kspha, double tau, int fla
     Handle *h = reinterpret_cast<Handle *>(hh);
     int rc = check_open(h);
     if (rc) return rc;
+    if ((rc = reject_f64(h, "hoe_set_coefficients"))) return rc;
     if (!std::isfinite(tau)) return set_err(h, HOE_EINVAL, "tau must be finite");
     if (tau != 0.0 && !(h->dwell > 0.0)) return set_err(h, HOE_EINVAL, "a delay requires dwell > 0");
     if (kspha && (flags & HOE_DEVICE_PTRS)) {
@@ -909,6 +994,7 @@ int hoe_set_kspha(hoe_handle *hh, const float *kspha, const float *k_nominal) {
     Handle *h = reinterpret_cast<Handle *>(hh);
     int rc = check_open(h);
     if (rc) return rc;
+    if ((rc = reject_f64(h, "hoe_set_kspha"))) return rc;
     if (!kspha) return set_err(h, HOE_EINVAL, "kspha is required");
     HOE_TRY_HIP(h, hipStreamSynchronize(h->stream));
     if (h->traces_stale) {
@@ -927,6 +1013,7 @@ int hoe_set_kspha_dt(hoe_handle *hh, const float *kspha_dt) {
     Handle *h = reinterpret_cast<Handle *>(hh);
     int rc = check_open(h);
     if (rc) return rc;
+    if ((rc = reject_f64(h, "hoe_set_kspha_dt"))) return rc;
     if (!kspha_dt) {
         if (!h->dt_on) return HOE_OK;
         HOE_TRY_HIP(h, hipStreamSynchronize(h->stream));
@@ -963,13 +1050,14 @@ int hoe_grad(hoe_handle *hh, const void *x, const void *y, float *g_h, double *g
     int rc = check_open(h);
     if (rc) return rc;
     if ((rc = reject_dt(h))) return rc;
+    if ((rc = reject_f64(h, "hoe_grad"))) return rc;
     (void)hipEventRecord(h->ev[4], h->stream);   // the call always waits for its results
     const void *dx, *dy;
     if ((rc = stage_in(h, flags, x, h->io_x, sizeof(float2) * h->n_grid, &dx))) return rc;
     if ((rc = stage_in(h, flags, y, h->io_y, sizeof(float2) * h->n_sam * h->n_cha, &dy))) return rc;
     // residual r = E x - y into ybuf (coil fastest)
     HOE_TRY_HIP(h, hoe::run_forward_contract(h, dx, false));
-    HOE_TRY_HIP(h, hoe::launch_reduce_fwd_ybuf(h, (float)(1.0 / std::sqrt((double)h->n_vox)), false, dy));
+    HOE_TRY_HIP(h, hoe::launch_reduce_fwd_ybuf(h, prec_scale(h, 1.0 / std::sqrt((double)h->n_vox)), false, dy));
     double *loss_dev = reinterpret_cast<double *>(h->scal.p) + 16;
     HOE_TRY_HIP(h, hoe::launch_norm2(h, h->ybuf.p, h->ns_pad * h->ncs, loss_dev));
     HOE_TRY_HIP(h, hoe::run_grad_contract(h, true));
@@ -1018,17 +1106,17 @@ int hoe_cg(hoe_handle *hh, const void *y, const float *D, int64_t n_d, int32_t i
     }
     if (!(flags & HOE_ASYNC)) (void)hipEventRecord(h->ev[4], h->stream);
     const void *dy;
-    if ((rc = stage_in(h, flags, y, h->io_y, sizeof(float2) * h->n_sam * h->n_cha, &dy))) return rc;
+    if ((rc = stage_in(h, flags, y, h->io_y, (size_t)h->cbytes * h->n_sam * h->n_cha, &dy))) return rc;
     void *dx = (flags & HOE_DEVICE_PTRS) ? x_out : h->io_x.p;
     // rhs = E^H D y (masked) into xm
     HOE_TRY_HIP(h, hoe::launch_load_ybuf(h, dy, true));
-    HOE_TRY_HIP(h, hoe::run_adjoint_contract(h, (float)(1.0 / std::sqrt((double)h->n_vox)), h->xm.p, false));
-    float2 *xs = reinterpret_cast<float2 *>(h->cg_vec.p) + 3 * h->nv_pad;
+    HOE_TRY_HIP(h, hoe::run_adjoint_contract(h, prec_scale(h, 1.0 / std::sqrt((double)h->n_vox)), h->xm.p, false));
+    void *xs = static_cast<char *>(h->cg_vec.p) + (size_t)3 * h->nv_pad * h->cbytes;
     HOE_TRY_HIP(h, hoe::cg_run(h, h->xm.p, iters, lambda, rel_tol, xs, (double *)h->cg_log.p));
-    HOE_TRY_HIP(h, hoe::launch_scatter(h, xs, dx, 1.0f));
+    HOE_TRY_HIP(h, hoe::launch_scatter(h, xs, dx, 1.0));
     if (resid_log)
         HOE_TRY_HIP(h, hipMemcpyAsync(resid_log, h->cg_log.p, log_bytes, hipMemcpyDeviceToHost, h->stream));
-    if ((rc = finish(h, flags, x_out, dx, sizeof(float2) * h->n_grid))) return rc;
+    if ((rc = finish(h, flags, x_out, dx, (size_t)h->cbytes * h->n_grid))) return rc;
     if (resid_log && (flags & HOE_ASYNC) && (flags & HOE_DEVICE_PTRS)) HOE_TRY_HIP(h, hipStreamSynchronize(h->stream));
     h->last_ms[0] = h->last_ms[1] = -1.0;
     if (!(flags & HOE_ASYNC)) h->last_ms[2] = ev_ms(h->ev[4], h->ev[5]);

@@ -53,6 +53,8 @@ struct Handle {
     int32_t sb = 0;                   // doubles per voxel record (>= LK)
     int32_t sk = 0;                   // doubles per sample record (>= LK + 1)
     int32_t backend = HOE_BACKEND_DIRECT;
+    bool f64 = false;                 // FP64 backend: ComplexF64 vectors, FP64 model and arithmetic
+    int32_t cbytes = 8;               // bytes per complex entry of vectors and workspaces (8 or 16)
     int32_t device = 0;
     int32_t cu_count = 256;
     bool closed = false;
@@ -66,6 +68,7 @@ struct Handle {
 
     // ---- host copies needed to rebuild coefficients ----
     std::vector<float> traces;        // L x n_sam, term fastest (prepared coefficients)
+    std::vector<double> traces64;     // the same for the FP64 backend
     bool traces_stale = false;        // traces_d was written from device memory; the host copy is older
     std::vector<float> times;         // n_sam
     std::vector<int32_t> mask_idx_h;  // n_vox
@@ -119,8 +122,8 @@ struct Handle {
     int gb_type = -1;     // backend whose element type gbfrag holds
 
     // schedules per backend (index = hoe_backend) and for the gradient kernel
-    Sched sched_f[3], sched_a[3], sched_g, sched_gm;
-    int occ_f[3] = {0, 0, 0}, occ_a[3] = {0, 0, 0};   // resident workgroups per CU
+    Sched sched_f[4], sched_a[4], sched_g, sched_gm;
+    int occ_f[4] = {0, 0, 0, 0}, occ_a[4] = {0, 0, 0, 0};   // resident workgroups per CU
 
     // timing of the last call (ms)
     double last_ms[3] = {-1, -1, -1};
@@ -143,25 +146,25 @@ bool grad_on_mfma(const Handle *h);
 // ---- launchers (hoe_kernels.hip) ----
 // All run on h->stream and return hipSuccess or the first HIP error.
 hipError_t launch_gather(Handle *h, const void *x_full, void *xm);
-hipError_t launch_scatter(Handle *h, const void *xm, void *x_full, float scale);
+hipError_t launch_scatter(Handle *h, const void *xm, void *x_full, double scale);
 // Forward: x_full (device, n_grid complex) -> part_f.  Uses the current backend.
 hipError_t run_forward_contract(Handle *h, const void *x_full, bool time_it);
 // Reduce part_f into the user signal layout (complex [c][s]) with scale.
-hipError_t launch_reduce_fwd_user(Handle *h, void *y_out, float scale);
+hipError_t launch_reduce_fwd_user(Handle *h, void *y_out, double scale);
 // Reduce part_f into ybuf (coil fastest) with scale and optional weights dvec
 // (weights == true) and optional subtraction of data (y_data != nullptr, user layout).
-hipError_t launch_reduce_fwd_ybuf(Handle *h, float scale, bool weights, const void *y_data);
+hipError_t launch_reduce_fwd_ybuf(Handle *h, double scale, bool weights, const void *y_data);
 // Load a user signal (complex [c][s]) into ybuf (coil fastest), optionally weighted.
 hipError_t launch_load_ybuf(Handle *h, const void *y_user, bool weights);
 // Adjoint contraction from ybuf -> part_a, then coil reduction into xm (masked) with scale.
-hipError_t run_adjoint_contract(Handle *h, float scale, void *xm_out, bool time_it);
+hipError_t run_adjoint_contract(Handle *h, double scale, void *xm_out, bool time_it);
 // Normal operator on masked vectors: out_m = scale * E^H D E x_m (+ lambda x_m).
 // have_max: slot 0 already holds max|xm_in| and the other slots are clear (launch from a
 // gather that computed it).
-hipError_t run_normal_masked(Handle *h, const void *xm_in, void *xm_out, float scale, double lambda,
+hipError_t run_normal_masked(Handle *h, const void *xm_in, void *xm_out, double scale, double lambda,
                              bool time_it, bool have_max = false);
 // Gather of a full-grid x (with max|x| for the fragment scale) and the normal operator.
-hipError_t run_normal_full(Handle *h, const void *x_full, void *xm_out, float scale, bool time_it);
+hipError_t run_normal_full(Handle *h, const void *x_full, void *xm_out, double scale, bool time_it);
 // Gradient partial kernel: uses zbuf (csm .* x) and ybuf (residual, coil fastest) -> gk.
 hipError_t run_grad_contract(Handle *h, bool time_it);
 // zbuf = csm .* xm

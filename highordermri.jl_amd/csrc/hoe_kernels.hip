@@ -78,6 +78,36 @@ __device__ __forceinline__ void sincos_cycles(double ph, float &sn, float &cs) {
     cs = __builtin_amdgcn_cosf(r);
 }
 
+// exp(2 pi i ph) in FP64 (the FP64 backend): ph - rint(ph) in [-1/2, 1/2] is exact, then
+// the device math library's sincospi of twice it.
+__device__ __forceinline__ void sincos_cycles(double ph, double &sn, double &cs) {
+    const double r = ph - rint(ph);
+    sincospi(2.0 * r, &sn, &cs);
+}
+
+// Interleaved complex type of a real type (FP32 backends: float2; FP64 backend: double2).
+template <typename R>
+struct cplx;
+template <>
+struct cplx<float> {
+    typedef float2 type;
+};
+template <>
+struct cplx<double> {
+    typedef double2 type;
+};
+template <typename R>
+using cplx_t = typename cplx<R>::type;
+template <typename R>
+__device__ __forceinline__ cplx_t<R> mkc(R re, R im) {
+    cplx_t<R> c;
+    c.x = re;
+    c.y = im;
+    return c;
+}
+__device__ __forceinline__ float fmaT(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmaT(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 // Power-of-two scale that maps |values| <= maxabs below 2^12 (FP16 hi/lo
 // split without overflow and without losing the low part to underflow).
 __device__ __forceinline__ float pow2_scale(float maxabs) {
@@ -182,13 +212,14 @@ struct mfma_t<__bf16> {
 // array_operator.jl:187-235, e(s,v) replaced by 2 pi i psi(s,v) e(s,v) with
 // psi = sum_l kdt_l(s) b_l(v) from the derivative records dtrec (same layout
 // as coef, t slot 0), evaluated like the phase.
-template <bool FWD, bool HX, int L, int CB, bool DT = false>
+template <bool FWD, bool HX, int L, int CB, bool DT = false, typename R = float>
 __global__ __launch_bounds__(256) void k_direct(
     const double *__restrict__ coef, int sk, const double *__restrict__ basis, int sb,
-    const float2 *__restrict__ src, float2 *__restrict__ part, int ncs, int n_stream,
+    const cplx_t<R> *__restrict__ src, cplx_t<R> *__restrict__ part, int ncs, int n_stream,
     const int4 *__restrict__ segs, const int *__restrict__ chunk_off, int cpx, HornerArgs hx,
     const double *__restrict__ dtrec) {
     static_assert(FWD || !DT, "the derivative product is a forward product");
+    static_assert(!DT || std::is_same<R, float>::value, "the derivative product runs in FP32");
     const int chunk = chunk_id(cpx);
     const int cb0 = blockIdx.y * CB;
     const int sg0 = chunk_off[chunk], sg1 = chunk_off[chunk + 1];
@@ -209,9 +240,9 @@ __global__ __launch_bounds__(256) void k_direct(
                 for (int i = 0; i < NSD; ++i) sd[i] = dtrec[(size_t)row * sk + i];
             }
         }
-        float ar[CB], ai[CB];
+        R ar[CB], ai[CB];
 #pragma unroll
-        for (int c = 0; c < CB; ++c) ar[c] = ai[c] = 0.0f;
+        for (int c = 0; c < CB; ++c) ar[c] = ai[c] = R(0);
         const int q0 = d.y * 8;
         const int q1 = min(d.z * 8, n_stream);
         double hc[4] = {0.0, 0.0, 0.0, 0.0}, hd[4] = {0.0, 0.0, 0.0, 0.0};
@@ -245,29 +276,29 @@ __global__ __launch_bounds__(256) void k_direct(
                 for (int i = 1; i < L; ++i) ph = fma(sm[i], st[i - 1], ph);
                 ph += sm[0];
             }
-            float sn, cs;
+            R sn, cs;
             sincos_cycles(ph, sn, cs);
             if (!FWD) sn = -sn;   // conj(e)
-            if (DT) {
+            if constexpr (DT) {
                 // 2 pi i psi e = 2 pi psi (-sin + i cos)
                 const float w = (float)(6.283185307179586 * psi);
                 const float c2 = -sn * w;
                 sn = cs * w;
                 cs = c2;
             }
-            const float2 *w = src + (size_t)q * ncs + cb0;
+            const cplx_t<R> *w = src + (size_t)q * ncs + cb0;
 #pragma unroll
             for (int c = 0; c < CB; ++c) {
-                const float2 u = w[c];
-                ar[c] = fmaf(cs, u.x, ar[c]);
-                ar[c] = fmaf(-sn, u.y, ar[c]);
-                ai[c] = fmaf(cs, u.y, ai[c]);
-                ai[c] = fmaf(sn, u.x, ai[c]);
+                const cplx_t<R> u = w[c];
+                ar[c] = fmaT(cs, u.x, ar[c]);
+                ar[c] = fmaT(-sn, u.y, ar[c]);
+                ai[c] = fmaT(cs, u.y, ai[c]);
+                ai[c] = fmaT(sn, u.x, ai[c]);
             }
         }
-        float2 *o = part + ((size_t)sgi * 256 + threadIdx.x) * ncs + cb0;
+        cplx_t<R> *o = part + ((size_t)sgi * 256 + threadIdx.x) * ncs + cb0;
 #pragma unroll
-        for (int c = 0; c < CB; ++c) o[c] = make_float2(ar[c], ai[c]);
+        for (int c = 0; c < CB; ++c) o[c] = mkc<R>(ar[c], ai[c]);
     }
 }
 
@@ -998,16 +1029,17 @@ __device__ __forceinline__ void wave_absmax_to(unsigned *dst, float m) {
         atomicMax(dst, __float_as_uint(m));
 }
 
-// xm[v] = x_full[mask_idx[v]] (zero for padded voxels); optional |.|max.
-__global__ void k_gather(const float2 *__restrict__ x, const int *__restrict__ idx, float2 *__restrict__ xm,
+// xm[v] = x_full[mask_idx[v]] (zero for padded voxels); optional |.|max (FP32).
+template <typename R>
+__global__ void k_gather(const cplx_t<R> *__restrict__ x, const int *__restrict__ idx, cplx_t<R> *__restrict__ xm,
                          int nv, int nv_pad, unsigned *maxslot) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     float m = 0.0f;
     if (v < nv_pad) {
-        float2 val = make_float2(0.0f, 0.0f);
+        cplx_t<R> val = mkc<R>(R(0), R(0));
         if (v < nv) val = x[idx[v]];
         xm[v] = val;
-        m = fmaxf(fabsf(val.x), fabsf(val.y));
+        m = fmaxf(fabsf((float)val.x), fabsf((float)val.y));
     }
     if (maxslot) wave_absmax_to(maxslot, m);
 }
@@ -1023,28 +1055,31 @@ __global__ void k_absmax2(const float2 *__restrict__ a, int n, unsigned *maxslot
     wave_absmax_to(maxslot, m);
 }
 
-__global__ void k_scatter(const float2 *__restrict__ xm, const int *__restrict__ idx, float2 *__restrict__ x,
-                          int nv, float scale) {
+template <typename R>
+__global__ void k_scatter(const cplx_t<R> *__restrict__ xm, const int *__restrict__ idx, cplx_t<R> *__restrict__ x,
+                          int nv, R scale) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v < nv) {
-        const float2 t = xm[v];
-        x[idx[v]] = make_float2(t.x * scale, t.y * scale);
+        const cplx_t<R> t = xm[v];
+        x[idx[v]] = mkc<R>(t.x * scale, t.y * scale);
     }
 }
 
-__global__ void k_fill0(float2 *p, int64_t n) {
+template <typename R>
+__global__ void k_fill0(cplx_t<R> *p, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = make_float2(0.0f, 0.0f);
+    if (i < n) p[i] = mkc<R>(R(0), R(0));
 }
 
 // z[v][c] = csm[v][c] * xm[v] (coil stride ncs; zero for padded coils/voxels).
-__global__ void k_build_z(const float2 *__restrict__ csm, const float2 *__restrict__ xm, float2 *__restrict__ z,
-                          int nv_pad, int ncs) {
+template <typename R>
+__global__ void k_build_z(const cplx_t<R> *__restrict__ csm, const cplx_t<R> *__restrict__ xm,
+                          cplx_t<R> *__restrict__ z, int nv_pad, int ncs) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (int64_t)nv_pad * ncs) return;
     const int v = (int)(i / ncs);
-    const float2 a = csm[i], b = xm[v];
-    z[i] = make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+    const cplx_t<R> a = csm[i], b = xm[v];
+    z[i] = mkc<R>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
 // MFMA B fragments of z = csm .* xm for the forward (K = voxels):
@@ -1119,61 +1154,65 @@ __global__ void k_build_yfrag(const float2 *__restrict__ y, uint4 *__restrict__ 
 //         zero for padded rows/coils, optional |.|max into maxslot.
 // The effective scale is scale / pow2_scale(max from scale_slot * scale_mul)
 // when scale_slot is given (undoing the fragment scaling of the mixed path).
-__device__ __forceinline__ float2 sum_segments(const float2 *__restrict__ part, const int *__restrict__ list,
-                                               int n, int r, int c, int ncs) {
-    float2 a[4];
+template <typename R>
+__device__ __forceinline__ cplx_t<R> sum_segments(const cplx_t<R> *__restrict__ part, const int *__restrict__ list,
+                                                  int n, int r, int c, int ncs) {
+    cplx_t<R> a[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) a[q] = make_float2(0.0f, 0.0f);
+    for (int q = 0; q < 4; ++q) a[q] = mkc<R>(R(0), R(0));
     int i = 0;
     for (; i + 3 < n; i += 4) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const float2 t = part[((size_t)list[i + q] * 256 + r) * ncs + c];
+            const cplx_t<R> t = part[((size_t)list[i + q] * 256 + r) * ncs + c];
             a[q].x += t.x;
             a[q].y += t.y;
         }
     }
     for (; i < n; ++i) {
-        const float2 t = part[((size_t)list[i] * 256 + r) * ncs + c];
+        const cplx_t<R> t = part[((size_t)list[i] * 256 + r) * ncs + c];
         a[0].x += t.x;
         a[0].y += t.y;
     }
-    return make_float2((a[0].x + a[1].x) + (a[2].x + a[3].x), (a[0].y + a[1].y) + (a[2].y + a[3].y));
+    return mkc<R>((a[0].x + a[1].x) + (a[2].x + a[3].x), (a[0].y + a[1].y) + (a[2].y + a[3].y));
 }
 
+template <typename R>
 __global__ __launch_bounds__(256) void k_reduce_fwd(
-    const float2 *__restrict__ part, const int *__restrict__ tile_off, const int *__restrict__ tile_list, int ncs,
-    int ns, int nc, float scale, const unsigned *__restrict__ scale_slot, float scale_mul, int mode,
-    float2 *__restrict__ out, const float *__restrict__ w, const float2 *__restrict__ y_data, unsigned *maxslot) {
-    __shared__ float2 tileb[32][33];
+    const cplx_t<R> *__restrict__ part, const int *__restrict__ tile_off, const int *__restrict__ tile_list, int ncs,
+    int ns, int nc, R scale, const unsigned *__restrict__ scale_slot, float scale_mul, int mode,
+    cplx_t<R> *__restrict__ out, const R *__restrict__ w, const cplx_t<R> *__restrict__ y_data, unsigned *maxslot) {
+    __shared__ cplx_t<R> tileb[32][33];
     const int s0 = blockIdx.x * 32, c0 = blockIdx.y * 32;
     const int tc = threadIdx.x & 31, tr = threadIdx.x >> 5;   // 8 row groups
     const int tile = s0 >> 8;
     const int *list = tile_list + tile_off[tile];
     const int nseg = tile_off[tile + 1] - tile_off[tile];
-    float sc = scale;
-    if (scale_slot) sc = scale / pow2_scale(__uint_as_float(*scale_slot) * scale_mul);
+    R sc = scale;
+    if constexpr (std::is_same<R, float>::value) {
+        if (scale_slot) sc = scale / pow2_scale(__uint_as_float(*scale_slot) * scale_mul);
+    }
     if (mode == 1 && y_data) {
         // stage the data tile transposed: y_data[c*ns + s]
         for (int k = tr; k < 32; k += 8) {
             const int c = c0 + k, s = s0 + tc;
-            tileb[k][tc] = (c < nc && s < ns) ? y_data[(size_t)c * ns + s] : make_float2(0.0f, 0.0f);
+            tileb[k][tc] = (c < nc && s < ns) ? y_data[(size_t)c * ns + s] : mkc<R>(R(0), R(0));
         }
         __syncthreads();
     }
     // four rows per thread and two segments per step, in separate sums, so that 8
     // loads are in flight
-    float2 rows[4], rows2[4];
+    cplx_t<R> rows[4], rows2[4];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) rows[q] = rows2[q] = make_float2(0.0f, 0.0f);
+    for (int q = 0; q < 4; ++q) rows[q] = rows2[q] = mkc<R>(R(0), R(0));
     int j = 0;
     for (; j + 2 <= nseg; j += 2) {
         const size_t b0 = (size_t)list[j] * 256, b1 = (size_t)list[j + 1] * 256;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int r = (s0 + tr + 8 * q) & 255;
-            const float2 t0 = part[(b0 + r) * ncs + c0 + tc];
-            const float2 t1 = part[(b1 + r) * ncs + c0 + tc];
+            const cplx_t<R> t0 = part[(b0 + r) * ncs + c0 + tc];
+            const cplx_t<R> t1 = part[(b1 + r) * ncs + c0 + tc];
             rows[q].x += t0.x;
             rows[q].y += t0.y;
             rows2[q].x += t1.x;
@@ -1184,7 +1223,7 @@ __global__ __launch_bounds__(256) void k_reduce_fwd(
         const size_t b0 = (size_t)list[j] * 256;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const float2 t = part[(b0 + ((s0 + tr + 8 * q) & 255)) * ncs + c0 + tc];
+            const cplx_t<R> t = part[(b0 + ((s0 + tr + 8 * q) & 255)) * ncs + c0 + tc];
             rows[q].x += t.x;
             rows[q].y += t.y;
         }
@@ -1199,7 +1238,7 @@ __global__ __launch_bounds__(256) void k_reduce_fwd(
     for (int q = 0; q < 4; ++q) {
         const int k = tr + 8 * q;
         const int s = s0 + k, c = c0 + tc;
-        float2 acc = rows[q];
+        cplx_t<R> acc = rows[q];
         acc.x *= sc;
         acc.y *= sc;
         if (mode == 0) {
@@ -1207,18 +1246,18 @@ __global__ __launch_bounds__(256) void k_reduce_fwd(
         } else {
             const bool valid = (s < ns) && (c < nc);
             if (y_data) {
-                const float2 d = tileb[tc][k];
+                const cplx_t<R> d = tileb[tc][k];
                 acc.x -= d.x;
                 acc.y -= d.y;
             }
             if (w) {
-                const float ww = w[(size_t)s * ncs + c];
+                const R ww = w[(size_t)s * ncs + c];
                 acc.x *= ww;
                 acc.y *= ww;
             }
-            if (!valid) acc = make_float2(0.0f, 0.0f);
+            if (!valid) acc = mkc<R>(R(0), R(0));
             out[(size_t)s * ncs + c] = acc;
-            m = fmaxf(m, fmaxf(fabsf(acc.x), fabsf(acc.y)));
+            m = fmaxf(m, fmaxf(fabsf((float)acc.x), fabsf((float)acc.y)));
         }
     }
     if (mode == 0) {
@@ -1233,52 +1272,58 @@ __global__ __launch_bounds__(256) void k_reduce_fwd(
 }
 
 // ybuf[s][c] = w[s][c] * y_user[c*ns + s] (zero padded), optional |.|max.
-__global__ __launch_bounds__(256) void k_load_ybuf(const float2 *__restrict__ y, const float *__restrict__ w,
-                                                   float2 *__restrict__ out, int ns, int nc, int ns_pad, int ncs,
+template <typename R>
+__global__ __launch_bounds__(256) void k_load_ybuf(const cplx_t<R> *__restrict__ y, const R *__restrict__ w,
+                                                   cplx_t<R> *__restrict__ out, int ns, int nc, int ns_pad, int ncs,
                                                    unsigned *maxslot) {
-    __shared__ float2 tileb[32][33];
+    __shared__ cplx_t<R> tileb[32][33];
     const int s0 = blockIdx.x * 32, c0 = blockIdx.y * 32;
     const int tc = threadIdx.x & 31, tr = threadIdx.x >> 5;
     for (int k = tr; k < 32; k += 8) {
         const int c = c0 + k, s = s0 + tc;
-        tileb[k][tc] = (c < nc && s < ns) ? y[(size_t)c * ns + s] : make_float2(0.0f, 0.0f);
+        tileb[k][tc] = (c < nc && s < ns) ? y[(size_t)c * ns + s] : mkc<R>(R(0), R(0));
     }
     __syncthreads();
     float m = 0.0f;
     for (int k = tr; k < 32; k += 8) {
         const int s = s0 + k, c = c0 + tc;
-        float2 v = tileb[tc][k];
+        cplx_t<R> v = tileb[tc][k];
         if (w) {
-            const float ww = w[(size_t)s * ncs + c];
+            const R ww = w[(size_t)s * ncs + c];
             v.x *= ww;
             v.y *= ww;
         }
         out[(size_t)s * ncs + c] = v;
-        m = fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y)));
+        m = fmaxf(m, fmaxf(fabsf((float)v.x), fabsf((float)v.y)));
     }
     if (maxslot) wave_absmax_to(maxslot, m);
 }
 
-// Adjoint reduction: xm_out[v] = sc * sum_c conj(csm[v][c]) * sum_segments part[seg][v%256][c]
+// Adjoint reduction: xm_out[v] = sc * sum_{c < nc} conj(csm[v][c]) * sum_segments part[seg][v%256][c]
 //                               (+ lambda * x_in[v]).
-// Half-wave per voxel: lanes over coils, shuffle reduction over 32 lanes.
+// Half-wave per voxel: lanes over coils, shuffle reduction over 32 lanes. Only the nc real
+// coils are read: the FP32/FP64 direct kernels write partial sums for their coil batches
+// only, so the padding columns beyond them hold no data.
+template <typename R>
 __global__ __launch_bounds__(256) void k_reduce_adj(
-    const float2 *__restrict__ part, const int *__restrict__ tile_off, const int *__restrict__ tile_list,
-    int nv_pad, int ncs, const float2 *__restrict__ csm, int nv, float scale,
-    const unsigned *__restrict__ scale_slot, float2 *__restrict__ xm_out, float lambda,
-    const float2 *__restrict__ x_in) {
+    const cplx_t<R> *__restrict__ part, const int *__restrict__ tile_off, const int *__restrict__ tile_list,
+    int nv_pad, int ncs, int nc, const cplx_t<R> *__restrict__ csm, int nv, R scale,
+    const unsigned *__restrict__ scale_slot, cplx_t<R> *__restrict__ xm_out, R lambda,
+    const cplx_t<R> *__restrict__ x_in) {
     const int v = (blockIdx.x * blockDim.x + threadIdx.x) >> 5;   // half-wave id = voxel
     const int cl = threadIdx.x & 31;
-    float sc = scale;
-    if (scale_slot) sc = scale / pow2_scale(__uint_as_float(*scale_slot));
-    float ar = 0.0f, ai = 0.0f;
+    R sc = scale;
+    if constexpr (std::is_same<R, float>::value) {
+        if (scale_slot) sc = scale / pow2_scale(__uint_as_float(*scale_slot));
+    }
+    R ar = R(0), ai = R(0);
     if (v < nv_pad) {
         const int tile = v >> 8;
         const int *list = tile_list + tile_off[tile];
         const int nseg = tile_off[tile + 1] - tile_off[tile];
-        for (int c = cl; c < ncs; c += 32) {
-            const float2 acc = sum_segments(part, list, nseg, v & 255, c, ncs);
-            const float2 s = csm[(size_t)v * ncs + c];
+        for (int c = cl; c < nc; c += 32) {
+            const cplx_t<R> acc = sum_segments<R>(part, list, nseg, v & 255, c, ncs);
+            const cplx_t<R> s = csm[(size_t)v * ncs + c];
             ar += s.x * acc.x + s.y * acc.y;   // conj(s) * acc
             ai += s.x * acc.y - s.y * acc.x;
         }
@@ -1288,10 +1333,10 @@ __global__ __launch_bounds__(256) void k_reduce_adj(
         ai += __shfl_xor(ai, o, 32);
     }
     if (cl == 0 && v < nv_pad) {
-        float2 r = make_float2(ar * sc, ai * sc);
-        if (v >= nv) r = make_float2(0.0f, 0.0f);
+        cplx_t<R> r = mkc<R>(ar * sc, ai * sc);
+        if (v >= nv) r = mkc<R>(R(0), R(0));
         if (x_in && v < nv) {
-            const float2 t = x_in[v];
+            const cplx_t<R> t = x_in[v];
             r.x += lambda * t.x;
             r.y += lambda * t.y;
         }
@@ -1407,13 +1452,14 @@ __global__ __launch_bounds__(256) void k_gtau_partial(const double *__restrict__
 // ---- FP64 reductions for CG and losses ----
 
 // block partial sums of Re<a,b> (b == nullptr: |a|^2)
-__global__ __launch_bounds__(256) void k_dot_partial(const float2 *__restrict__ a, const float2 *__restrict__ b,
+template <typename R>
+__global__ __launch_bounds__(256) void k_dot_partial(const cplx_t<R> *__restrict__ a, const cplx_t<R> *__restrict__ b,
                                                      int64_t n, double *__restrict__ red) {
     __shared__ double sh[4];
     double acc = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const float2 x = a[i];
-        const float2 y = b ? b[i] : x;
+        const cplx_t<R> x = a[i];
+        const cplx_t<R> y = b ? b[i] : x;
         acc += (double)x.x * (double)y.x + (double)x.y * (double)y.y;
     }
     for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
@@ -1467,22 +1513,23 @@ __global__ __launch_bounds__(256) void k_cg_scalar(const double *__restrict__ re
 }
 
 // x += alpha p; r -= alpha q; partial |r|^2 (skipped when done)
-__global__ __launch_bounds__(256) void k_cg_update(float2 *__restrict__ x, float2 *__restrict__ r,
-                                                   const float2 *__restrict__ p, const float2 *__restrict__ q,
+template <typename R>
+__global__ __launch_bounds__(256) void k_cg_update(cplx_t<R> *__restrict__ x, cplx_t<R> *__restrict__ r,
+                                                   const cplx_t<R> *__restrict__ p, const cplx_t<R> *__restrict__ q,
                                                    int64_t n, const double *__restrict__ cg, double *__restrict__ red) {
     __shared__ double sh[4];
     const bool done = cg[5] != 0.0;
-    const float alpha = (float)cg[3];
+    const R alpha = (R)cg[3];
     double acc = 0.0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        float2 rv = r[i];
+        cplx_t<R> rv = r[i];
         if (!done) {
-            const float2 pv = p[i], qv = q[i];
-            float2 xv = x[i];
-            xv.x = fmaf(alpha, pv.x, xv.x);
-            xv.y = fmaf(alpha, pv.y, xv.y);
-            rv.x = fmaf(-alpha, qv.x, rv.x);
-            rv.y = fmaf(-alpha, qv.y, rv.y);
+            const cplx_t<R> pv = p[i], qv = q[i];
+            cplx_t<R> xv = x[i];
+            xv.x = fmaT(alpha, pv.x, xv.x);
+            xv.y = fmaT(alpha, pv.y, xv.y);
+            rv.x = fmaT(-alpha, qv.x, rv.x);
+            rv.y = fmaT(-alpha, qv.y, rv.y);
             x[i] = xv;
             r[i] = rv;
         }
@@ -1495,18 +1542,20 @@ __global__ __launch_bounds__(256) void k_cg_update(float2 *__restrict__ x, float
 }
 
 // p = r + beta p (skipped when done)
-__global__ void k_cg_pupdate(float2 *__restrict__ p, const float2 *__restrict__ r, int64_t n,
+template <typename R>
+__global__ void k_cg_pupdate(cplx_t<R> *__restrict__ p, const cplx_t<R> *__restrict__ r, int64_t n,
                              const double *__restrict__ cg) {
     if (cg[5] != 0.0) return;
-    const float beta = (float)cg[4];
+    const R beta = (R)cg[4];
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
-        const float2 rv = r[i], pv = p[i];
-        p[i] = make_float2(fmaf(beta, pv.x, rv.x), fmaf(beta, pv.y, rv.y));
+        const cplx_t<R> rv = r[i], pv = p[i];
+        p[i] = mkc<R>(fmaT(beta, pv.x, rv.x), fmaT(beta, pv.y, rv.y));
     }
 }
 
-__global__ void k_copy2(float2 *__restrict__ d, const float2 *__restrict__ s, int64_t n) {
+template <typename R>
+__global__ void k_copy2(cplx_t<R> *__restrict__ d, const cplx_t<R> *__restrict__ s, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) d[i] = s[i];
 }
@@ -1525,45 +1574,62 @@ static inline int blocks_for(int64_t n, int b = 256) { return (int)((n + b - 1) 
 
 static void zero_slots(Handle *h) { k_zero_u32<<<1, 64, 0, h->stream>>>(slot(h, 0), 64); }
 
+// f(R{}) with R the real type of the handle's vectors: double for the FP64 backend.
+template <class F>
+static void by_prec(const Handle *h, F &&f) {
+    if (h->f64) f(double{});
+    else f(float{});
+}
+
 hipError_t launch_gather(Handle *h, const void *x_full, void *xm) {
-    k_gather<<<blocks_for(h->nv_pad), 256, 0, h->stream>>>((const float2 *)x_full, (const int *)h->mask_idx.p,
-                                                           (float2 *)xm, (int)h->n_vox, (int)h->nv_pad, nullptr);
+    by_prec(h, [&](auto r) {
+        using R = decltype(r);
+        k_gather<R><<<blocks_for(h->nv_pad), 256, 0, h->stream>>>(
+            (const cplx_t<R> *)x_full, (const int *)h->mask_idx.p, (cplx_t<R> *)xm, (int)h->n_vox, (int)h->nv_pad,
+            nullptr);
+    });
     return hipGetLastError();
 }
 
 // Gather that also leaves max|xm| in slot 0 (all slots cleared first): the
 // fragment scale of the mixed forward, without a separate pass over xm.
 static hipError_t launch_gather_max(Handle *h, const void *x_full, void *xm) {
+    if (h->f64) return launch_gather(h, x_full, xm);
     zero_slots(h);
-    k_gather<<<blocks_for(h->nv_pad), 256, 0, h->stream>>>((const float2 *)x_full, (const int *)h->mask_idx.p,
-                                                           (float2 *)xm, (int)h->n_vox, (int)h->nv_pad, slot(h, 0));
+    k_gather<float><<<blocks_for(h->nv_pad), 256, 0, h->stream>>>(
+        (const float2 *)x_full, (const int *)h->mask_idx.p, (float2 *)xm, (int)h->n_vox, (int)h->nv_pad, slot(h, 0));
     return hipGetLastError();
 }
 
-hipError_t launch_scatter(Handle *h, const void *xm, void *x_full, float scale) {
-    // without a mask every grid entry is written below
-    if (h->n_vox != h->n_grid) k_fill0<<<blocks_for(h->n_grid), 256, 0, h->stream>>>((float2 *)x_full, h->n_grid);
-    k_scatter<<<blocks_for(h->n_vox), 256, 0, h->stream>>>((const float2 *)xm, (const int *)h->mask_idx.p,
-                                                           (float2 *)x_full, (int)h->n_vox, scale);
+hipError_t launch_scatter(Handle *h, const void *xm, void *x_full, double scale) {
+    by_prec(h, [&](auto r) {
+        using R = decltype(r);
+        // without a mask every grid entry is written below
+        if (h->n_vox != h->n_grid)
+            k_fill0<R><<<blocks_for(h->n_grid), 256, 0, h->stream>>>((cplx_t<R> *)x_full, h->n_grid);
+        k_scatter<R><<<blocks_for(h->n_vox), 256, 0, h->stream>>>(
+            (const cplx_t<R> *)xm, (const int *)h->mask_idx.p, (cplx_t<R> *)x_full, (int)h->n_vox, (R)scale);
+    });
     return hipGetLastError();
 }
 
 // ---- kernel instances over (L, coil batch): visitors hand the instance to a launcher ----
 
-template <bool FWD, bool HX, int L, bool DT, class F>
+template <bool FWD, bool HX, int L, bool DT, typename R, class F>
 static void visit_direct_cb(int cb, F &&f) {
-    if (cb == 8) f(k_direct<FWD, HX, L, 8, DT>);
-    else if (cb == 16) f(k_direct<FWD, HX, L, 16, DT>);
-    else f(k_direct<FWD, HX, L, 32, DT>);
+    if (cb == 8) f(k_direct<FWD, HX, L, 8, DT, R>);
+    else if (cb == 16) f(k_direct<FWD, HX, L, 16, DT, R>);
+    else if constexpr (std::is_same<R, float>::value) f(k_direct<FWD, HX, L, 32, DT, R>);
 }
-template <bool FWD, bool HX, bool DT = false, class F>
+// FP64 instances batch at most 16 coils (direct_cb), FP32 ones up to 32.
+template <bool FWD, bool HX, bool DT = false, typename R = float, class F>
 static void visit_direct(int L, int cb, F &&f) {
     switch (L) {
-        case 6: visit_direct_cb<FWD, HX, 6, DT>(cb, f); break;
-        case 10: visit_direct_cb<FWD, HX, 10, DT>(cb, f); break;
-        case 12: visit_direct_cb<FWD, HX, 12, DT>(cb, f); break;
-        case 16: visit_direct_cb<FWD, HX, 16, DT>(cb, f); break;
-        default: visit_direct_cb<FWD, HX, 20, DT>(cb, f); break;
+        case 6: visit_direct_cb<FWD, HX, 6, DT, R>(cb, f); break;
+        case 10: visit_direct_cb<FWD, HX, 10, DT, R>(cb, f); break;
+        case 12: visit_direct_cb<FWD, HX, 12, DT, R>(cb, f); break;
+        case 16: visit_direct_cb<FWD, HX, 16, DT, R>(cb, f); break;
+        default: visit_direct_cb<FWD, HX, 20, DT, R>(cb, f); break;
     }
 }
 template <typename T16, bool FWD, bool HX, int L, class F>
@@ -1628,6 +1694,11 @@ static void visit_gmixed(Handle *h, F &&f) {
 // the Horner variant when the handle's runs are long enough).
 template <bool FWD, class F>
 static void visit_contract_direct(Handle *h, F &&f) {
+    if (h->f64) {
+        if (FWD && h->horner) visit_direct<FWD, true, false, double>(h->LK, direct_cb(h), f);
+        else visit_direct<FWD, false, false, double>(h->LK, direct_cb(h), f);
+        return;
+    }
     if constexpr (FWD) {
         if (h->dt_on) {
             if (h->horner) visit_direct<true, true, true>(h->LK, direct_cb(h), f);
@@ -1663,12 +1734,15 @@ static void visit_contract_mixed(Handle *h, int backend, F &&f) {
 }
 template <bool FWD, class F>
 static void visit_contract(Handle *h, int backend, F &&f) {
-    if (backend == HOE_BACKEND_DIRECT) visit_contract_direct<FWD>(h, f);
+    if (backend == HOE_BACKEND_DIRECT || backend == HOE_BACKEND_FP64) visit_contract_direct<FWD>(h, f);
     else visit_contract_mixed<FWD>(h, backend, f);
 }
 
 // ---- coil batching ----
-int direct_cb(const Handle *h) { return h->n_cha <= 8 ? 8 : (h->n_cha <= 16 ? 16 : 32); }
+int direct_cb(const Handle *h) {
+    if (h->f64) return h->n_cha <= 8 ? 8 : 16;   // 4 FP64 registers per complex accumulator
+    return h->n_cha <= 8 ? 8 : (h->n_cha <= 16 ? 16 : 32);
+}
 int direct_batches(const Handle *h) { return (int)((h->n_cha + direct_cb(h) - 1) / direct_cb(h)); }
 int mixed_nb(const Handle *h) { return h->ncs / 32 >= 2 ? 2 : 1; }
 int mixed_groups(const Handle *h) { return (h->ncs / 32 + mixed_nb(h) - 1) / mixed_nb(h); }
@@ -1797,7 +1871,8 @@ static int blocks_per_cu(K kern) {
 hipError_t prepare_backend(Handle *h, int backend) {
     Sched &sf = h->sched_f[backend], &sa = h->sched_a[backend];
     if (sf.built && sa.built) return hipSuccess;
-    const int groups = backend == HOE_BACKEND_DIRECT ? direct_batches(h) : mixed_groups(h);
+    const bool is_mixed = backend == HOE_BACKEND_MIXED_FP16 || backend == HOE_BACKEND_MIXED_BF16;
+    const int groups = is_mixed ? mixed_groups(h) : direct_batches(h);
     int occ_f = 1, occ_a = 1;
     visit_contract<true>(h, backend, [&](auto k) { occ_f = blocks_per_cu(k); });
     visit_contract<false>(h, backend, [&](auto k) { occ_a = blocks_per_cu(k); });
@@ -1807,7 +1882,7 @@ hipError_t prepare_backend(Handle *h, int backend) {
     // The cut adjoint segments cost partial-sum memory (about nVox nSam nCoil / 1024 bytes
     // with 1 024-unit segments); beyond 8 GiB the cut length doubles until it fits.
     int kmax_a = 0;
-    if (backend != HOE_BACKEND_DIRECT) {
+    if (is_mixed) {
         const int64_t tiles = h->nv_pad / 256, kx = (h->ns_pad / 8 + 7) / 8;
         const int64_t seg_bytes = (int64_t)256 * h->ncs * (int64_t)sizeof(float2);
         kmax_a = HOE_MFMA_ADJ_KMAX;
@@ -1819,15 +1894,15 @@ hipError_t prepare_backend(Handle *h, int backend) {
     if ((e = build_schedule(sf, (int)(h->ns_pad / 256), (int)(h->nv_pad / 8), slots_f, 0, nullptr,
                             fweighted ? &h->fwd_k_w : nullptr)) != hipSuccess)
         return e;
-    const bool weighted = backend != HOE_BACKEND_DIRECT &&
+    const bool weighted = is_mixed &&
                           std::any_of(h->adj_tile_w.begin(), h->adj_tile_w.end(), [](double w) { return w != 1.0; });
     if ((e = build_schedule(sa, (int)(h->nv_pad / 256), (int)(h->ns_pad / 8), slots_a, kmax_a,
                             weighted ? &h->adj_tile_w : nullptr)) != hipSuccess)
         return e;
     h->occ_f[backend] = occ_f;
     h->occ_a[backend] = occ_a;
-    if ((e = grow(h->part_f, (size_t)sf.n_seg * 256 * h->ncs * sizeof(float2))) != hipSuccess) return e;
-    return grow(h->part_a, (size_t)sa.n_seg * 256 * h->ncs * sizeof(float2));
+    if ((e = grow(h->part_f, (size_t)sf.n_seg * 256 * h->ncs * h->cbytes)) != hipSuccess) return e;
+    return grow(h->part_a, (size_t)sa.n_seg * 256 * h->ncs * h->cbytes);
 }
 
 hipError_t prepare_grad(Handle *h) {
@@ -1886,41 +1961,63 @@ static hipError_t prepare_grad_mfma(Handle *h) {
 
 // ---- contraction launches ----
 
+// Launch of a k_direct instance; the complex type of its operand and partial sums follows
+// from the instance (float2 or double2).
+template <typename C>
+static void launch_direct(void (*k)(const double *, int, const double *, int, const C *, C *, int, int, const int4 *,
+                                    const int *, int, HornerArgs, const double *),
+                          dim3 grid, hipStream_t st, const double *coef, int sk, const double *basis, int sb,
+                          const void *operand, void *part, int ncs, int n_stream, const int4 *segs, const int *coff,
+                          int cpx, HornerArgs hx, const double *dtrec) {
+    k<<<grid, 256, 0, st>>>(coef, sk, basis, sb, (const C *)operand, (C *)part, ncs, n_stream, segs, coff, cpx, hx,
+                            dtrec);
+}
+
 template <bool FWD>
-static void launch_contract(Handle *h, const void *operand, float2 *part) {
+static void launch_contract(Handle *h, const void *operand, void *part) {
     const int backend = FWD ? fwd_backend(h) : h->backend;
     const Sched &sc = FWD ? h->sched_f[backend] : h->sched_a[backend];
     const double *coef = (const double *)h->coef.p, *basis = (const double *)h->basis.p;
     const int4 *segs = (const int4 *)sc.seg.p;
     const int *coff = (const int *)sc.chunk_off.p;
     HornerArgs hx{(const double2 *)h->hrec.p, (const int *)h->vrun.p, (const double *)h->beta.p, h->t_index};
-    if (backend == HOE_BACKEND_DIRECT) {
+    if (backend == HOE_BACKEND_DIRECT || backend == HOE_BACKEND_FP64) {
         dim3 grid(sc.n_chunks, direct_batches(h));
         const int n_stream = (int)(FWD ? h->n_vox : h->n_sam);
         visit_contract_direct<FWD>(h, [&](auto k) {
-            k<<<grid, 256, 0, h->stream>>>(coef, h->sk, basis, h->sb, (const float2 *)operand, part, h->ncs,
-                                           n_stream, segs, coff, sc.cpx, hx, (const double *)h->dtrec.p);
+            launch_direct(k, grid, h->stream, coef, h->sk, basis, h->sb, operand, part, h->ncs, n_stream, segs, coff,
+                          sc.cpx, hx, (const double *)h->dtrec.p);
         });
     } else {
         dim3 grid(sc.n_chunks, mixed_groups(h));
         visit_contract_mixed<FWD>(h, backend, [&](auto k) {
-            k<<<grid, 256, 0, h->stream>>>(coef, h->sk, basis, h->sb, (const uint4 *)operand, sc.n_ks, part,
-                                           h->ncs, segs, coff, sc.cpx, hx);
+            k<<<grid, 256, 0, h->stream>>>(coef, h->sk, basis, h->sb, (const uint4 *)operand, sc.n_ks,
+                                           (float2 *)part, h->ncs, segs, coff, sc.cpx, hx);
         });
     }
 }
 
+// The FP32 direct kernels serve the direct backend, the derivative product and (in FP64)
+// the FP64 backend: no MFMA fragments.
+static inline bool direct_path(int backend) {
+    return backend == HOE_BACKEND_DIRECT || backend == HOE_BACKEND_FP64;
+}
+
 // Builds the forward operand from masked xm (device) and runs the forward
 // contraction into part_f. have_max: slot 0 already holds max|xm| (gathered).
-static hipError_t forward_from_xm(Handle *h, const float2 *xm, bool time_it, bool have_max = false) {
-    if (fwd_backend(h) == HOE_BACKEND_DIRECT) {
+static hipError_t forward_from_xm(Handle *h, const void *xmv, bool time_it, bool have_max = false) {
+    if (direct_path(fwd_backend(h))) {
         const int64_t nz = h->nv_pad * h->ncs;
-        k_build_z<<<blocks_for(nz), 256, 0, h->stream>>>((const float2 *)h->csm.p, xm, (float2 *)h->zbuf.p,
-                                                          (int)h->nv_pad, h->ncs);
+        by_prec(h, [&](auto r) {
+            using R = decltype(r);
+            k_build_z<R><<<blocks_for(nz), 256, 0, h->stream>>>((const cplx_t<R> *)h->csm.p, (const cplx_t<R> *)xmv,
+                                                                (cplx_t<R> *)h->zbuf.p, (int)h->nv_pad, h->ncs);
+        });
         if (time_it) (void)hipEventRecord(h->ev[0], h->stream);
-        launch_contract<true>(h, h->zbuf.p, (float2 *)h->part_f.p);
+        launch_contract<true>(h, h->zbuf.p, h->part_f.p);
         if (time_it) (void)hipEventRecord(h->ev[1], h->stream);
     } else {
+        const float2 *xm = (const float2 *)xmv;
         const int n_ks = (int)(h->nv_pad / 8);
         const int n_gb = frag_groups(h);
         if (!have_max) {
@@ -1935,7 +2032,7 @@ static hipError_t forward_from_xm(Handle *h, const float2 *xm, bool time_it, boo
             k_build_zfrag<__bf16><<<blocks_for(nt), 256, 0, h->stream>>>(
                 (const float2 *)h->csm.p, xm, (uint4 *)h->zfrag.p, n_ks, h->ncs, n_gb, slot(h, 0), h->csm_max);
         if (time_it) (void)hipEventRecord(h->ev[0], h->stream);
-        launch_contract<true>(h, h->zfrag.p, (float2 *)h->part_f.p);
+        launch_contract<true>(h, h->zfrag.p, h->part_f.p);
         if (time_it) (void)hipEventRecord(h->ev[1], h->stream);
     }
     return hipGetLastError();
@@ -1944,53 +2041,62 @@ static hipError_t forward_from_xm(Handle *h, const float2 *xm, bool time_it, boo
 hipError_t run_forward_contract(Handle *h, const void *x_full, bool time_it) {
     hipError_t e = launch_gather_max(h, x_full, h->xm.p);
     if (e != hipSuccess) return e;
-    return forward_from_xm(h, (const float2 *)h->xm.p, time_it, true);
+    return forward_from_xm(h, h->xm.p, time_it, true);
 }
 
-hipError_t run_normal_full(Handle *h, const void *x_full, void *xm_out, float scale, bool time_it) {
+hipError_t run_normal_full(Handle *h, const void *x_full, void *xm_out, double scale, bool time_it) {
     hipError_t e = launch_gather_max(h, x_full, h->xm.p);
     if (e != hipSuccess) return e;
     return run_normal_masked(h, h->xm.p, xm_out, scale, 0.0, time_it, true);
 }
 
-static inline bool mixed(Handle *h) { return h->backend != HOE_BACKEND_DIRECT; }
+static inline bool mixed(Handle *h) { return !direct_path(h->backend); }
 
-hipError_t launch_reduce_fwd_user(Handle *h, void *y_out, float scale) {
+hipError_t launch_reduce_fwd_user(Handle *h, void *y_out, double scale) {
     const Sched &sc = h->sched_f[fwd_backend(h)];
-    const bool unscale = fwd_backend(h) != HOE_BACKEND_DIRECT;
+    const bool unscale = !direct_path(fwd_backend(h));
     dim3 grid((unsigned)(h->ns_pad / 32), (unsigned)(h->ncs / 32));
-    k_reduce_fwd<<<grid, 256, 0, h->stream>>>((const float2 *)h->part_f.p, (const int *)sc.tile_off.p,
-                                              (const int *)sc.tile_list.p, h->ncs, (int)h->n_sam, (int)h->n_cha,
-                                              scale, unscale ? slot(h, 0) : nullptr, h->csm_max, 0,
-                                              (float2 *)y_out, nullptr, nullptr, nullptr);
+    by_prec(h, [&](auto r) {
+        using R = decltype(r);
+        k_reduce_fwd<R><<<grid, 256, 0, h->stream>>>(
+            (const cplx_t<R> *)h->part_f.p, (const int *)sc.tile_off.p, (const int *)sc.tile_list.p, h->ncs,
+            (int)h->n_sam, (int)h->n_cha, (R)scale, unscale ? slot(h, 0) : nullptr, h->csm_max, 0,
+            (cplx_t<R> *)y_out, nullptr, nullptr, nullptr);
+    });
     return hipGetLastError();
 }
 
-hipError_t launch_reduce_fwd_ybuf(Handle *h, float scale, bool weights, const void *y_data) {
+hipError_t launch_reduce_fwd_ybuf(Handle *h, double scale, bool weights, const void *y_data) {
     const Sched &sc = h->sched_f[h->backend];
     dim3 grid((unsigned)(h->ns_pad / 32), (unsigned)(h->ncs / 32));
-    k_reduce_fwd<<<grid, 256, 0, h->stream>>>((const float2 *)h->part_f.p, (const int *)sc.tile_off.p,
-                                              (const int *)sc.tile_list.p, h->ncs, (int)h->n_sam, (int)h->n_cha,
-                                              scale, mixed(h) ? slot(h, 0) : nullptr, h->csm_max, 1,
-                                              (float2 *)h->ybuf.p, weights ? (const float *)h->dvec.p : nullptr,
-                                              (const float2 *)y_data, slot(h, 1));
+    by_prec(h, [&](auto r) {
+        using R = decltype(r);
+        k_reduce_fwd<R><<<grid, 256, 0, h->stream>>>(
+            (const cplx_t<R> *)h->part_f.p, (const int *)sc.tile_off.p, (const int *)sc.tile_list.p, h->ncs,
+            (int)h->n_sam, (int)h->n_cha, (R)scale, mixed(h) ? slot(h, 0) : nullptr, h->csm_max, 1,
+            (cplx_t<R> *)h->ybuf.p, weights ? (const R *)h->dvec.p : nullptr, (const cplx_t<R> *)y_data,
+            h->f64 ? nullptr : slot(h, 1));
+    });
     return hipGetLastError();
 }
 
 hipError_t launch_load_ybuf(Handle *h, const void *y_user, bool weights) {
-    zero_slots(h);
     dim3 grid((unsigned)(h->ns_pad / 32), (unsigned)(h->ncs / 32));
-    k_load_ybuf<<<grid, 256, 0, h->stream>>>((const float2 *)y_user, weights ? (const float *)h->dvec.p : nullptr,
-                                             (float2 *)h->ybuf.p, (int)h->n_sam, (int)h->n_cha, (int)h->ns_pad,
-                                             h->ncs, slot(h, 1));
+    if (!h->f64) zero_slots(h);
+    by_prec(h, [&](auto r) {
+        using R = decltype(r);
+        k_load_ybuf<R><<<grid, 256, 0, h->stream>>>((const cplx_t<R> *)y_user, weights ? (const R *)h->dvec.p : nullptr,
+                                                     (cplx_t<R> *)h->ybuf.p, (int)h->n_sam, (int)h->n_cha,
+                                                     (int)h->ns_pad, h->ncs, h->f64 ? nullptr : slot(h, 1));
+    });
     return hipGetLastError();
 }
 
 // Adjoint contraction from ybuf (slot 1 must hold |ybuf|max for the mixed path).
 static hipError_t adjoint_contract_only(Handle *h, bool time_it) {
-    if (h->backend == HOE_BACKEND_DIRECT) {
+    if (direct_path(h->backend)) {
         if (time_it) (void)hipEventRecord(h->ev[2], h->stream);
-        launch_contract<false>(h, h->ybuf.p, (float2 *)h->part_a.p);
+        launch_contract<false>(h, h->ybuf.p, h->part_a.p);
         if (time_it) (void)hipEventRecord(h->ev[3], h->stream);
     } else {
         const int n_ks = (int)(h->ns_pad / 8);
@@ -2005,39 +2111,45 @@ static hipError_t adjoint_contract_only(Handle *h, bool time_it) {
                                                                           (uint4 *)h->yfrag.p, n_ks, h->ncs, n_gb,
                                                                           slot(h, 1));
         if (time_it) (void)hipEventRecord(h->ev[2], h->stream);
-        launch_contract<false>(h, h->yfrag.p, (float2 *)h->part_a.p);
+        launch_contract<false>(h, h->yfrag.p, h->part_a.p);
         if (time_it) (void)hipEventRecord(h->ev[3], h->stream);
     }
     return hipGetLastError();
 }
 
-static hipError_t adjoint_reduce(Handle *h, float scale, void *xm_out, float lambda, const float2 *x_in) {
+static hipError_t adjoint_reduce(Handle *h, double scale, void *xm_out, double lambda, const void *x_in) {
     const Sched &sc = h->sched_a[h->backend];
     const int64_t threads = h->nv_pad * 32;
-    k_reduce_adj<<<blocks_for(threads), 256, 0, h->stream>>>(
-        (const float2 *)h->part_a.p, (const int *)sc.tile_off.p, (const int *)sc.tile_list.p, (int)h->nv_pad,
-        h->ncs, (const float2 *)h->csm.p, (int)h->n_vox, scale, mixed(h) ? slot(h, 1) : nullptr, (float2 *)xm_out,
-        lambda, x_in);
+    by_prec(h, [&](auto r) {
+        using R = decltype(r);
+        k_reduce_adj<R><<<blocks_for(threads), 256, 0, h->stream>>>(
+            (const cplx_t<R> *)h->part_a.p, (const int *)sc.tile_off.p, (const int *)sc.tile_list.p, (int)h->nv_pad,
+            h->ncs, (int)h->n_cha, (const cplx_t<R> *)h->csm.p, (int)h->n_vox, (R)scale,
+            mixed(h) ? slot(h, 1) : nullptr,
+            (cplx_t<R> *)xm_out, (R)lambda, (const cplx_t<R> *)x_in);
+    });
     return hipGetLastError();
 }
 
-hipError_t run_adjoint_contract(Handle *h, float scale, void *xm_out, bool time_it) {
+hipError_t run_adjoint_contract(Handle *h, double scale, void *xm_out, bool time_it) {
     hipError_t e = adjoint_contract_only(h, time_it);
     if (e != hipSuccess) return e;
     return adjoint_reduce(h, scale, xm_out, 0.0f, nullptr);
 }
 
-hipError_t run_normal_masked(Handle *h, const void *xm_in, void *xm_out, float scale, double lambda,
+hipError_t run_normal_masked(Handle *h, const void *xm_in, void *xm_out, double scale, double lambda,
                              bool time_it, bool have_max) {
     // slot 0 / slot 1 hold the fragment scales of the mixed path
-    if (!have_max) zero_slots(h);
-    hipError_t e = forward_from_xm(h, (const float2 *)xm_in, time_it, have_max);
+    if (!have_max && !h->f64) zero_slots(h);
+    hipError_t e = forward_from_xm(h, xm_in, time_it, have_max);
     if (e != hipSuccess) return e;
-    e = launch_reduce_fwd_ybuf(h, 1.0f, true, nullptr);
+    e = launch_reduce_fwd_ybuf(h, 1.0, true, nullptr);
     if (e != hipSuccess) return e;
     e = adjoint_contract_only(h, time_it);
     if (e != hipSuccess) return e;
-    return adjoint_reduce(h, scale, xm_out, (float)lambda, lambda != 0.0 ? (const float2 *)xm_in : nullptr);
+    // FP32 backends: lambda enters the reduction as float, as it always has
+    const double lam = h->f64 ? lambda : (double)(float)lambda;
+    return adjoint_reduce(h, scale, xm_out, lam, lambda != 0.0 ? xm_in : nullptr);
 }
 
 // Mixed backends with 32 coils: Z and residual fragments (slot 0 = max|xm|,
@@ -2105,8 +2217,8 @@ hipError_t run_grad_contract(Handle *h, bool time_it) {
 
 hipError_t launch_build_zbuf(Handle *h, const void *xm) {
     const int64_t nz = h->nv_pad * h->ncs;
-    k_build_z<<<blocks_for(nz), 256, 0, h->stream>>>((const float2 *)h->csm.p, (const float2 *)xm,
-                                                      (float2 *)h->zbuf.p, (int)h->nv_pad, h->ncs);
+    k_build_z<float><<<blocks_for(nz), 256, 0, h->stream>>>((const float2 *)h->csm.p, (const float2 *)xm,
+                                                             (float2 *)h->zbuf.p, (int)h->nv_pad, h->ncs);
     return hipGetLastError();
 }
 
@@ -2135,36 +2247,42 @@ hipError_t launch_grad_chain(Handle *h, double *gtau_dev, bool shift, int m, con
 
 hipError_t launch_norm2(Handle *h, const void *v, int64_t n, double *out_dev) {
     const int nb = 512;
-    k_dot_partial<<<nb, 256, 0, h->stream>>>((const float2 *)v, nullptr, n, (double *)h->red.p);
+    k_dot_partial<float><<<nb, 256, 0, h->stream>>>((const float2 *)v, nullptr, n, (double *)h->red.p);
     k_cg_scalar<<<1, 256, 0, h->stream>>>((const double *)h->red.p, nb, out_dev, 0, nullptr, 0, 0.0);
     return hipGetLastError();
 }
 
 hipError_t cg_run(Handle *h, const void *rhs_m, int iters, double lambda, double rel_tol, void *x_m,
                   double *log_dev) {
-    const int64_t n = h->nv_pad;
-    float2 *x = (float2 *)x_m;
-    float2 *r = (float2 *)h->cg_vec.p;
-    float2 *p = r + n;
-    float2 *q = p + n;
-    double *cg = cgs(h);
-    double *red = (double *)h->red.p;
-    const int nb = 512;
-    const float nscale = (float)(1.0 / (double)h->n_vox);
-    k_fill0<<<blocks_for(n), 256, 0, h->stream>>>(x, n);
-    k_copy2<<<blocks_for(n), 256, 0, h->stream>>>(r, (const float2 *)rhs_m, n);
-    k_copy2<<<blocks_for(n), 256, 0, h->stream>>>(p, (const float2 *)rhs_m, n);
-    k_dot_partial<<<nb, 256, 0, h->stream>>>(r, nullptr, n, red);
-    k_cg_scalar<<<1, 256, 0, h->stream>>>(red, nb, cg, 3, log_dev, 0, rel_tol);
-    for (int it = 1; it <= iters; ++it) {
-        hipError_t e = run_normal_masked(h, p, q, nscale, lambda, false);
-        if (e != hipSuccess) return e;
-        k_dot_partial<<<nb, 256, 0, h->stream>>>(p, q, n, red);
-        k_cg_scalar<<<1, 256, 0, h->stream>>>(red, nb, cg, 1, log_dev, it, rel_tol);
-        k_cg_update<<<nb, 256, 0, h->stream>>>(x, r, p, q, n, cg, red);
-        k_cg_scalar<<<1, 256, 0, h->stream>>>(red, nb, cg, 2, log_dev, it, rel_tol);
-        k_cg_pupdate<<<blocks_for(n), 256, 0, h->stream>>>(p, r, n, cg);
-    }
+    hipError_t err = hipSuccess;
+    by_prec(h, [&](auto rr) {
+        using R = decltype(rr);
+        using C = cplx_t<R>;
+        const int64_t n = h->nv_pad;
+        C *x = (C *)x_m;
+        C *r = (C *)h->cg_vec.p;
+        C *p = r + n;
+        C *q = p + n;
+        double *cg = cgs(h);
+        double *red = (double *)h->red.p;
+        const int nb = 512;
+        const double nscale = 1.0 / (double)h->n_vox;
+        k_fill0<R><<<blocks_for(n), 256, 0, h->stream>>>(x, n);
+        k_copy2<R><<<blocks_for(n), 256, 0, h->stream>>>(r, (const C *)rhs_m, n);
+        k_copy2<R><<<blocks_for(n), 256, 0, h->stream>>>(p, (const C *)rhs_m, n);
+        k_dot_partial<R><<<nb, 256, 0, h->stream>>>(r, nullptr, n, red);
+        k_cg_scalar<<<1, 256, 0, h->stream>>>(red, nb, cg, 3, log_dev, 0, rel_tol);
+        for (int it = 1; it <= iters && err == hipSuccess; ++it) {
+            err = run_normal_masked(h, p, q, h->f64 ? nscale : (double)(float)nscale, lambda, false);
+            if (err != hipSuccess) break;
+            k_dot_partial<R><<<nb, 256, 0, h->stream>>>(p, q, n, red);
+            k_cg_scalar<<<1, 256, 0, h->stream>>>(red, nb, cg, 1, log_dev, it, rel_tol);
+            k_cg_update<R><<<nb, 256, 0, h->stream>>>(x, r, p, q, n, cg, red);
+            k_cg_scalar<<<1, 256, 0, h->stream>>>(red, nb, cg, 2, log_dev, it, rel_tol);
+            k_cg_pupdate<R><<<blocks_for(n), 256, 0, h->stream>>>(p, r, n, cg);
+        }
+    });
+    if (err != hipSuccess) return err;
     return hipGetLastError();
 }
 

@@ -13,7 +13,7 @@ PKG_DIR = Path(__file__).resolve().parent.parent          # highordermri.jl_amd/
 LIB_PATH = PKG_DIR / "libhoe.so"
 
 HOE_OK, HOE_EINVAL, HOE_EDIM, HOE_EASSERT, HOE_ENOMEM, HOE_EHIP, HOE_ECLOSED, HOE_ENODEV = range(8)
-BACKEND_DIRECT, BACKEND_MIXED_FP16, BACKEND_MIXED_BF16 = 0, 1, 2
+BACKEND_DIRECT, BACKEND_MIXED_FP16, BACKEND_MIXED_BF16, BACKEND_FP64 = 0, 1, 2, 3
 HOE_DEVICE_PTRS, HOE_ASYNC = 1, 2
 HOE_CURRENT_WEIGHTS = -1
 
@@ -22,6 +22,7 @@ BACKENDS = {
     "mixed": BACKEND_MIXED_FP16,
     "mixed_fp16": BACKEND_MIXED_FP16,
     "mixed_bf16": BACKEND_MIXED_BF16,
+    "fp64": BACKEND_FP64,
 }
 
 # Every symbol include/hoe.h declares.
@@ -30,7 +31,7 @@ EXPORTS = (
     "hoe_set_backend", "hoe_forward", "hoe_adjoint", "hoe_normal", "hoe_set_weights",
     "hoe_set_coefficients", "hoe_set_kspha", "hoe_set_kspha_dt", "hoe_grad", "hoe_cg", "hoe_close", "hoe_destroy",
     "hoe_sync", "hoe_last_kernel_ms", "hoe_apply_girf", "hoe_apply_girf_f64", "hoe_girf_fft_length",
-    "hoe_normal_noalloc",
+    "hoe_normal_noalloc", "hoe_create_f64", "hoe_set_weights_f64",
 )
 
 
@@ -84,6 +85,21 @@ class ModelDesc(C.Structure):
     ]
 
 
+class ModelDescF64(C.Structure):
+    """hoe_model_desc_f64: the FP64 backend's model (double data, no dwell or backend)."""
+    _fields_ = [
+        ("n_x", C.c_int32), ("n_y", C.c_int32), ("n_z", C.c_int32),
+        ("x", C.c_void_p), ("y", C.c_void_p), ("z", C.c_void_p),
+        ("mask", C.c_void_p),
+        ("n_term", C.c_int32), ("n_sam", C.c_int32),
+        ("kspha", C.c_void_p), ("times", C.c_void_p), ("fieldmap", C.c_void_p),
+        ("n_cha", C.c_int32),
+        ("csm", C.c_void_p), ("concomitant", C.c_void_p),
+        ("recon_terms", C.c_char_p), ("k_nominal", C.c_void_p),
+        ("device", C.c_int32),
+    ]
+
+
 class Info(C.Structure):
     _fields_ = [
         ("n_grid", C.c_int64), ("n_vox", C.c_int64), ("n_sam", C.c_int64), ("n_cha", C.c_int64),
@@ -110,6 +126,8 @@ def load(path: str | os.PathLike | None = None):
         "hoe_last_error": (C.c_char_p, [vp]),
         "hoe_device_count": (C.c_int, []),
         "hoe_create": (C.c_int, [C.POINTER(ModelDesc), C.POINTER(vp)]),
+        "hoe_create_f64": (C.c_int, [C.POINTER(ModelDescF64), C.POINTER(vp)]),
+        "hoe_set_weights_f64": (C.c_int, [vp, vp, i64]),
         "hoe_get_info": (C.c_int, [vp, C.POINTER(Info)]),
         "hoe_set_backend": (C.c_int, [vp, i32]),
         "hoe_forward": (C.c_int, [vp, vp, vp, C.c_int]),

@@ -40,18 +40,23 @@ def _f32(a, name):
     return a
 
 
-def _c64_host(a):
-    return np.ascontiguousarray(np.asarray(a, dtype=np.complex64).reshape(-1))
+_NAMES = {np.complex64: "complex64 (ComplexF32)", np.complex128: "complex128 (ComplexF64)"}
+
+
+def _torch_cdtype(cdtype):
+    return torch.complex64 if cdtype == np.complex64 else torch.complex128
 
 
 class _Vec:
-    """A vector argument: host numpy (copied) or device torch (in place)."""
+    """A vector argument: host numpy (copied) or device torch (in place), of the operator's
+    complex type (complex64, or complex128 for the FP64 backend)."""
 
-    def __init__(self, a, n: int, name: str):
+    def __init__(self, a, n: int, name: str, cdtype=np.complex64):
         self.dev = _is_torch(a) and a.is_cuda
+        self.cdtype = cdtype
         if self.dev:
-            if a.dtype != torch.complex64:
-                raise ArgumentError(f"{name} must be complex64 (ComplexF32)")
+            if a.dtype != _torch_cdtype(cdtype):
+                raise ArgumentError(f"{name} must be {_NAMES[cdtype]}")
             t = a.reshape(-1)
             if not t.is_contiguous():
                 t = t.contiguous()
@@ -60,7 +65,7 @@ class _Vec:
         else:
             if _is_torch(a):
                 a = a.detach().numpy()
-            h = _c64_host(a)
+            h = np.ascontiguousarray(np.asarray(a, dtype=cdtype).reshape(-1))
             self.obj = h
             self.ptr = h.ctypes.data
         size = self.obj.numel() if self.dev else self.obj.size
@@ -69,8 +74,8 @@ class _Vec:
 
     def like(self, n: int):
         if self.dev:
-            return torch.empty(n, dtype=torch.complex64, device=self.obj.device)
-        return np.empty(n, dtype=np.complex64)
+            return torch.empty(n, dtype=_torch_cdtype(self.cdtype), device=self.obj.device)
+        return np.empty(n, dtype=self.cdtype)
 
 
 def _ptr(o):
@@ -88,16 +93,16 @@ def _out(out, vec: _Vec, n: int, name: str = "out"):
             raise ArgumentError(f"{name} must be a device tensor when the input is one")
         if out.device != vec.obj.device:
             raise ArgumentError(f"{name} must be on the input's device")
-        if out.dtype != torch.complex64:
-            raise ArgumentError(f"{name} must be complex64 (ComplexF32)")
+        if out.dtype != _torch_cdtype(vec.cdtype):
+            raise ArgumentError(f"{name} must be {_NAMES[vec.cdtype]}")
         if not out.is_contiguous():
             raise ArgumentError(f"{name} must be contiguous")
         size = out.numel()
     else:
         if not isinstance(out, np.ndarray):
             raise ArgumentError(f"{name} must be a numpy array when the input is a host array")
-        if out.dtype != np.complex64:
-            raise ArgumentError(f"{name} must be complex64 (ComplexF32)")
+        if out.dtype != vec.cdtype:
+            raise ArgumentError(f"{name} must be {_NAMES[vec.cdtype]}")
         if not (out.flags.c_contiguous or out.flags.f_contiguous) or not out.flags.writeable:
             raise ArgumentError(f"{name} must be a contiguous, writeable array")
         size = out.size
@@ -111,8 +116,10 @@ class HighOrderExplicitOp:
 
     Parameters mirror the reference constructor. ``backend`` selects the
     libhoe execution path: ``"direct"`` (FP32 VALU contraction), ``"mixed"``
-    / ``"mixed_fp16"`` or ``"mixed_bf16"`` (MFMA phase-reuse). ``dwell`` is
-    the coefficient sampling interval used by ``set_coefficients(..., tau)``.
+    / ``"mixed_fp16"`` or ``"mixed_bf16"`` (MFMA phase-reuse), or ``"fp64"``
+    (the Float64 operator: FP64 model and arithmetic, complex128 vectors; the
+    reference's operators with ``Grid{Float64}``). ``dwell`` is the coefficient
+    sampling interval used by ``set_coefficients(..., tau)``.
     """
 
     def __init__(self, grid: Grid, kspha, times, *, implementation: str = "kernel", fieldmap=None,
@@ -134,13 +141,17 @@ class HighOrderExplicitOp:
         if len(gpus) > 1:
             raise ArgumentError("one GPU per operator: sample-sharded multi-GPU execution is not built")
         lib = _lib.load()
-        kspha = _f32(kspha, "kspha")
+        f64 = backend == "fp64"
+        RT = np.float64 if f64 else np.float32
+        self._cdtype = np.complex128 if f64 else np.complex64
+        _real = (lambda a, name: np.ascontiguousarray(np.asarray(a, dtype=np.float64))) if f64 else _f32
+        kspha = _real(kspha, "kspha")
         if kspha.ndim != 2:
             raise DimensionMismatch("kspha must be (nTerm, nSam)")
         n_term, n_sam = kspha.shape
         if n_term not in (9, 16):
             raise ArgumentError("kspha requires 9 or 16 terms")
-        times = _f32(times, "times").reshape(-1)
+        times = _real(times, "times").reshape(-1)
         if times.size != n_sam:
             raise DimensionMismatch("times and kspha")
         nX, nY, nZ = grid.matrixSize
@@ -155,14 +166,14 @@ class HighOrderExplicitOp:
                                            dtype=np.uint8)
         fm = None
         if fieldmap is not None:
-            f = np.asarray(fieldmap, dtype=np.float32)
+            f = np.asarray(fieldmap, dtype=RT)
             if f.size != n_grid:
                 raise DimensionMismatch("fieldmap and mask")
             fm = np.ascontiguousarray(f.reshape(-1, order="F") if f.ndim > 1 else f)
         if csm is None:
             n_cha, cs = 1, None
         else:
-            c = np.asarray(csm, dtype=np.complex64)
+            c = np.asarray(csm, dtype=self._cdtype)
             if c.ndim == 1:
                 c = c.reshape(-1, 1)
             n_cha = c.shape[-1]
@@ -172,6 +183,8 @@ class HighOrderExplicitOp:
             cs = np.ascontiguousarray(cs.T)   # [coil][grid]: grid fastest per coil
         kdt = None
         if kspha_dt is not None:
+            if f64:
+                raise ArgumentError("kspha_dt (the derivative product) is not provided on the FP64 backend")
             kdt = _f32(kspha_dt, "kspha_dt")
             if kdt.shape != kspha.shape:
                 raise AssertionError("kspha_dt must have same size as kspha")
@@ -184,18 +197,18 @@ class HighOrderExplicitOp:
                 raise ArgumentError("concomitant must be a real array or nothing")
             if ca.shape != (4, n_sam):
                 raise DimensionMismatch(f"concomitant must have size (4, {n_sam}), got {ca.shape}")
-            conc = _f32(ca, "concomitant")
+            conc = _real(ca, "concomitant")
         kn = None
         if k_nominal is not None:
-            kn = _f32(k_nominal, "k_nominal")
+            kn = _real(k_nominal, "k_nominal")
             if kn.shape[0] != 3:
                 raise AssertionError("k_nominal must have 3 terms (row) for kx, ky, kz")
             if kn.shape != (3, n_sam):
                 raise DimensionMismatch("k_nominal must be (3, nSam)")
         # column-major host arrays: Julia (nTerm, nSam) is term-fastest = numpy (nSam, nTerm) C order
-        self._keep = [grid.x.astype(np.float32), grid.y.astype(np.float32), grid.z.astype(np.float32),
-                      mask_u8, np.ascontiguousarray(kspha.T), times]
-        d = _lib.ModelDesc()
+        self._keep = [np.ascontiguousarray(grid.x, dtype=RT), np.ascontiguousarray(grid.y, dtype=RT),
+                      np.ascontiguousarray(grid.z, dtype=RT), mask_u8, np.ascontiguousarray(kspha.T), times]
+        d = _lib.ModelDescF64() if f64 else _lib.ModelDesc()
         d.n_x, d.n_y, d.n_z = nX, nY, nZ
         d.x, d.y, d.z = (self._keep[0].ctypes.data, self._keep[1].ctypes.data, self._keep[2].ctypes.data)
         d.mask = mask_u8.ctypes.data
@@ -219,11 +232,12 @@ class HighOrderExplicitOp:
             kt = np.ascontiguousarray(kn.T)
             self._keep.append(kt)
             d.k_nominal = kt.ctypes.data
-        d.dwell = float(dwell)
-        d.backend = _lib.BACKENDS[backend]
+        if not f64:
+            d.dwell = float(dwell)
+            d.backend = _lib.BACKENDS[backend]
         d.device = gpus[0]
         h = C.c_void_p()
-        check(lib.hoe_create(C.byref(d), C.byref(h)))
+        check(lib.hoe_create_f64(C.byref(d), C.byref(h)) if f64 else lib.hoe_create(C.byref(d), C.byref(h)))
         self._keep = None
         self._h = h
         self._lib = lib
@@ -311,7 +325,7 @@ class HighOrderExplicitOp:
         """``op * x``: x of length prod(matrixSize) -> signal of length nSam*nCha
         (B x while ``kspha_dt`` is set)."""
         h = self._handle()
-        xv = _Vec(x, self.ncol, "x")
+        xv = _Vec(x, self.ncol, "x", self._cdtype)
         self._on_device(xv)
         y = _out(out, xv, self.nrow)
         flags = _lib.HOE_DEVICE_PTRS if xv.dev else 0
@@ -323,7 +337,7 @@ class HighOrderExplicitOp:
     def ctprod(self, y, out=None):
         """``adjoint(op) * y``."""
         h = self._handle()
-        yv = _Vec(y, self.nrow, "y")
+        yv = _Vec(y, self.nrow, "y", self._cdtype)
         self._on_device(yv)
         x = _out(out, yv, self.ncol)
         flags = _lib.HOE_DEVICE_PTRS if yv.dev else 0
@@ -452,7 +466,7 @@ def compose(W: WeightingOp, op: HighOrderExplicitOp) -> ComposedOp:
     return ComposedOp(W, op)
 
 
-def _check_diag(D, nrow, nsam):
+def _check_diag(D, nrow, nsam, dtype=np.float32):
     D = np.asarray(D).reshape(-1)
     if D.size == 0:
         raise DimensionMismatch("normal weights")
@@ -468,7 +482,7 @@ def _check_diag(D, nrow, nsam):
         raise ArgumentError("normal weights must be nonnegative")
     if D.size not in (nsam, nrow):
         raise DimensionMismatch("normal weights")
-    return np.ascontiguousarray(Dr.astype(np.float32))
+    return np.ascontiguousarray(Dr.astype(dtype))
 
 
 class NormalOp:
@@ -479,7 +493,8 @@ class NormalOp:
     def __init__(self, op: HighOrderExplicitOp, D=None):
         op._handle()
         self.op = op
-        self.D = None if D is None else _check_diag(D, op.nrow, op.nSam)
+        self.D = None if D is None else _check_diag(D, op.nrow, op.nSam,
+                                                     np.float64 if op.backend == "fp64" else np.float32)
         NormalOp._gen += 1
         self._id = NormalOp._gen
         self.symmetric, self.hermitian = False, True
@@ -491,16 +506,17 @@ class NormalOp:
     def _sync_weights(self, h):
         lib = self.op._lib
         if self.op._weights_owner != self._id:
+            setw = lib.hoe_set_weights_f64 if self.op.backend == "fp64" else lib.hoe_set_weights
             if self.D is None:
-                check(lib.hoe_set_weights(h, None, 0), h)
+                check(setw(h, None, 0), h)
             else:
-                check(lib.hoe_set_weights(h, self.D.ctypes.data, self.D.size), h)
+                check(setw(h, self.D.ctypes.data, self.D.size), h)
             self.op._weights_owner = self._id
 
     def prod(self, x, out=None):
         h = self.op._handle()
         self._sync_weights(h)
-        xv = _Vec(x, self.op.ncol, "x")
+        xv = _Vec(x, self.op.ncol, "x", self.op._cdtype)
         self.op._on_device(xv)
         o = _out(out, xv, self.op.ncol)
         flags = _lib.HOE_DEVICE_PTRS if xv.dev else 0
@@ -573,24 +589,30 @@ def recon_HOOp(HOOp: HighOrderExplicitOp, Data, weight, recParams: dict, *, retu
     h = op._handle()
     data = Data
     if _is_torch(data) and data.is_cuda:
-        yv = _Vec(data.t().contiguous().reshape(-1) if data.dim() == 2 else data, op.nrow, "Data")
+        yv = _Vec(data.t().contiguous().reshape(-1) if data.dim() == 2 else data, op.nrow, "Data", op._cdtype)
     else:
         d = np.asarray(data)
-        yv = _Vec(d.reshape(-1, order="F") if d.ndim == 2 else d, op.nrow, "Data")
+        yv = _Vec(d.reshape(-1, order="F") if d.ndim == 2 else d, op.nrow, "Data", op._cdtype)
     op._on_device(yv)
     w = weight.detach().cpu().numpy() if _is_torch(weight) else np.asarray(weight)
     w = w.reshape(-1)
     if w.size != op.nSam:
         raise DimensionMismatch("weight must have one entry per sample")
-    D = np.ascontiguousarray((np.abs(w.astype(np.complex128)) ** 2).astype(np.float32))
     op._weights_owner = None
     x = yv.like(op.ncol)
     log = np.zeros(iters + 1, dtype=np.float64)
     flags = _lib.HOE_DEVICE_PTRS if yv.dev else 0
     if yv.dev:
         torch.cuda.current_stream(yv.obj.device).synchronize()
-    check(op._lib.hoe_cg(h, yv.ptr, D.ctypes.data, D.size, iters, lam, rel_tol, _ptr(x), log.ctypes.data,
-                         flags), h)
+    if op.backend == "fp64":
+        D = np.ascontiguousarray(np.abs(w.astype(np.complex128)) ** 2)
+        check(op._lib.hoe_set_weights_f64(h, D.ctypes.data, D.size), h)
+        check(op._lib.hoe_cg(h, yv.ptr, None, _lib.HOE_CURRENT_WEIGHTS, iters, lam, rel_tol, _ptr(x),
+                             log.ctypes.data, flags), h)
+    else:
+        D = np.ascontiguousarray((np.abs(w.astype(np.complex128)) ** 2).astype(np.float32))
+        check(op._lib.hoe_cg(h, yv.ptr, D.ctypes.data, D.size, iters, lam, rel_tol, _ptr(x), log.ctypes.data,
+                             flags), h)
     img = x.detach().cpu().numpy() if _is_torch(x) else x
     img = np.asarray(img).reshape(recParams["reconSize"], order="F")
     return (img, log) if return_log else img

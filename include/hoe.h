@@ -65,7 +65,13 @@ typedef enum {
      * (replaces the phase-reuse GEMM path, explicit_gpu_operator.jl:117-297). */
     HOE_BACKEND_MIXED_FP16 = 1,
     /* As MIXED_FP16 with BF16 hi+lo tiles (v_mfma_f32_32x32x16_bf16). */
-    HOE_BACKEND_MIXED_BF16 = 2
+    HOE_BACKEND_MIXED_BF16 = 2,
+    /* Float64 operator (the reference's ComplexF64 operators, test_explicit_gpu.jl:51):
+     * FP64 model, FP64 phase, FP64 sincos (device math library sincospi after an exact
+     * reduction modulo one cycle), FP64 VALU contraction and reductions. Handles of this
+     * backend come from hoe_create_f64 and take ComplexF64 vectors; they cannot switch
+     * to another backend, nor another handle to this one. */
+    HOE_BACKEND_FP64 = 3
 } hoe_backend;
 
 enum {
@@ -101,6 +107,26 @@ typedef struct {
     int32_t device;              /* HIP device ordinal ("gpus=[id]") */
 } hoe_model_desc;
 
+/* Model description for hoe_create_f64: the fields of hoe_model_desc with Float64
+ * data (Grid{Float64}, Float64 kspha/times/fieldmap, ComplexF64 csm), for the
+ * FP64 backend. No dwell (delays are not provided in FP64) and no backend field. */
+typedef struct {
+    int32_t n_x, n_y, n_z;
+    const double *x, *y, *z;
+    const uint8_t *mask;
+    int32_t n_term;
+    int32_t n_sam;
+    const double *kspha;         /* n_term x n_sam, term fastest */
+    const double *times;
+    const double *fieldmap;      /* NULL = zeros */
+    int32_t n_cha;
+    const double *csm;           /* complex n_grid x n_cha interleaved (re, im), grid fastest; NULL = ones */
+    const double *concomitant;   /* 4 x n_sam; NULL = none */
+    const char *recon_terms;
+    const double *k_nominal;     /* 3 x n_sam; NULL = kspha rows 2..4 */
+    int32_t device;
+} hoe_model_desc_f64;
+
 typedef struct {
     int64_t n_grid;   /* prod(matrixSize): length of image vectors */
     int64_t n_vox;    /* voxels inside the mask */
@@ -130,6 +156,16 @@ int hoe_device_count(void);
  * construction (kernel_operator.jl:199-315, explicit_gpu_operator.jl:299-392).
  * The FP64 basis is evaluated on the host from the FP32 coordinates. */
 int hoe_create(const hoe_model_desc *desc, hoe_handle **out);
+
+/* The FP64 operator (HOE_BACKEND_FP64): same validation, coefficient preparation,
+ * basis (FP64 from the FP64 coordinates) and masking as hoe_create. On its handle
+ * hoe_forward, hoe_adjoint, hoe_normal and hoe_cg take ComplexF64 vectors (interleaved
+ * double pairs, same layouts); weights passed as float to hoe_normal / hoe_cg /
+ * hoe_set_weights are widened, hoe_set_weights_f64 takes them in FP64. hoe_grad,
+ * hoe_set_coefficients, hoe_set_kspha, hoe_set_kspha_dt and hoe_normal_noalloc return
+ * HOE_EINVAL. Replaces the reference's Float64 operators (HighOrderExplicitOp with
+ * Grid{Float64}, array_operator.jl:83-181; test_explicit_gpu.jl:51, 2e-12). */
+int hoe_create_f64(const hoe_model_desc_f64 *desc, hoe_handle **out);
 
 int hoe_get_info(const hoe_handle *h, hoe_info_t *info);
 
@@ -173,6 +209,10 @@ int hoe_normal_noalloc(hoe_handle *h, const void *x, void *out, int flags);
  * weights. The stored weights play the role of the diagonal captured by
  * normalOperator(op, D) (explicit_gpu_operator.jl:399-416). */
 int hoe_set_weights(hoe_handle *h, const float *D, int64_t n_d);
+
+/* hoe_set_weights with FP64 weights, for FP64 handles (same rules; on other handles the
+ * weights are rounded to FP32). */
+int hoe_set_weights_f64(hoe_handle *h, const double *D, int64_t n_d);
 
 /* Replace the field-coefficient traces and the global field/data delay tau
  * (seconds). kspha holds n_term_eff x n_sam values (term fastest) that are

@@ -134,20 +134,24 @@ class Model:
 
 
 def make_model(grid, kspha, times, fieldmap=None, csm=None, mask=None, concomitant=None, recon_terms=None,
-               k_nominal=None):
-    """array_operator.jl:102-139 (masking, prep_kspha, basisfunc_spha, prep_concomitant)."""
-    kspha = np.asarray(kspha, dtype=np.float32)
+               k_nominal=None, f64=False):
+    """array_operator.jl:102-139 (masking, prep_kspha, basisfunc_spha, prep_concomitant).
+    ``f64``: the model of a Float64 operator (Grid{Float64}, Float64 coefficients, times and
+    maps, ComplexF64 coil maps): inputs kept in FP64 and the basis evaluated from the FP64
+    coordinates; otherwise the inputs are Float32, as for the ComplexF32 operators."""
+    RT = np.float64 if f64 else np.float32
+    kspha = np.asarray(kspha, dtype=RT)
     n_term, ns = kspha.shape
     n_grid = grid.nX * grid.nY * grid.nZ
     m = np.ones(n_grid, dtype=bool) if mask is None else np.asarray(mask).reshape(-1, order="F").astype(bool)
     idx = np.nonzero(m)[0]
-    kn = kspha[1:4, :] if k_nominal is None else np.asarray(k_nominal, dtype=np.float32)
+    kn = kspha[1:4, :] if k_nominal is None else np.asarray(k_nominal, dtype=RT)
     k = prep_kspha(kspha, kn, n_term, recon_terms)
     x, y, z = grid.x[idx], grid.y[idx], grid.z[idx]
-    b64 = basis_f64(x, y, z)[:, :n_term]
+    b64 = (basis_f64d if f64 else basis_f64)(x, y, z)[:, :n_term]
     b32 = basis_f32(x, y, z)[:, :n_term]
     if concomitant is not None:
-        conc = np.asarray(concomitant, dtype=np.float32)
+        conc = np.asarray(concomitant, dtype=RT)
         X, Y, Z = (a.astype(np.float64) for a in (x, y, z))
         c64 = np.stack([Z * Z, X * X + Y * Y, X * Z, Y * Z], axis=1)
         xf, yf, zf = x.astype(np.float32), y.astype(np.float32), z.astype(np.float32)
@@ -155,12 +159,13 @@ def make_model(grid, kspha, times, fieldmap=None, csm=None, mask=None, concomita
         b64 = np.concatenate([b64, c64], axis=1)
         b32 = np.concatenate([b32, c32], axis=1)
         k = np.concatenate([k, conc], axis=0)
-    fm = np.zeros(n_grid, dtype=np.float32) if fieldmap is None else \
-        np.asarray(fieldmap, dtype=np.float32).reshape(-1, order="F")
+    fm = np.zeros(n_grid, dtype=RT) if fieldmap is None else \
+        np.asarray(fieldmap, dtype=RT).reshape(-1, order="F")
+    CT = np.complex128 if f64 else np.complex64
     if csm is None:
-        cs = np.ones((1, idx.size), dtype=np.complex64)
+        cs = np.ones((1, idx.size), dtype=CT)
     else:
-        c = np.asarray(csm, dtype=np.complex64)
+        c = np.asarray(csm, dtype=CT)
         c = c.reshape(n_grid, -1, order="F") if c.ndim != 2 else c
         cs = np.ascontiguousarray(c[idx, :].T)
     L = k.shape[0]
@@ -168,7 +173,7 @@ def make_model(grid, kspha, times, fieldmap=None, csm=None, mask=None, concomita
                  bf64=np.ascontiguousarray(b64), bf32=np.ascontiguousarray(b32, dtype=np.float32),
                  k64=np.ascontiguousarray(k.T.astype(np.float64)),
                  k32=np.ascontiguousarray(k.T.astype(np.float32)),
-                 times=np.ascontiguousarray(np.asarray(times, dtype=np.float32).reshape(-1)),
+                 times=np.ascontiguousarray(np.asarray(times, dtype=RT).reshape(-1)),
                  fieldmap=np.ascontiguousarray(fm[idx]), csm=cs)
 
 

@@ -80,3 +80,25 @@ def test_noalloc_rejects_oversized_and_host_vectors(gpu_lib):
         rc = lib.hoe_normal_noalloc(h, xh.ctypes.data, oh.ctypes.data, 0)
         assert rc == _lib.HOE_EINVAL
         assert "HOE_DEVICE_PTRS" in lib.hoe_last_error(h).decode()
+
+
+def test_noalloc_stale_nonfinite_scratch_does_not_leak(gpu_lib):
+    """The direct kernels write partial sums only for their coil batches (8 of the 32 padded
+    columns for 3 coils); the reductions must read only real coils. A first call with a NaN
+    image leaves non-finite partial sums in the static array; a 3-coil operator's call
+    afterwards must still equal hoe_normal."""
+    import torch
+    c = synth.make(1, n_sam=4000, n_cha=32, seed=21)
+    with hoe.HighOrderExplicitOp(c["grid"], c["kspha"], c["times"], csm=c["csm"], backend="direct") as op:
+        lib, h = op._lib, op._handle()
+        x = torch.full((op.ncol,), complex(float("nan"), float("nan")), dtype=torch.complex64, device="cuda:0")
+        o = torch.empty_like(x)
+        _lib.check(lib.hoe_set_weights(h, None, 0), h)
+        _lib.check(lib.hoe_normal_noalloc(h, x.data_ptr(), o.data_ptr(), _lib.HOE_DEVICE_PTRS), h)
+        torch.cuda.synchronize()
+        assert torch.isnan(o).any()
+    op, x3, w = _case(0, "direct", 2000, 3, 22)
+    with op:
+        ref, out = _normal_pair(op, x3, w)
+        assert np.isfinite(ref).all() and np.isfinite(out).all()
+        assert np.array_equal(ref, out)
