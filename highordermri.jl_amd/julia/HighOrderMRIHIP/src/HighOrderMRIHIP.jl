@@ -29,7 +29,8 @@ const libhoe = get(ENV, "HOE_LIBRARY", joinpath(@__DIR__, "..", "..", "..", "lib
 
 const HOE_DEVICE_PTRS = Cint(1)
 const HOE_CURRENT_WEIGHTS = Int64(-1)
-const BACKENDS = Dict(:direct => Int32(0), :mixed => Int32(1), :mixed_fp16 => Int32(1), :mixed_bf16 => Int32(2))
+const BACKENDS = Dict(:direct => Int32(0), :mixed => Int32(1), :mixed_fp16 => Int32(1), :mixed_bf16 => Int32(2),
+                      :fp64 => Int32(3))
 
 # Field order and types of hoe_model_desc.
 struct ModelDesc
@@ -43,6 +44,19 @@ struct ModelDesc
     recon_terms::Ptr{UInt8}; k_nominal::Ptr{Float32}
     dwell::Float64
     backend::Int32; device::Int32
+end
+
+# Field order and types of hoe_model_desc_f64 (the FP64 backend).
+struct ModelDescF64
+    n_x::Int32; n_y::Int32; n_z::Int32
+    x::Ptr{Float64}; y::Ptr{Float64}; z::Ptr{Float64}
+    mask::Ptr{UInt8}
+    n_term::Int32; n_sam::Int32
+    kspha::Ptr{Float64}; times::Ptr{Float64}; fieldmap::Ptr{Float64}
+    n_cha::Int32
+    csm::Ptr{ComplexF64}; concomitant::Ptr{Float64}
+    recon_terms::Ptr{UInt8}; k_nominal::Ptr{Float64}
+    device::Int32
 end
 
 function check(rc::Integer, h::Ptr{Cvoid}=C_NULL)
@@ -74,7 +88,9 @@ end
 const WEIGHTS_GEN = Threads.Atomic{UInt}(0)
 next_weights_id() = Threads.atomic_add!(WEIGHTS_GEN, UInt(1)) + UInt(1)
 
-mutable struct HIPExplicitOp{F,G} <: HighOrderExplicitOp{ComplexF32}
+# T = ComplexF32 (backends :direct, :mixed, :mixed_bf16) or ComplexF64 (:fp64, built from
+# Grid{Float64} inputs).
+mutable struct HIPExplicitOp{T,F,G} <: HighOrderExplicitOp{T}
     nrow::Int
     ncol::Int
     symmetric::Bool
@@ -85,8 +101,8 @@ mutable struct HIPExplicitOp{F,G} <: HighOrderExplicitOp{ComplexF32}
     nprod::Int
     ntprod::Int
     nctprod::Int
-    Mv::Vector{ComplexF32}
-    Mtu::Vector{ComplexF32}
+    Mv::Vector{T}
+    Mtu::Vector{T}
     handle::Ptr{Cvoid}
     nSam::Int
     nCha::Int
@@ -154,17 +170,61 @@ function HIPExplicitOp(grid::Grid{Float32}, kspha::AbstractMatrix{Float32}, time
         end
     end
     nTermEff = nTerm + (concomitant === nothing ? 0 : 4)
-    op = HIPExplicitOp(nSam * nCha, prod(grid.matrixSize), false, false, nothing, nothing, nothing,
-                       0, 0, 0, ComplexF32[], ComplexF32[], h[], nSam, nCha, nTermEff, UInt(0))
+    return wrap_handle(ComplexF32, h[], nSam, nCha, prod(grid.matrixSize), nTermEff)
+end
+
+"""
+    HIPExplicitOp(grid::Grid{Float64}, kspha::AbstractMatrix{Float64}, times::AbstractVector{Float64};
+                  fieldmap, csm, mask, concomitant, recon_terms, k_nominal, gpus=[0])
+
+The Float64 operator (backend `:fp64`, `hoe_create_f64`): FP64 model, phase, sincos and
+contraction, `ComplexF64` vectors; the reference's operators with `Grid{Float64}`
+(test_explicit_gpu.jl:51). The derivative product and delays are not provided in FP64.
+"""
+function HIPExplicitOp(grid::Grid{Float64}, kspha::AbstractMatrix{Float64}, times::AbstractVector{Float64};
+                       fieldmap=zeros(Float64, grid.matrixSize...), csm=ones(ComplexF64, grid.matrixSize..., 1),
+                       mask=trues(grid.matrixSize...), concomitant=nothing, kspha_dt=nothing,
+                       recon_terms=nothing, k_nominal=nothing, gpus=[0], backend::Symbol=:fp64, dwell::Real=0)
+    length(gpus) == 1 || throw(ArgumentError("HIPExplicitOp runs on one GPU"))
+    kspha_dt === nothing || throw(ArgumentError("kspha_dt (the derivative product) is not provided in Float64"))
+    backend in (:fp64, :mixed, :direct, :mixed_fp16, :mixed_bf16) ||
+        throw(ArgumentError("unknown explicit backend $backend"))
+    nTerm, nSam = size(kspha)
+    nCha = size(csm)[end]
+    x, y, z = Vector{Float64}(grid.x), Vector{Float64}(grid.y), Vector{Float64}(grid.z)
+    m = Vector{UInt8}(vec(mask))
+    k = Matrix{Float64}(kspha)
+    t = Vector{Float64}(times)
+    f = Vector{Float64}(vec(fieldmap))
+    c = Vector{ComplexF64}(vec(csm))
+    conc = concomitant === nothing ? nothing : Matrix{Float64}(concomitant)
+    kn = k_nominal === nothing ? nothing : Matrix{Float64}(k_nominal)
+    rt = recon_terms === nothing ? nothing : Vector{UInt8}(codeunits(recon_terms * "\0"))
+    h = Ref{Ptr{Cvoid}}(C_NULL)
+    GC.@preserve x y z m k t f c conc kn rt begin
+        d = ModelDescF64(grid.nX, grid.nY, grid.nZ, pointer(x), pointer(y), pointer(z), pointer(m),
+                         nTerm, nSam, pointer(k), pointer(t), pointer(f), nCha, pointer(c),
+                         conc === nothing ? C_NULL : pointer(conc), rt === nothing ? C_NULL : pointer(rt),
+                         kn === nothing ? C_NULL : pointer(kn), Int32(first(gpus)))
+        check(ccall((:hoe_create_f64, libhoe), Cint, (Ref{ModelDescF64}, Ref{Ptr{Cvoid}}), d, h))
+    end
+    nTermEff = nTerm + (concomitant === nothing ? 0 : 4)
+    return wrap_handle(ComplexF64, h[], nSam, nCha, prod(grid.matrixSize), nTermEff)
+end
+
+# The LinearOperators fields and prod!/ctprod! closures around a libhoe handle.
+function wrap_handle(::Type{T}, h::Ptr{Cvoid}, nSam, nCha, nGrid, nTermEff) where {T<:Union{ComplexF32,ComplexF64}}
+    op = HIPExplicitOp{T,Any,Any}(nSam * nCha, nGrid, false, false, nothing, nothing, nothing,
+                                  0, 0, 0, T[], T[], h, nSam, nCha, nTermEff, UInt(0))
     op.prod! = (res, xv) -> begin
         sync_for(xv, res)
-        check(ccall((:hoe_forward, libhoe), Cint, (Ptr{Cvoid}, Ptr{ComplexF32}, Ptr{ComplexF32}, Cint),
+        check(ccall((:hoe_forward, libhoe), Cint, (Ptr{Cvoid}, Ptr{T}, Ptr{T}, Cint),
                     handle(op), pointer(xv), pointer(res), flags(xv)), op.handle)
         res
     end
     op.ctprod! = (res, yv) -> begin
         sync_for(yv, res)
-        check(ccall((:hoe_adjoint, libhoe), Cint, (Ptr{Cvoid}, Ptr{ComplexF32}, Ptr{ComplexF32}, Cint),
+        check(ccall((:hoe_adjoint, libhoe), Cint, (Ptr{Cvoid}, Ptr{T}, Ptr{T}, Cint),
                     handle(op), pointer(yv), pointer(res), flags(yv)), op.handle)
         res
     end
@@ -213,8 +273,8 @@ function HighOrderMRI.HighOrderExplicitOp(grid::Grid{T}, kspha::Matrix{T}, times
     throw(ArgumentError("implementation must be :array, :kernel or :hip"))
 end
 
-function Base.adjoint(op::HIPExplicitOp)
-    return LinearOperator{ComplexF32}(op.ncol, op.nrow, false, false, op.ctprod!, nothing, op.prod!)
+function Base.adjoint(op::HIPExplicitOp{T}) where {T}
+    return LinearOperator{T}(op.ncol, op.nrow, false, false, op.ctprod!, nothing, op.prod!)
 end
 
 function Base.close(op::HIPExplicitOp)
@@ -236,29 +296,35 @@ function normal_weights(op::HIPExplicitOp, raw)
         maximum(abs ∘ imag, A) <= 10 * eps(Float32) * scale ||
             throw(ArgumentError("normal weights must already be real nonnegative WᴴW"))
     end
-    D = Vector{Float32}(real.(vec(A)))
+    D = Vector{real(eltype(op.Mv))}(real.(vec(A)))
     minimum(D) >= 0 || throw(ArgumentError("normal weights must be nonnegative"))
     length(D) in (op.nSam, op.nrow) || throw(DimensionMismatch("normal weights"))
     return D
 end
 
-function LinearOperatorCollection.normalOperator(op::HIPExplicitOp, weights=nothing; kwargs...)
+set_weights_ccall(h, D::Union{Nothing,Vector{Float32}}) =
+    GC.@preserve D ccall((:hoe_set_weights, libhoe), Cint, (Ptr{Cvoid}, Ptr{Float32}, Int64), h,
+                         D === nothing ? C_NULL : pointer(D), D === nothing ? 0 : length(D))
+set_weights_ccall(h, D::Vector{Float64}) =
+    GC.@preserve D ccall((:hoe_set_weights_f64, libhoe), Cint, (Ptr{Cvoid}, Ptr{Float64}, Int64), h,
+                         pointer(D), length(D))
+
+function LinearOperatorCollection.normalOperator(op::HIPExplicitOp{T}, weights=nothing; kwargs...) where {T}
     raw = weights isa LinearOperatorCollection.WeightingOp ? weights.weights : weights
     D = normal_weights(op, raw)
     id = next_weights_id()
     normal = (res, xv) -> begin
         h = handle(op)
         if op.weights_id != id
-            GC.@preserve D check(ccall((:hoe_set_weights, libhoe), Cint, (Ptr{Cvoid}, Ptr{Float32}, Int64), h,
-                                       D === nothing ? C_NULL : pointer(D), D === nothing ? 0 : length(D)), h)
+            check(set_weights_ccall(h, D), h)
             op.weights_id = id
         end
         sync_for(xv, res)
-        check(ccall((:hoe_normal, libhoe), Cint, (Ptr{Cvoid}, Ptr{Float32}, Int64, Ptr{ComplexF32}, Ptr{ComplexF32}, Cint),
+        check(ccall((:hoe_normal, libhoe), Cint, (Ptr{Cvoid}, Ptr{Float32}, Int64, Ptr{T}, Ptr{T}, Cint),
                     h, C_NULL, HOE_CURRENT_WEIGHTS, pointer(xv), pointer(res), flags(xv)), h)
         res
     end
-    return LinearOperator{ComplexF32}(op.ncol, op.ncol, false, true, normal, nothing, normal)
+    return LinearOperator{T}(op.ncol, op.ncol, false, true, normal, nothing, normal)
 end
 
 """
@@ -276,7 +342,7 @@ explicit_gpu_normal(op::HIPExplicitOp; weights=nothing) =
 `‖E(h, τ) x − y‖²` with coefficient traces `h` `(nTerm_eff, nSam)` and delay `τ`
 (seconds); differentiable through `ChainRulesCore.rrule` with respect to `h` and `τ`.
 """
-function hip_explicit_loss(op::HIPExplicitOp, h::AbstractMatrix{Float32}, τ::Real, x, y)
+function hip_explicit_loss(op::HIPExplicitOp{ComplexF32}, h::AbstractMatrix{Float32}, τ::Real, x, y)
     loss, _, _ = hip_explicit_grad(op, h, τ, x, y)
     return loss
 end
@@ -289,7 +355,7 @@ its gradient with respect to `h` and its derivative with respect to `τ`. `x` an
 both host `Array`s or both device arrays; with device arrays `h` may be a device array
 too and `g_h` is one (no host round trip).
 """
-function hip_explicit_grad(op::HIPExplicitOp, h::AbstractMatrix{Float32}, τ::Real, x, y)
+function hip_explicit_grad(op::HIPExplicitOp{ComplexF32}, h::AbstractMatrix{Float32}, τ::Real, x, y)
     ondevice(x) == ondevice(y) || throw(ArgumentError("x and y must both be host or both be device arrays"))
     size(h) == (op.nTerm, op.nSam) || throw(DimensionMismatch("coefficients"))
     hd = handle(op)

@@ -53,7 +53,7 @@ def _jl_kind(jtype):
 
 def _header_struct_fields(name):
     text = _strip_c_comments(HDR.read_text())
-    body = re.search(r"typedef struct \{(.*?)\}\s*" + name + ";", text, flags=re.S).group(1)
+    body = re.search(r"typedef struct \{([^{}]*)\}\s*" + name + ";", text).group(1)
     fields = []
     for decl in body.split(";"):
         decl = decl.strip()
@@ -127,23 +127,37 @@ def _julia_ccalls(text):
     return calls
 
 
-def test_julia_model_desc_matches_header():
-    text = (JL / "src" / "HighOrderMRIHIP.jl").read_text()
-    body = re.search(r"struct ModelDesc(.*?)\nend", text, flags=re.S).group(1)
-    jl = []
+def _julia_struct(text, name):
+    body = re.search(r"struct " + name + r"\n(.*?)\nend", text, flags=re.S).group(1)
+    fields = []
     for line in body.splitlines():
         line = line.split("#")[0]
         for decl in line.split(";"):
             decl = decl.strip()
             if decl:
                 n, t = decl.split("::")
-                jl.append((n.strip(), t.strip()))
-    hdr = _header_struct_fields("hoe_model_desc")
+                fields.append((n.strip(), t.strip()))
+    return fields
+
+
+_ELEM = {"float": {"Float32", "ComplexF32"}, "double": {"Float64", "ComplexF64"},
+         "uint8_t": {"UInt8"}, "char": {"UInt8"}}
+
+
+@pytest.mark.parametrize("jl_name,c_name,py", [("ModelDesc", "hoe_model_desc", "ModelDesc"),
+                                              ("ModelDescF64", "hoe_model_desc_f64", "ModelDescF64")])
+def test_julia_model_desc_matches_header(jl_name, c_name, py):
+    text = (JL / "src" / "HighOrderMRIHIP.jl").read_text()
+    jl = _julia_struct(text, jl_name)
+    hdr = _header_struct_fields(c_name)
     assert [n for n, _ in jl] == [n for n, _ in hdr]
     for (n, jt), (_, ct) in zip(jl, hdr):
         assert _jl_kind(jt) == _c_kind(ct), (n, jt, ct)
+        if _c_kind(ct) == "ptr":
+            elem = ct.replace("const", "").replace("*", "").strip()
+            assert jt[4:-1] in _ELEM[elem], (n, jt, ct)
     # the Python binding's mirror too
-    assert [f[0] for f in _lib.ModelDesc._fields_] == [n for n, _ in hdr]
+    assert [f[0] for f in getattr(_lib, py)._fields_] == [n for n, _ in hdr]
 
 
 def test_julia_ccalls_match_prototypes():
