@@ -1,10 +1,17 @@
 """Reader of the 7 T fixtures written by make_7t_fixture.py (shared by the real-data
-tests and bench.py). load() returns None when the fixture has not been generated."""
+tests and bench.py). load() returns None when the large per-dataset fixture has not been
+generated; reference() reads the small committed one (the reference's rendered images
+and stored delays)."""
 from pathlib import Path
 
 import numpy as np
 
 HERE = Path(__file__).resolve().parent
+
+
+def reference():
+    """The committed reference-held values: rendered reconstructions and stored delays."""
+    return np.load(HERE / "7t_reference.npz")
 
 
 def load(name, trajectory):
@@ -16,6 +23,9 @@ def load(name, trajectory):
     if not path.exists():
         return None
     z = np.load(path)
+    r = reference()
+    gm = r[f"{name}_golden_measured"].astype(np.float32)
+    gn = r[f"{name}_golden_nominal"].astype(np.float32)
     nX, nY, nZ = (int(v) for v in z["matrix"])
     res = z["resolution"]
     grid = hoe.Grid(nX, nY, nZ, res[0], res[1], res[2])
@@ -25,11 +35,11 @@ def load(name, trajectory):
     if trajectory == "measured":
         kspha = z["kspha_measured"]
         data = fieldcam.demodulate(z["kdata"], z["k0_adc"], kspha[0]).astype(np.complex64)
-        weight, terms, golden, other = z["weight_measured"], "0111", z["golden_measured"], z["golden_nominal"]
+        weight, terms, golden, other = z["weight_measured"], "0111", gm, gn
     else:
         kspha = z["kspha_nominal"]
         data = z["kdata"]
-        weight, terms, golden, other = z["weight_nominal"], "010", z["golden_nominal"], z["golden_measured"]
+        weight, terms, golden, other = z["weight_nominal"], "010", gn, gm
     return dict(grid=grid, kspha=kspha, times=z["times"], csm=csm, fieldmap=z["fieldmap"], mask=mask,
                 data=data, weight=weight, recon_terms=terms, golden=golden, golden_other=other)
 

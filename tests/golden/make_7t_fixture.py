@@ -1,15 +1,18 @@
 """Writes tests/golden/7t_spiral.npz and tests/golden/7t_epi.npz from the reference's
-7 T demo data (demo/7T_2D_{Spiral,EPI}_1p0_200_r4.mat) and its own reconstructions of
-them (demo/result/*_{Measured,Nominal}.png), for the real-data GPU tests.
+7 T demo data (demo/7T_2D_{Spiral,EPI}_1p0_200_r4.mat), and tests/golden/7t_reference.npz
+from its own reconstructions of them (demo/result/*_{Measured,Nominal}.png) and the
+delays stored with the data, for the real-data GPU tests.
 
 The arrays are the operator inputs prepared by hoe.fieldcam.prepare_demo (coil maps
 stored on the masked voxels only: they are zero elsewhere), the raw k-space data with
 its k0_adc, both field coefficient sets at the ADC times and as measured (with their
 start times, sampling intervals and the delays stored with the data), their density
-weights, and the two rendered reconstructions of the
-reference reduced to the 200 x 200 image grid: the plot area of each PNG (below the
-title) box-averaged onto 200 x 200 pixels, gray levels in [0, 1]. The files are large
-(about 16 MB each) and are generated, not committed. Run from the repository root:
+weights. These two files are large (about 16 MB each) and are generated, not committed.
+7t_reference.npz is small and committed: per dataset the two rendered reconstructions
+of the reference reduced to the 200 x 200 image grid (the plot area of each PNG below the
+title, box-averaged onto 200 x 200 pixels, gray levels in [0, 1], stored as float16) and
+the stored delays tauMeasured / tauNominal with the ADC dwell. Run from the repository
+root:
 
     python tests/golden/make_7t_fixture.py [REFERENCE_ROOT]
 """
@@ -36,8 +39,9 @@ def rendered_image(png, n):
     return (np.asarray(small, dtype=np.float32) / 255.0)
 
 
-def main(ref):
-    demo = Path(ref) / "demo"
+def main(reference_root):
+    demo = Path(reference_root) / "demo"
+    ref = {}
     for name, stem in DATASETS.items():
         raw = fieldcam.load_demo_mat(demo / f"{stem}.mat")
         meas = fieldcam.prepare_demo(raw, "measured")
@@ -58,12 +62,19 @@ def main(ref):
             field_nominal=np.asarray(raw["ksphaNominal"], dtype=np.float32),
             start_measured=float(raw["startMeasured"]), start_nominal=float(raw["startNominal"]),
             dt_measured=float(raw["dt_Measured"]), dt_nominal=float(raw["dt_Nominal"]),
-            golden_measured=rendered_image(demo / "result" / f"{stem}_Measured.png", g.nX),
-            golden_nominal=rendered_image(demo / "result" / f"{stem}_Nominal.png", g.nX),
         )
         path = Path(__file__).parent / f"7t_{name}.npz"
         np.savez_compressed(path, **out)
         print(f"wrote {path} ({path.stat().st_size / 1e6:.1f} MB)")
+        for traj in ("Measured", "Nominal"):
+            ref[f"{name}_golden_{traj.lower()}"] = rendered_image(demo / "result" / f"{stem}_{traj}.png",
+                                                                  g.nX).astype(np.float16)
+        ref[f"{name}_tau_measured"] = float(raw["tauMeasured"])
+        ref[f"{name}_tau_nominal"] = float(raw["tauNominal"])
+        ref[f"{name}_dt_adc"] = float(raw["dt_adc"])
+    path = Path(__file__).parent / "7t_reference.npz"
+    np.savez_compressed(path, **ref)
+    print(f"wrote {path} ({path.stat().st_size / 1e3:.0f} kB)")
 
 
 if __name__ == "__main__":

@@ -92,3 +92,29 @@ def test_prepare_demo_against_the_mat_file(stem):
     nom = fieldcam.prepare_demo(raw, "nominal")
     assert nom["kspha"].shape == (9, t.size) and nom["recon_terms"] == "010"
     assert np.array_equal(nom["data"], np.asarray(raw["kdata"]).astype(np.complex64))
+
+
+def test_committed_reference_values_of_the_7t_demo():
+    """tests/golden/7t_reference.npz (committed; the large operator inputs are generated):
+    the reference's own rendered reconstructions of its two 7 T demo datasets
+    (demo/result/*_{Measured,Nominal}.png on the 200 x 200 grid) and the delays stored with
+    the data (tauMeasured / tauNominal, in ADC dwells), which the GPU real-data tests
+    compare against."""
+    import sys
+    from pathlib import Path
+    sys.path.insert(0, str(Path(__file__).parent / "golden"))
+    import fixture7t
+    r = fixture7t.reference()
+    taus = {("spiral", "measured"): -0.568, ("spiral", "nominal"): 5.008,
+            ("epi", "measured"): -0.408, ("epi", "nominal"): 1.675}
+    for (name, traj), tau in taus.items():
+        assert abs(float(r[f"{name}_tau_{traj}"]) - tau) < 1e-3
+        img = r[f"{name}_golden_{traj}"].astype(np.float32)
+        assert img.shape == (200, 200)
+        assert img.min() >= 0.0 and img.max() <= 1.0 and img.std() > 0.1
+    assert float(r["spiral_dt_adc"]) == 1e-6 and abs(float(r["epi_dt_adc"]) - 1.4e-6) < 1e-12
+    # the four images are distinct reconstructions
+    ims = [r[k].astype(np.float64).ravel() for k in r.files if "golden" in k]
+    for i in range(len(ims)):
+        for j in range(i + 1, len(ims)):
+            assert fixture7t.corr(ims[i], ims[j]) < 0.99

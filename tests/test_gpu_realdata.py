@@ -40,7 +40,10 @@ rendered, corr = fixture7t.rendered, fixture7t.corr
 def load(name, trajectory):
     p = fixture7t.load(name, trajectory)
     if p is None:
-        pytest.skip(f"7t_{name}.npz not generated: tests/golden/make_7t_fixture.py needs the reference's demo data")
+        pytest.fail(f"tests/golden/7t_{name}.npz is missing: it holds the reference's 7 T demo data "
+                    f"(demo/7T_2D_*.mat), which is not committed; generate it with "
+                    f"`python tests/golden/make_7t_fixture.py REFERENCE_ROOT` (build() does this when the "
+                    f"reference is present) before running the GPU tests", pytrace=False)
     return p
 
 
@@ -110,15 +113,16 @@ def test_find_delay_recovers_the_delays_stored_with_the_data(name, trajectory, g
     nX, nY = p["grid"].nX, p["grid"].nY
     csm = p["csm"].reshape(nX, nY, -1, order="F")
     b0 = p["fieldmap"].reshape(nX, nY, order="F")
-    dt_adc = float(z["dt_adc"])
+    r = fixture7t.reference()
+    dt_adc = float(r[f"{name}_dt_adc"])
     if trajectory == "measured":
         data = (z["kdata"] / np.exp(2j * np.pi * z["k0_adc"])[:, None]).astype(np.complex64)
         field, start, dt, terms, tau_ref = (z["field_measured"], z["start_measured"], z["dt_measured"], "1111",
-                                            float(z["tau_measured"]))
+                                            float(r[f"{name}_tau_measured"]))
     else:
         data = z["kdata"]
         field, start, dt, terms, tau_ref = (z["field_nominal"], z["start_nominal"], z["dt_nominal"], "010",
-                                            float(z["tau_nominal"]))
+                                            float(r[f"{name}_tau_nominal"]))
     weight = lambda k: hoe.samplingDensity(np.asarray(k)[1:3], (nX, nY))
     tau = hoe.FindDelay(p["grid"], data, field, p["times"], np.float32(start), np.float32(dt),
                         Δτ_min=np.float32(0.005 * dt_adc), fieldmap=b0, csm=csm, recon_terms=terms, iter_max=10,
