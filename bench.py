@@ -94,17 +94,47 @@ def measured_traffic(workload):
         return None
 
 
-def cpu_baseline(cfg, n_sub, threads):
-    """Restated reference CPU Array path (oracle/cpu_ref, OpenMP) on n_sub samples: E^H E per second,
-    scaled to the full sample count (cost is linear in samples)."""
+def host_cpu():
+    """Cores this process may run on, the machine's logical CPU count and the CPU model."""
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    model = "unknown"
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return usable, os.cpu_count() or usable, model
+
+
+def cpu_baseline(cfg, threads, budget_s=30.0, n_sub=0):
+    """Restated reference CPU Array path (oracle/cpu_ref, OpenMP on `threads` cores) E^H E per
+    second. The sample count is the whole readout when 1 warm-up and 3 timed runs fit in
+    about `budget_s` seconds (probed on 512 samples), else the largest slice of the readout
+    that does (cost is linear in samples: the per-sample work is independent), scaled to the
+    full readout. Returns (seconds per full E^H E, samples timed)."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle as O
     O.lib().oracle_set_threads(threads)
-    sub = dict(cfg)
-    sub["kspha"] = cfg["kspha"][:, :n_sub]
-    sub["times"] = cfg["times"][:n_sub]
-    m = O.make_model(sub["grid"], sub["kspha"], sub["times"], sub["fieldmap"], sub["csm"], sub["mask"],
-                     concomitant=None if cfg.get("concomitant") is None else cfg["concomitant"][:, :n_sub])
+    ns_full = cfg["kspha"].shape[1]
+
+    def model(n):
+        conc = None if cfg.get("concomitant") is None else cfg["concomitant"][:, :n]
+        return O.make_model(cfg["grid"], cfg["kspha"][:, :n], cfg["times"][:n], cfg["fieldmap"], cfg["csm"],
+                            cfg["mask"], concomitant=conc)
+    if not n_sub:
+        probe = min(512, ns_full)
+        mp = model(probe)
+        O.cpu_ref_normal(mp, cfg["image"])
+        t0 = time.perf_counter()
+        O.cpu_ref_normal(mp, cfg["image"])
+        per_sample = (time.perf_counter() - t0) / probe
+        n_sub = int(min(ns_full, max(probe, budget_s / (4.0 * per_sample))))
+    m = model(n_sub)
     O.cpu_ref_normal(m, cfg["image"])           # warm-up
     ts = []
     for _ in range(3):
@@ -112,8 +142,75 @@ def cpu_baseline(cfg, n_sub, threads):
         O.cpu_ref_normal(m, cfg["image"])
         ts.append(time.perf_counter() - t0)
     t = float(np.median(ts))
-    ns_full = cfg["kspha"].shape[1]
-    return t * ns_full / n_sub, m
+    return t * ns_full / n_sub, n_sub
+
+
+def cpu_object(t_full, n_sub, ns, threads, machine, what):
+    usable, logical, model = machine
+    return {"value": round(1.0 / t_full, 6), "unit": "E^H E applies/s", "cores": threads, "kind": "port",
+            "usable_cores": usable, "logical_cpus": logical, "cpu_model": model,
+            "sample": (f"oracle cpu_ref (restated array_operator.jl:241-348, OpenMP, {threads} threads) E^H E of "
+                       f"{what}: median of 3 after 1 warm-up on "
+                       + ("the whole readout" if n_sub == ns else
+                          f"the first {n_sub} of {ns} samples, scaled x{ns / n_sub:.1f} (extrapolated)"))}
+
+
+def workload_entry(w, steps, with_cpu, with_grad, threads, machine):
+    """E^H E rate of the mixed FP16 and FP32 backends on workload w, their contraction
+    kernels' roofline fractions, the gradient (workload 3 convention) when asked, and the CPU
+    baseline on a bounded sample."""
+    import torch
+    import hoe
+    from hoe import synth
+    cfg = synth.make(w)
+    ns, nc = cfg["kspha"].shape[1], cfg["csm"].shape[1]
+    dev = torch.device("cuda:0")
+    x_dev = torch.from_numpy(cfg["image"]).to(dev)
+    out = torch.empty(cfg["grid"].n, dtype=torch.complex64, device=dev)
+    res = {"workload": synth.describe(w), "n_sam": ns, "n_cha": nc}
+    k_big = w == 4
+    with make_op(cfg, "mixed_fp16") as op:
+        res["n_vox"] = nv = op.nVox
+        flops = 8.0 * ns * nv * nc
+        for be, k in (("mixed_fp16", steps), ("direct", max(2, steps // 4))):
+            op.set_backend(be)
+            t, kf, ka = time_normal(op, x_dev, out, k, 1 if k_big else 3)
+            kf_avg, ka_avg = float(np.mean(kf)), float(np.mean(ka))
+            tflops = flops / (0.5 * (kf_avg + ka_avg) * 1e-3) / 1e12
+            peak = PEAK_F16_MFMA_TFLOPS if be == "mixed_fp16" else PEAK_FP32_TFLOPS
+            res[be] = {"value": round(k / t, 4), "unit": "E^H E applies/s", "ms_per_step": round(1e3 * t / k, 3),
+                       "steps": k, "fwd_kernel_ms": round(kf_avg, 4), "adj_kernel_ms": round(ka_avg, 4),
+                       "tflops": round(tflops, 2), "frac_of_peak": round(tflops / peak, 4), "peak_tflops": peak}
+            log(f"workload {w} {be}: {k / t:.3f} E^H E/s, kernels {kf_avg:.3f} / {ka_avg:.3f} ms")
+        res["mixed_over_fp32"] = round(res["mixed_fp16"]["value"] / res["direct"]["value"], 3)
+        if with_grad:
+            kp = synth.perturb(cfg)
+            with make_op(dict(cfg, kspha=kp), "mixed_fp16") as gop:
+                gop.set_coefficients(None, 3e-6)
+                y_obs = gop * x_dev
+            torch.cuda.synchronize()
+            grad = {}
+            for be in ("mixed_fp16", "direct"):
+                op.set_backend(be)
+                gts = []
+                for i in range(4):          # 1 warm-up call, then 3 timed
+                    t0 = time.perf_counter()
+                    g, gtau, loss = op.grad(x_dev, y_obs)
+                    torch.cuda.synchronize()
+                    if i:
+                        gts.append(time.perf_counter() - t0)
+                grad[be] = {"ms": round(1e3 * float(np.median(gts)), 3), "kernel_ms": round(op.last_kernel_ms(0), 3),
+                            "dL_dtau": gtau}
+                log(f"workload {w} grad {be}: {grad[be]['ms']:.1f} ms (kernel {grad[be]['kernel_ms']:.1f} ms)")
+            res["grad"] = grad
+    if with_cpu:
+        # the 3-D volume: a bounded slice of one stack partition (its samples cost as much as any)
+        t_cpu, n_sub = cpu_baseline(cfg, threads, budget_s=20.0 if k_big else 15.0)
+        what = f"workload {w}" + (" (one partition's first samples, x the readout)" if k_big else "")
+        res["cpu_baseline"] = cpu_object(t_cpu, n_sub, ns, threads, machine, what)
+        res["gpu_over_cpu"] = round(res["mixed_fp16"]["value"] * t_cpu, 1)
+        log(f"workload {w} cpu: {1.0 / t_cpu:.6f} applies/s on {threads} threads ({n_sub} samples)")
+    return res
 
 
 def real_data(steps):
@@ -187,6 +284,7 @@ def main():
     ap.add_argument("--no-cg", action="store_true")
     ap.add_argument("--cpu-samples", type=int, default=0)
     ap.add_argument("--no-realdata", action="store_true")
+    ap.add_argument("--no-workloads", action="store_true", help="skip workloads 0, 2 and 4")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
@@ -310,15 +408,21 @@ def main():
     }
     kf_d_avg, ka_d_avg = float(np.mean(kf_d)), float(np.mean(ka_d))
     direct_tf = flops_launch / (0.5 * (kf_d_avg + ka_d_avg) * 1e-3) / 1e12
+    machine = host_cpu()
+    threads = machine[0]             # every core this process may use
     cpu = None
     if not args.no_cpu:
-        threads = min(os.cpu_count() or 1, 16)
-        n_sub = args.cpu_samples or max(256, min(ns, 4096 * threads // 16))
-        t_cpu, _ = cpu_baseline(cfg, n_sub, threads)
-        cpu = {"value": round(1.0 / t_cpu, 5), "unit": "E^H E applies/s", "cores": threads, "kind": "port",
-               "sample": f"oracle cpu_ref (restated array_operator.jl:241-348, OpenMP) E^H E on {n_sub} of {ns} "
-                         f"samples, full 40000 voxels x {nc} coils, median of 3, scaled x{ns / n_sub:.1f}"}
-        log(f"cpu baseline: {1.0 / t_cpu:.4f} applies/s on {threads} threads")
+        t_cpu, n_sub = cpu_baseline(cfg, threads, budget_s=30.0, n_sub=args.cpu_samples)
+        cpu = cpu_object(t_cpu, n_sub, ns, threads, machine, f"workload {args.workload} ({nv} voxels x {nc} coils)")
+        log(f"cpu baseline: {1.0 / t_cpu:.4f} applies/s on {threads} threads ({n_sub} samples)")
+    workloads = None
+    if not args.no_workloads:
+        workloads = {}
+        for w in (0, 2, 4):
+            if w == args.workload:
+                continue
+            workloads[str(w)] = workload_entry(w, 3 if w == 4 else max(10, args.steps // 2), not args.no_cpu,
+                                               w == 4 and not args.no_grad, threads, machine)
     line = {
         "metric": METRIC, "value": round(value, 3), "unit": "E^H E applies/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * t_mixed / args.steps, 4),
@@ -337,6 +441,7 @@ def main():
         "grad_ms": round(grad_ms, 3), "grad_kernel_ms": round(grad_kernel_ms, 3), "grad": grad,
         "cg30": cg,
         "realdata": realdata,
+        "workloads": workloads,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
