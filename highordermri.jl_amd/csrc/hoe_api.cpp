@@ -221,12 +221,12 @@ int check_open(Handle *h) {
 // hence its occupancy and the stream-K slot count, depends on it); on failure dt_on goes
 // back to prev_dt and the schedules are rebuilt for it.
 int reschedule_direct(Handle *h, bool prev_dt) {
-    h->sched_f[HOE_BACKEND_DIRECT].built = false;
+    hoe::drop_passes(h->sched_f[HOE_BACKEND_DIRECT]);
     hipError_t e = hoe::prepare_backend(h, HOE_BACKEND_DIRECT);
     if (e == hipSuccess) return HOE_OK;
     const std::string msg = std::string("schedule setup failed: ") + hipGetErrorString(e);
     h->dt_on = prev_dt;
-    h->sched_f[HOE_BACKEND_DIRECT].built = false;
+    hoe::drop_passes(h->sched_f[HOE_BACKEND_DIRECT]);
     (void)hoe::prepare_backend(h, HOE_BACKEND_DIRECT);
     return set_err(h, e == hipErrorOutOfMemory ? HOE_ENOMEM : HOE_EHIP, msg);
 }
@@ -304,8 +304,11 @@ void release(Handle *h) {
                               &h->gbfrag,   &h->gbscale, &h->part_gm, &h->ghbuf, &h->growmap, &h->dtrec,
                               &h->traces_d, &h->times_d,  &h->act_d};
     for (auto *b : bufs) dfree(*b);
-    hoe::Sched *scheds[] = {&h->sched_f[0], &h->sched_f[1], &h->sched_f[2], &h->sched_a[0],
-                            &h->sched_a[1], &h->sched_a[2], &h->sched_g,    &h->sched_gm};
+    for (int b = 0; b < 4; ++b) {
+        hoe::drop_passes(h->sched_f[b]);
+        hoe::drop_passes(h->sched_a[b]);
+    }
+    hoe::Sched *scheds[] = {&h->sched_g, &h->sched_gm};
     for (auto *sc : scheds) {
         dfree(sc->seg);
         dfree(sc->chunk_off);
@@ -789,8 +792,8 @@ int hoe_forward(hoe_handle *hh, const void *x, void *y, int flags) {
     const void *dx;
     if ((rc = stage_in(h, flags, x, h->io_x, (size_t)h->cbytes * h->n_grid, &dx))) return rc;
     void *dy = (flags & HOE_DEVICE_PTRS) ? y : h->io_y.p;
-    HOE_TRY_HIP(h, hoe::run_forward_contract(h, dx, !(flags & HOE_ASYNC)));
-    HOE_TRY_HIP(h, hoe::launch_reduce_fwd_user(h, dy, prec_scale(h, 1.0 / std::sqrt((double)h->n_vox))));
+    HOE_TRY_HIP(h, hoe::run_forward_user(h, dx, dy, prec_scale(h, 1.0 / std::sqrt((double)h->n_vox)),
+                                         !(flags & HOE_ASYNC)));
     if ((rc = finish(h, flags, y, dy, (size_t)h->cbytes * h->n_sam * h->n_cha))) return rc;
     if (!(flags & HOE_ASYNC)) record_times(h, true, false);
     return HOE_OK;
@@ -904,6 +907,12 @@ inline size_t align256(size_t b) { return (b + 255) / 256 * 256; }
 
 // Bytes of every workspace of a normal-operator application on the handle's backend,
 // in the order they are carved out of the scratch array.
+size_t max_segs(const std::vector<hoe::Sched> &passes) {
+    size_t m = 0;
+    for (const hoe::Sched &sc : passes) m = std::max(m, (size_t)sc.n_seg);
+    return m;
+}
+
 void noalloc_plan(const Handle *h, size_t sizes[9]) {
     const bool mixed = h->backend != HOE_BACKEND_DIRECT;
     const size_t n_gb = (size_t)hoe::mixed_groups(h) * hoe::mixed_nb(h);
@@ -911,8 +920,8 @@ void noalloc_plan(const Handle *h, size_t sizes[9]) {
     sizes[1] = sizeof(float2) * (size_t)h->nv_pad;                                // xm
     sizes[2] = sizeof(float2) * (size_t)h->nv_pad;                                // cg_vec: masked output
     sizes[3] = sizeof(float2) * (size_t)h->ns_pad * h->ncs;                       // ybuf
-    sizes[4] = sizeof(float2) * 256 * (size_t)h->ncs * h->sched_f[h->backend].n_seg;   // part_f
-    sizes[5] = sizeof(float2) * 256 * (size_t)h->ncs * h->sched_a[h->backend].n_seg;   // part_a
+    sizes[4] = sizeof(float2) * 256 * (size_t)h->ncs * max_segs(h->sched_f[h->backend]);   // part_f
+    sizes[5] = sizeof(float2) * 256 * (size_t)h->ncs * max_segs(h->sched_a[h->backend]);   // part_a
     sizes[6] = mixed ? n_gb * (size_t)(h->nv_pad / 8) * 64 * 64 : 0;             // zfrag
     sizes[7] = mixed ? n_gb * (size_t)(h->ns_pad / 8) * 64 * 64 : 0;             // yfrag
     sizes[8] = mixed ? 0 : sizeof(float2) * (size_t)h->nv_pad * h->ncs;          // zbuf
@@ -1056,8 +1065,7 @@ int hoe_grad(hoe_handle *hh, const void *x, const void *y, float *g_h, double *g
     if ((rc = stage_in(h, flags, x, h->io_x, sizeof(float2) * h->n_grid, &dx))) return rc;
     if ((rc = stage_in(h, flags, y, h->io_y, sizeof(float2) * h->n_sam * h->n_cha, &dy))) return rc;
     // residual r = E x - y into ybuf (coil fastest)
-    HOE_TRY_HIP(h, hoe::run_forward_contract(h, dx, false));
-    HOE_TRY_HIP(h, hoe::launch_reduce_fwd_ybuf(h, prec_scale(h, 1.0 / std::sqrt((double)h->n_vox)), false, dy));
+    HOE_TRY_HIP(h, hoe::run_forward_ybuf(h, dx, prec_scale(h, 1.0 / std::sqrt((double)h->n_vox)), false, dy, false));
     double *loss_dev = reinterpret_cast<double *>(h->scal.p) + 16;
     HOE_TRY_HIP(h, hoe::launch_norm2(h, h->ybuf.p, h->ns_pad * h->ncs, loss_dev));
     HOE_TRY_HIP(h, hoe::run_grad_contract(h, true));

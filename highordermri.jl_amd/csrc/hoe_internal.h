@@ -23,10 +23,12 @@ struct DeviceBuf {
     size_t bytes = 0;
 };
 
-// Stream-K schedule of one contraction (see build_schedule in hoe_kernels.hip).
+// Stream-K schedule of one contraction (see build_schedule in hoe_kernels.hip), or of one
+// pass of it: a pass covers the stationary tiles [t_lo, t_hi) over the whole K range.
 struct Sched {
     bool built = false;
     int tiles = 0;      // 256-row tiles of the stationary index
+    int t_lo = 0, t_hi = 0;   // tiles of this pass
     int n_ks = 0;       // K units (8 streamed rows each)
     int cpx = 0;        // chunks per XCD part
     int n_chunks = 0;   // workgroups launched (8 * cpx)
@@ -121,8 +123,10 @@ struct Handle {
     std::vector<double> fwd_k_w;      // stream-K work weight of each K unit (8 voxels) in the Horner forward
     int gb_type = -1;     // backend whose element type gbfrag holds
 
-    // schedules per backend (index = hoe_backend) and for the gradient kernel
-    Sched sched_f[4], sched_a[4], sched_g, sched_gm;
+    // schedules per backend (index = hoe_backend), as passes over tile ranges whose
+    // partial sums share part_f / part_a, and for the gradient kernels
+    std::vector<Sched> sched_f[4], sched_a[4];
+    Sched sched_g, sched_gm;
     int occ_f[4] = {0, 0, 0, 0}, occ_a[4] = {0, 0, 0, 0};   // resident workgroups per CU
 
     // timing of the last call (ms)
@@ -138,6 +142,8 @@ hipError_t build_schedule(Sched &sc, int tiles, int n_ks, int slots, int kmax,
                           const std::vector<double> *tile_w = nullptr, const std::vector<double> *k_w = nullptr);
 // Build the forward/adjoint schedules of a backend and size the partial buffers.
 hipError_t prepare_backend(Handle *h, int backend);
+// Free the device arrays of the passes and empty the list.
+void drop_passes(std::vector<Sched> &passes);
 hipError_t prepare_grad(Handle *h);
 // True when the current backend computes the gradient on the matrix cores.
 bool grad_on_mfma(const Handle *h);
@@ -147,16 +153,17 @@ bool grad_on_mfma(const Handle *h);
 // All run on h->stream and return hipSuccess or the first HIP error.
 hipError_t launch_gather(Handle *h, const void *x_full, void *xm);
 hipError_t launch_scatter(Handle *h, const void *xm, void *x_full, double scale);
-// Forward: x_full (device, n_grid complex) -> part_f.  Uses the current backend.
-hipError_t run_forward_contract(Handle *h, const void *x_full, bool time_it);
-// Reduce part_f into the user signal layout (complex [c][s]) with scale.
-hipError_t launch_reduce_fwd_user(Handle *h, void *y_out, double scale);
-// Reduce part_f into ybuf (coil fastest) with scale and optional weights dvec
+// Forward of x_full (device, n_grid complex) into the user signal layout (complex [c][s])
+// with scale: contraction into part_f and its reduction, pass by pass.
+hipError_t run_forward_user(Handle *h, const void *x_full, void *y_out, double scale, bool time_it);
+// Forward of x_full into ybuf (coil fastest) with scale, optional weights dvec
 // (weights == true) and optional subtraction of data (y_data != nullptr, user layout).
-hipError_t launch_reduce_fwd_ybuf(Handle *h, double scale, bool weights, const void *y_data);
+hipError_t run_forward_ybuf(Handle *h, const void *x_full, double scale, bool weights, const void *y_data,
+                            bool time_it);
 // Load a user signal (complex [c][s]) into ybuf (coil fastest), optionally weighted.
 hipError_t launch_load_ybuf(Handle *h, const void *y_user, bool weights);
-// Adjoint contraction from ybuf -> part_a, then coil reduction into xm (masked) with scale.
+// Adjoint contraction from ybuf -> part_a and coil reduction into xm (masked) with scale,
+// pass by pass.
 hipError_t run_adjoint_contract(Handle *h, double scale, void *xm_out, bool time_it);
 // Normal operator on masked vectors: out_m = scale * E^H D E x_m (+ lambda x_m).
 // have_max: slot 0 already holds max|xm_in| and the other slots are clear (launch from a

@@ -23,6 +23,7 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <cstdlib>
 #include <type_traits>
 
 namespace hoe {
@@ -48,6 +49,25 @@ namespace hoe {
 // 9.6e-6 for a random image, so its segments stay uncut.
 #ifndef HOE_MFMA_ADJ_KMAX
 #define HOE_MFMA_ADJ_KMAX 1024
+#endif
+// Largest partial-sum buffer of one contraction, in bytes. A schedule whose segments
+// would need more runs as several passes over disjoint tile ranges (each scheduled over
+// the whole device and reduced before the next starts), which share one buffer of the
+// largest pass's size. A segment holds 256 x ncs complex values (64 KiB at 32 FP32
+// coils), and an MFMA adjoint tile has 8 * ceil(nSam / 8 / 8 / 1024) segments, so one
+// schedule of a 96 x 96 x 48 grid at 384 000 samples would hold 5.4 GB. The environment
+// variable HOE_PART_BUDGET_MIB (a positive integer, read when a backend's schedules are
+// built) overrides it.
+#ifndef HOE_PART_BUDGET
+#define HOE_PART_BUDGET ((int64_t)512 << 20)
+#endif
+// K-part length (units) beyond which a schedule gives its chunks whole tiles first (see
+// plan_schedule): 1 024 units of MFMA fragments are 4 MiB, one XCD's L2. On a 3-D
+// workload (96 x 96 x 48, 384 000 samples) its 1 728 adjoint tiles split into 18 whole
+// tiles per chunk, and the adjoint fetched 20 GB past L2 per launch; cut into 133-tile
+// passes whose chunks start at scattered K offsets, 209 GB.
+#ifndef HOE_DP_MIN_KX
+#define HOE_DP_MIN_KX 1024
 #endif
 
 typedef _Float16 h8_t __attribute__((ext_vector_type(8)));
@@ -1181,9 +1201,10 @@ template <typename R>
 __global__ __launch_bounds__(256) void k_reduce_fwd(
     const cplx_t<R> *__restrict__ part, const int *__restrict__ tile_off, const int *__restrict__ tile_list, int ncs,
     int ns, int nc, R scale, const unsigned *__restrict__ scale_slot, float scale_mul, int mode,
-    cplx_t<R> *__restrict__ out, const R *__restrict__ w, const cplx_t<R> *__restrict__ y_data, unsigned *maxslot) {
+    cplx_t<R> *__restrict__ out, const R *__restrict__ w, const cplx_t<R> *__restrict__ y_data, unsigned *maxslot,
+    int s_base) {
     __shared__ cplx_t<R> tileb[32][33];
-    const int s0 = blockIdx.x * 32, c0 = blockIdx.y * 32;
+    const int s0 = s_base + blockIdx.x * 32, c0 = blockIdx.y * 32;
     const int tc = threadIdx.x & 31, tr = threadIdx.x >> 5;   // 8 row groups
     const int tile = s0 >> 8;
     const int *list = tile_list + tile_off[tile];
@@ -1309,8 +1330,8 @@ __global__ __launch_bounds__(256) void k_reduce_adj(
     const cplx_t<R> *__restrict__ part, const int *__restrict__ tile_off, const int *__restrict__ tile_list,
     int nv_pad, int ncs, int nc, const cplx_t<R> *__restrict__ csm, int nv, R scale,
     const unsigned *__restrict__ scale_slot, cplx_t<R> *__restrict__ xm_out, R lambda,
-    const cplx_t<R> *__restrict__ x_in) {
-    const int v = (blockIdx.x * blockDim.x + threadIdx.x) >> 5;   // half-wave id = voxel
+    const cplx_t<R> *__restrict__ x_in, int v_base) {
+    const int v = v_base + ((blockIdx.x * blockDim.x + threadIdx.x) >> 5);   // half-wave id = voxel
     const int cl = threadIdx.x & 31;
     R sc = scale;
     if constexpr (std::is_same<R, float>::value) {
@@ -1760,20 +1781,37 @@ static hipError_t upload_ints(DeviceBuf &b, const std::vector<int> &v) {
     return hipMemcpy(b.p, v.data(), v.size() * sizeof(int), hipMemcpyHostToDevice);
 }
 
-// Cut tiles x n_ks units into 8 K-parts (one per XCD) and each part into
-// `slots/8` equal chunks of consecutive (tile, k) units; record the segments.
-// kmax > 0 cuts a chunk's run of one tile into near-equal segments of at most
-// kmax units. tile_w (optional) weights the units of each tile, k_w (optional) those
-// of each K index; the K parts and the chunks of a part then hold equal weighted work
-// instead of equal unit counts.
-hipError_t build_schedule(Sched &sc, int tiles, int n_ks, int slots, int kmax,
-                          const std::vector<double> *tile_w, const std::vector<double> *k_w) {
+// Host side of a schedule: the arrays build_schedule uploads.
+struct SchedPlan {
+    int cpx = 0, n_chunks = 0, n_seg = 0;
+    std::vector<int> seg, chunk_off, tile_off, tile_list;
+};
+
+// Cut tiles [t_lo, t_hi) x n_ks units into 8 K-parts (one per XCD) and each part into
+// cpx = slots/8 chunks of equal work; record the segments. When a K part is longer
+// than HOE_DP_MIN_KX units and the tiles carry no weights, each chunk first takes
+// floor(nt / cpx) whole tiles (the part's full K range each): all chunks of an XCD
+// then stream the same K rows at the same time and share them in its L2 (a shorter
+// part fits in the L2 whatever the order). The tiles left over, and all tiles
+// otherwise, are cut stream-K: into cpx equal runs of consecutive (tile, k) units,
+// appended to the chunks.
+// kmax > 0 cuts a chunk's run of one tile into near-equal segments of at most kmax
+// units. k_w (optional) weights the units of each K index; the K parts and the
+// stream-K runs then hold equal weighted work instead of equal unit counts. tile_off
+// spans all `tiles` (tiles outside the range have no segments).
+static void plan_schedule(SchedPlan &sc, int tiles, int n_ks, int slots, int kmax, const std::vector<double> *tile_w,
+                          const std::vector<double> *k_w, int t_lo, int t_hi) {
     const int X = 8;
-    sc.tiles = tiles;
-    sc.n_ks = n_ks;
     sc.cpx = std::max(1, slots / X);
+    const bool tweighted = tile_w && (int)tile_w->size() == tiles;
+    const bool dp = !tweighted && n_ks / X > HOE_DP_MIN_KX;
+    const int m_dp = dp ? (t_hi - t_lo) / sc.cpx : 0;   // whole tiles per chunk
+    const int t_sk = t_lo + m_dp * sc.cpx;                      // first stream-K tile
+    const int nt = t_hi - t_sk;
     sc.n_chunks = sc.cpx * X;
-    std::vector<int> seg, chunk_off(sc.n_chunks + 1, 0);
+    std::vector<int> &seg = sc.seg, &chunk_off = sc.chunk_off;
+    seg.clear();
+    chunk_off.assign((size_t)sc.n_chunks + 1, 0);
     const bool kw = k_w && (int)k_w->size() == n_ks;
     std::vector<double> kcum;   // prefix sums of the K weights
     std::vector<int> kb(X + 1);
@@ -1791,24 +1829,25 @@ hipError_t build_schedule(Sched &sc, int tiles, int n_ks, int slots, int kmax,
     for (int x = 0; x < X; ++x) {
         const int K0 = kb[x], K1 = kb[x + 1];
         const int Kx = K1 - K0;
-        const int64_t units = (int64_t)tiles * Kx;
-        // first unit of each chunk (ub[cpx] = units)
+        const int64_t units = (int64_t)nt * Kx;
+        // first stream-K unit of each chunk (ub[cpx] = units), counted from tile t_sk
         std::vector<int64_t> ub((size_t)sc.cpx + 1, units);
-        if (tile_w && (int)tile_w->size() == tiles) {
-            std::vector<double> cum((size_t)tiles + 1, 0.0);
-            for (int t = 0; t < tiles; ++t) cum[(size_t)t + 1] = cum[(size_t)t] + (*tile_w)[(size_t)t] * Kx;
+        if (tweighted) {
+            std::vector<double> cum((size_t)nt + 1, 0.0);
+            for (int t = 0; t < nt; ++t) cum[(size_t)t + 1] = cum[(size_t)t] + (*tile_w)[(size_t)(t_sk + t)] * Kx;
             int t = 0;
             for (int c = 0; c < sc.cpx; ++c) {
-                const double target = cum[(size_t)tiles] * c / sc.cpx;
-                while (t + 1 < tiles && cum[(size_t)t + 1] <= target) ++t;
-                const int64_t k = std::min<int64_t>(Kx, (int64_t)std::ceil((target - cum[(size_t)t]) / (*tile_w)[(size_t)t]));
+                const double target = cum[(size_t)nt] * c / sc.cpx;
+                while (t + 1 < nt && cum[(size_t)t + 1] <= target) ++t;
+                const int64_t k = std::min<int64_t>(
+                    Kx, (int64_t)std::ceil((target - cum[(size_t)t]) / (*tile_w)[(size_t)(t_sk + t)]));
                 ub[(size_t)c] = std::max<int64_t>(c ? ub[(size_t)c - 1] : 0, (int64_t)t * Kx + std::max<int64_t>(0, k));
             }
-        } else if (kw && Kx > 0) {
+        } else if (kw && Kx > 0 && nt > 0) {
             const double wt = kcum[(size_t)K1] - kcum[(size_t)K0];   // weighted work of one tile in the part
             for (int c = 0; c < sc.cpx; ++c) {
-                const double target = wt * tiles * c / sc.cpx;
-                const int t = std::min(tiles - 1, (int)(target / wt));
+                const double target = wt * nt * c / sc.cpx;
+                const int t = std::min(nt - 1, (int)(target / wt));
                 const double rem = kcum[(size_t)K0] + (target - wt * t);
                 const int k = (int)(std::lower_bound(kcum.begin() + K0, kcum.begin() + K1 + 1, rem) - kcum.begin());
                 ub[(size_t)c] = std::max<int64_t>(c ? ub[(size_t)c - 1] : 0, (int64_t)t * Kx + (k - K0));
@@ -1817,38 +1856,156 @@ hipError_t build_schedule(Sched &sc, int tiles, int n_ks, int slots, int kmax,
             const int64_t U = (units + sc.cpx - 1) / sc.cpx;
             for (int c = 0; c < sc.cpx; ++c) ub[(size_t)c] = std::min(units, (int64_t)c * U);
         }
+        // segments of tile t over K0 + [kk, kk + len), cut at kmax
+        auto run = [&](int t, int kk, int len) {
+            const int pieces = kmax > 0 ? (len + kmax - 1) / kmax : 1;
+            for (int p = 0; p < pieces; ++p) {
+                seg.push_back(t);
+                seg.push_back(K0 + kk + (int)((int64_t)len * p / pieces));
+                seg.push_back(K0 + kk + (int)((int64_t)len * (p + 1) / pieces));
+                seg.push_back(0);
+            }
+        };
         for (int c = 0; c < sc.cpx; ++c) {
             chunk_off[x * sc.cpx + c] = (int)(seg.size() / 4);
+            if (Kx > 0)
+                for (int j = 0; j < m_dp; ++j) run(t_lo + c * m_dp + j, 0, Kx);
             int64_t u = ub[(size_t)c];
             const int64_t u1 = ub[(size_t)c + 1];
             while (u < u1) {
                 const int t = (int)(u / Kx), kk = (int)(u % Kx);
                 const int len = (int)std::min<int64_t>(u1 - u, Kx - kk);
-                const int pieces = kmax > 0 ? (len + kmax - 1) / kmax : 1;
-                for (int p = 0; p < pieces; ++p) {
-                    seg.push_back(t);
-                    seg.push_back(K0 + kk + (int)((int64_t)len * p / pieces));
-                    seg.push_back(K0 + kk + (int)((int64_t)len * (p + 1) / pieces));
-                    seg.push_back(0);
-                }
+                run(t_sk + t, kk, len);
                 u += len;
             }
         }
     }
     sc.n_seg = (int)(seg.size() / 4);
     chunk_off[sc.n_chunks] = sc.n_seg;
-    std::vector<int> tile_off(tiles + 1, 0), tile_list(sc.n_seg);
+    std::vector<int> &tile_off = sc.tile_off, &tile_list = sc.tile_list;
+    tile_off.assign((size_t)tiles + 1, 0);
+    tile_list.assign((size_t)sc.n_seg, 0);
     for (int i = 0; i < sc.n_seg; ++i) tile_off[seg[4 * i] + 1]++;
     for (int t = 0; t < tiles; ++t) tile_off[t + 1] += tile_off[t];
     std::vector<int> fill(tile_off.begin(), tile_off.end() - 1);
     for (int i = 0; i < sc.n_seg; ++i) tile_list[fill[seg[4 * i]]++] = i;
+}
+
+static hipError_t upload_plan(Sched &sc, const SchedPlan &pl, int tiles, int n_ks, int t_lo, int t_hi) {
+    sc.tiles = tiles;
+    sc.n_ks = n_ks;
+    sc.t_lo = t_lo;
+    sc.t_hi = t_hi;
+    sc.cpx = pl.cpx;
+    sc.n_chunks = pl.n_chunks;
+    sc.n_seg = pl.n_seg;
     hipError_t e;
-    if ((e = upload_ints(sc.seg, seg)) != hipSuccess) return e;
-    if ((e = upload_ints(sc.chunk_off, chunk_off)) != hipSuccess) return e;
-    if ((e = upload_ints(sc.tile_off, tile_off)) != hipSuccess) return e;
-    if ((e = upload_ints(sc.tile_list, tile_list)) != hipSuccess) return e;
+    if ((e = upload_ints(sc.seg, pl.seg)) != hipSuccess) return e;
+    if ((e = upload_ints(sc.chunk_off, pl.chunk_off)) != hipSuccess) return e;
+    if ((e = upload_ints(sc.tile_off, pl.tile_off)) != hipSuccess) return e;
+    if ((e = upload_ints(sc.tile_list, pl.tile_list)) != hipSuccess) return e;
     sc.built = true;
     return hipSuccess;
+}
+
+// One schedule over all tiles.
+hipError_t build_schedule(Sched &sc, int tiles, int n_ks, int slots, int kmax,
+                          const std::vector<double> *tile_w, const std::vector<double> *k_w) {
+    SchedPlan pl;
+    plan_schedule(pl, tiles, n_ks, slots, kmax, tile_w, k_w, 0, tiles);
+    return upload_plan(sc, pl, tiles, n_ks, 0, tiles);
+}
+
+void drop_passes(std::vector<Sched> &passes) {
+    for (Sched &sc : passes)
+        for (DeviceBuf *b : {&sc.seg, &sc.chunk_off, &sc.tile_off, &sc.tile_list}) {
+            if (b->p) (void)hipFree(b->p);
+            b->p = nullptr;
+            b->bytes = 0;
+        }
+    passes.clear();
+}
+
+// Schedules of one contraction as passes over contiguous tile ranges, as few as keep
+// each pass's partial sums (n_seg * seg_bytes) within `budget` (one tile per pass at
+// most). Every pass is scheduled over `slots` workgroups, so each still fills the
+// device; the passes run one after another and reuse one partial-sum buffer. When
+// plan_schedule gives chunks whole tiles, the passes are made of whole blocks of cpx
+// tiles (the tiles left over join the last pass), so that every pass keeps that
+// order; otherwise, or when a one-block pass is still too large, the ranges hold
+// near-equal (weighted) work. tb receives the P + 1 tile bounds.
+static void plan_passes(std::vector<SchedPlan> &plans, std::vector<int> &tb, int tiles, int n_ks, int slots,
+                        int kmax, const std::vector<double> *tile_w, const std::vector<double> *k_w,
+                        int64_t seg_bytes, int64_t budget) {
+    const bool tw = tile_w && (int)tile_w->size() == tiles;
+    auto try_bounds = [&]() {
+        const int P = (int)tb.size() - 1;
+        plans.assign((size_t)P, SchedPlan());
+        int64_t worst = 0;
+        for (int p = 0; p < P; ++p) {
+            plan_schedule(plans[(size_t)p], tiles, n_ks, slots, kmax, tile_w, k_w, tb[(size_t)p], tb[(size_t)p + 1]);
+            worst = std::max(worst, (int64_t)plans[(size_t)p].n_seg * seg_bytes);
+        }
+        return worst;
+    };
+    const int cpx = std::max(1, slots / 8);
+    const int nblk = (tw || n_ks / 8 <= HOE_DP_MIN_KX) ? 0 : tiles / cpx;
+    int last_bp = 0;
+    for (int P = 1; P <= nblk; ++P) {
+        const int bp = (nblk + P - 1) / P;   // most blocks in one pass
+        if (bp == last_bp) continue;
+        last_bp = bp;
+        tb.assign((size_t)P + 1, tiles);
+        for (int p = 0; p < P; ++p) tb[(size_t)p] = cpx * (int)((int64_t)nblk * p / P);
+        if (try_bounds() <= budget) return;
+    }
+    std::vector<double> cum((size_t)tiles + 1, 0.0);
+    for (int t = 0; t < tiles; ++t) cum[(size_t)t + 1] = cum[(size_t)t] + (tw ? (*tile_w)[(size_t)t] : 1.0);
+    int P = std::min(tiles, nblk + 1);
+    for (;;) {
+        tb.assign((size_t)P + 1, tiles);
+        tb[0] = 0;
+        for (int p = 1; p < P; ++p) {
+            const double target = cum[(size_t)tiles] * p / P;
+            const int t = (int)(std::lower_bound(cum.begin(), cum.end(), target) - cum.begin());
+            tb[(size_t)p] = std::min(tiles - (P - p), std::max(tb[(size_t)p - 1] + 1, t));
+        }
+        const int64_t worst = try_bounds();
+        if (worst <= budget || P >= tiles) return;
+        // at least 25 % more passes per step, so that planning ends after O(log tiles) steps
+        P = (int)std::min<int64_t>(tiles, std::max<int64_t>(P + std::max(1, P / 4), (P * worst + budget - 1) / budget));
+    }
+}
+
+static hipError_t build_passes(std::vector<Sched> &out, int tiles, int n_ks, int slots, int kmax,
+                               const std::vector<double> *tile_w, const std::vector<double> *k_w,
+                               int64_t seg_bytes, int64_t budget) {
+    drop_passes(out);
+    std::vector<SchedPlan> plans;
+    std::vector<int> tb;
+    plan_passes(plans, tb, tiles, n_ks, slots, kmax, tile_w, k_w, seg_bytes, budget);
+    out.resize(plans.size());
+    for (size_t p = 0; p < plans.size(); ++p) {
+        hipError_t e = upload_plan(out[p], plans[p], tiles, n_ks, tb[p], tb[p + 1]);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+static int64_t part_budget() {
+    const char *v = std::getenv("HOE_PART_BUDGET_MIB");
+    if (v && *v) {
+        char *end = nullptr;
+        const long long mib = std::strtoll(v, &end, 10);
+        if (end && *end == '\0' && mib > 0 && mib < ((long long)1 << 40)) return (int64_t)mib << 20;
+    }
+    return HOE_PART_BUDGET;
+}
+
+static int max_segments(const std::vector<Sched> &passes) {
+    int m = 0;
+    for (const Sched &sc : passes) m = std::max(m, sc.n_seg);
+    return m;
 }
 
 static hipError_t grow(DeviceBuf &b, size_t bytes) {
@@ -1869,8 +2026,8 @@ static int blocks_per_cu(K kern) {
 }
 
 hipError_t prepare_backend(Handle *h, int backend) {
-    Sched &sf = h->sched_f[backend], &sa = h->sched_a[backend];
-    if (sf.built && sa.built) return hipSuccess;
+    std::vector<Sched> &sf = h->sched_f[backend], &sa = h->sched_a[backend];
+    if (!sf.empty() && !sa.empty()) return hipSuccess;
     const bool is_mixed = backend == HOE_BACKEND_MIXED_FP16 || backend == HOE_BACKEND_MIXED_BF16;
     const int groups = is_mixed ? mixed_groups(h) : direct_batches(h);
     int occ_f = 1, occ_a = 1;
@@ -1879,30 +2036,23 @@ hipError_t prepare_backend(Handle *h, int backend) {
     const int slots_f = std::max(8, h->cu_count * occ_f / groups);
     const int slots_a = std::max(8, h->cu_count * occ_a / groups);
     hipError_t e;
-    // The cut adjoint segments cost partial-sum memory (about nVox nSam nCoil / 1024 bytes
-    // with 1 024-unit segments); beyond 8 GiB the cut length doubles until it fits.
-    int kmax_a = 0;
-    if (is_mixed) {
-        const int64_t tiles = h->nv_pad / 256, kx = (h->ns_pad / 8 + 7) / 8;
-        const int64_t seg_bytes = (int64_t)256 * h->ncs * (int64_t)sizeof(float2);
-        kmax_a = HOE_MFMA_ADJ_KMAX;
-        while (kmax_a < kx && tiles * 8 * ((kx + kmax_a - 1) / kmax_a) * seg_bytes > ((int64_t)8 << 30)) kmax_a *= 2;
-    }
+    const int kmax_a = is_mixed ? HOE_MFMA_ADJ_KMAX : 0;
+    const int64_t seg_bytes = (int64_t)256 * h->ncs * h->cbytes, budget = part_budget();
     // the Horner forward (MFMA and FP32) weighs its run-crossing K units
     const bool fweighted = h->horner && !h->dt_on &&
                            std::any_of(h->fwd_k_w.begin(), h->fwd_k_w.end(), [](double w) { return w != 1.0; });
-    if ((e = build_schedule(sf, (int)(h->ns_pad / 256), (int)(h->nv_pad / 8), slots_f, 0, nullptr,
-                            fweighted ? &h->fwd_k_w : nullptr)) != hipSuccess)
+    if ((e = build_passes(sf, (int)(h->ns_pad / 256), (int)(h->nv_pad / 8), slots_f, 0, nullptr,
+                          fweighted ? &h->fwd_k_w : nullptr, seg_bytes, budget)) != hipSuccess)
         return e;
     const bool weighted = is_mixed &&
                           std::any_of(h->adj_tile_w.begin(), h->adj_tile_w.end(), [](double w) { return w != 1.0; });
-    if ((e = build_schedule(sa, (int)(h->nv_pad / 256), (int)(h->ns_pad / 8), slots_a, kmax_a,
-                            weighted ? &h->adj_tile_w : nullptr)) != hipSuccess)
+    if ((e = build_passes(sa, (int)(h->nv_pad / 256), (int)(h->ns_pad / 8), slots_a, kmax_a,
+                          weighted ? &h->adj_tile_w : nullptr, nullptr, seg_bytes, budget)) != hipSuccess)
         return e;
     h->occ_f[backend] = occ_f;
     h->occ_a[backend] = occ_a;
-    if ((e = grow(h->part_f, (size_t)sf.n_seg * 256 * h->ncs * h->cbytes)) != hipSuccess) return e;
-    return grow(h->part_a, (size_t)sa.n_seg * 256 * h->ncs * h->cbytes);
+    if ((e = grow(h->part_f, (size_t)max_segments(sf) * seg_bytes)) != hipSuccess) return e;
+    return grow(h->part_a, (size_t)max_segments(sa) * seg_bytes);
 }
 
 hipError_t prepare_grad(Handle *h) {
@@ -1974,9 +2124,8 @@ static void launch_direct(void (*k)(const double *, int, const double *, int, co
 }
 
 template <bool FWD>
-static void launch_contract(Handle *h, const void *operand, void *part) {
+static void launch_contract(Handle *h, const Sched &sc, const void *operand, void *part) {
     const int backend = FWD ? fwd_backend(h) : h->backend;
-    const Sched &sc = FWD ? h->sched_f[backend] : h->sched_a[backend];
     const double *coef = (const double *)h->coef.p, *basis = (const double *)h->basis.p;
     const int4 *segs = (const int4 *)sc.seg.p;
     const int *coff = (const int *)sc.chunk_off.p;
@@ -2003,9 +2152,27 @@ static inline bool direct_path(int backend) {
     return backend == HOE_BACKEND_DIRECT || backend == HOE_BACKEND_FP64;
 }
 
+// Contraction passes of one direction, each followed by its reduction red(pass); the
+// events (when timed) bracket the contractions, with the reductions of all passes but
+// the last in between.
+template <bool FWD, class Red>
+static hipError_t contract_passes(Handle *h, const void *operand, void *part, bool time_it, int ev0, Red red) {
+    const std::vector<Sched> &passes = FWD ? h->sched_f[fwd_backend(h)] : h->sched_a[h->backend];
+    if (time_it) (void)hipEventRecord(h->ev[ev0], h->stream);
+    for (size_t p = 0; p < passes.size(); ++p) {
+        launch_contract<FWD>(h, passes[p], operand, part);
+        if (time_it && p + 1 == passes.size()) (void)hipEventRecord(h->ev[ev0 + 1], h->stream);
+        const hipError_t e = red(passes[p]);
+        if (e != hipSuccess) return e;
+    }
+    return hipGetLastError();
+}
+
 // Builds the forward operand from masked xm (device) and runs the forward
-// contraction into part_f. have_max: slot 0 already holds max|xm| (gathered).
-static hipError_t forward_from_xm(Handle *h, const void *xmv, bool time_it, bool have_max = false) {
+// contraction into part_f and its reduction red(pass), pass by pass. have_max: slot 0
+// already holds max|xm| (gathered).
+template <class Red>
+static hipError_t forward_from_xm(Handle *h, const void *xmv, bool time_it, bool have_max, Red red) {
     if (direct_path(fwd_backend(h))) {
         const int64_t nz = h->nv_pad * h->ncs;
         by_prec(h, [&](auto r) {
@@ -2013,9 +2180,7 @@ static hipError_t forward_from_xm(Handle *h, const void *xmv, bool time_it, bool
             k_build_z<R><<<blocks_for(nz), 256, 0, h->stream>>>((const cplx_t<R> *)h->csm.p, (const cplx_t<R> *)xmv,
                                                                 (cplx_t<R> *)h->zbuf.p, (int)h->nv_pad, h->ncs);
         });
-        if (time_it) (void)hipEventRecord(h->ev[0], h->stream);
-        launch_contract<true>(h, h->zbuf.p, h->part_f.p);
-        if (time_it) (void)hipEventRecord(h->ev[1], h->stream);
+        return contract_passes<true>(h, h->zbuf.p, h->part_f.p, time_it, 0, red);
     } else {
         const float2 *xm = (const float2 *)xmv;
         const int n_ks = (int)(h->nv_pad / 8);
@@ -2031,17 +2196,8 @@ static hipError_t forward_from_xm(Handle *h, const void *xmv, bool time_it, bool
         else
             k_build_zfrag<__bf16><<<blocks_for(nt), 256, 0, h->stream>>>(
                 (const float2 *)h->csm.p, xm, (uint4 *)h->zfrag.p, n_ks, h->ncs, n_gb, slot(h, 0), h->csm_max);
-        if (time_it) (void)hipEventRecord(h->ev[0], h->stream);
-        launch_contract<true>(h, h->zfrag.p, h->part_f.p);
-        if (time_it) (void)hipEventRecord(h->ev[1], h->stream);
+        return contract_passes<true>(h, h->zfrag.p, h->part_f.p, time_it, 0, red);
     }
-    return hipGetLastError();
-}
-
-hipError_t run_forward_contract(Handle *h, const void *x_full, bool time_it) {
-    hipError_t e = launch_gather_max(h, x_full, h->xm.p);
-    if (e != hipSuccess) return e;
-    return forward_from_xm(h, h->xm.p, time_it, true);
 }
 
 hipError_t run_normal_full(Handle *h, const void *x_full, void *xm_out, double scale, bool time_it) {
@@ -2052,32 +2208,51 @@ hipError_t run_normal_full(Handle *h, const void *x_full, void *xm_out, double s
 
 static inline bool mixed(Handle *h) { return !direct_path(h->backend); }
 
-hipError_t launch_reduce_fwd_user(Handle *h, void *y_out, double scale) {
-    const Sched &sc = h->sched_f[fwd_backend(h)];
+// Reduction of one forward pass (its 32-row blocks of samples) into the user signal
+// layout (complex [c][s]) with scale.
+static hipError_t launch_reduce_fwd_user(Handle *h, const Sched &sc, void *y_out, double scale) {
     const bool unscale = !direct_path(fwd_backend(h));
-    dim3 grid((unsigned)(h->ns_pad / 32), (unsigned)(h->ncs / 32));
+    dim3 grid((unsigned)((sc.t_hi - sc.t_lo) * 8), (unsigned)(h->ncs / 32));
     by_prec(h, [&](auto r) {
         using R = decltype(r);
         k_reduce_fwd<R><<<grid, 256, 0, h->stream>>>(
             (const cplx_t<R> *)h->part_f.p, (const int *)sc.tile_off.p, (const int *)sc.tile_list.p, h->ncs,
             (int)h->n_sam, (int)h->n_cha, (R)scale, unscale ? slot(h, 0) : nullptr, h->csm_max, 0,
-            (cplx_t<R> *)y_out, nullptr, nullptr, nullptr);
+            (cplx_t<R> *)y_out, nullptr, nullptr, nullptr, sc.t_lo * 256);
     });
     return hipGetLastError();
 }
 
-hipError_t launch_reduce_fwd_ybuf(Handle *h, double scale, bool weights, const void *y_data) {
-    const Sched &sc = h->sched_f[h->backend];
-    dim3 grid((unsigned)(h->ns_pad / 32), (unsigned)(h->ncs / 32));
+// Reduction of one forward pass into ybuf (coil fastest) with scale, optional weights
+// dvec and optional subtraction of data (user layout).
+static hipError_t launch_reduce_fwd_ybuf(Handle *h, const Sched &sc, double scale, bool weights,
+                                         const void *y_data) {
+    dim3 grid((unsigned)((sc.t_hi - sc.t_lo) * 8), (unsigned)(h->ncs / 32));
     by_prec(h, [&](auto r) {
         using R = decltype(r);
         k_reduce_fwd<R><<<grid, 256, 0, h->stream>>>(
             (const cplx_t<R> *)h->part_f.p, (const int *)sc.tile_off.p, (const int *)sc.tile_list.p, h->ncs,
             (int)h->n_sam, (int)h->n_cha, (R)scale, mixed(h) ? slot(h, 0) : nullptr, h->csm_max, 1,
             (cplx_t<R> *)h->ybuf.p, weights ? (const R *)h->dvec.p : nullptr, (const cplx_t<R> *)y_data,
-            h->f64 ? nullptr : slot(h, 1));
+            h->f64 ? nullptr : slot(h, 1), sc.t_lo * 256);
     });
     return hipGetLastError();
+}
+
+hipError_t run_forward_user(Handle *h, const void *x_full, void *y_out, double scale, bool time_it) {
+    hipError_t e = launch_gather_max(h, x_full, h->xm.p);
+    if (e != hipSuccess) return e;
+    return forward_from_xm(h, h->xm.p, time_it, true,
+                           [&](const Sched &sc) { return launch_reduce_fwd_user(h, sc, y_out, scale); });
+}
+
+hipError_t run_forward_ybuf(Handle *h, const void *x_full, double scale, bool weights, const void *y_data,
+                            bool time_it) {
+    hipError_t e = launch_gather_max(h, x_full, h->xm.p);
+    if (e != hipSuccess) return e;
+    return forward_from_xm(h, h->xm.p, time_it, true, [&](const Sched &sc) {
+        return launch_reduce_fwd_ybuf(h, sc, scale, weights, y_data);
+    });
 }
 
 hipError_t launch_load_ybuf(Handle *h, const void *y_user, bool weights) {
@@ -2092,12 +2267,16 @@ hipError_t launch_load_ybuf(Handle *h, const void *y_user, bool weights) {
     return hipGetLastError();
 }
 
-// Adjoint contraction from ybuf (slot 1 must hold |ybuf|max for the mixed path).
-static hipError_t adjoint_contract_only(Handle *h, bool time_it) {
+static hipError_t adjoint_reduce(Handle *h, const Sched &sc, double scale, void *xm_out, double lambda,
+                                 const void *x_in);
+
+// Adjoint contraction from ybuf (slot 1 must hold |ybuf|max for the mixed path) and its
+// reduction into xm_out (+ lambda x_in), pass by pass.
+static hipError_t adjoint_passes(Handle *h, bool time_it, double scale, void *xm_out, double lambda,
+                                 const void *x_in) {
+    auto red = [&](const Sched &sc) { return adjoint_reduce(h, sc, scale, xm_out, lambda, x_in); };
     if (direct_path(h->backend)) {
-        if (time_it) (void)hipEventRecord(h->ev[2], h->stream);
-        launch_contract<false>(h, h->ybuf.p, h->part_a.p);
-        if (time_it) (void)hipEventRecord(h->ev[3], h->stream);
+        return contract_passes<false>(h, h->ybuf.p, h->part_a.p, time_it, 2, red);
     } else {
         const int n_ks = (int)(h->ns_pad / 8);
         const int n_gb = frag_groups(h);
@@ -2110,46 +2289,40 @@ static hipError_t adjoint_contract_only(Handle *h, bool time_it) {
             k_build_yfrag<__bf16><<<blocks_for(nt), 256, 0, h->stream>>>((const float2 *)h->ybuf.p,
                                                                           (uint4 *)h->yfrag.p, n_ks, h->ncs, n_gb,
                                                                           slot(h, 1));
-        if (time_it) (void)hipEventRecord(h->ev[2], h->stream);
-        launch_contract<false>(h, h->yfrag.p, h->part_a.p);
-        if (time_it) (void)hipEventRecord(h->ev[3], h->stream);
+        return contract_passes<false>(h, h->yfrag.p, h->part_a.p, time_it, 2, red);
     }
-    return hipGetLastError();
 }
 
-static hipError_t adjoint_reduce(Handle *h, double scale, void *xm_out, double lambda, const void *x_in) {
-    const Sched &sc = h->sched_a[h->backend];
-    const int64_t threads = h->nv_pad * 32;
+// Reduction of one adjoint pass (the voxels of its tiles).
+static hipError_t adjoint_reduce(Handle *h, const Sched &sc, double scale, void *xm_out, double lambda,
+                                 const void *x_in) {
+    const int64_t threads = (int64_t)(sc.t_hi - sc.t_lo) * 256 * 32;
     by_prec(h, [&](auto r) {
         using R = decltype(r);
         k_reduce_adj<R><<<blocks_for(threads), 256, 0, h->stream>>>(
             (const cplx_t<R> *)h->part_a.p, (const int *)sc.tile_off.p, (const int *)sc.tile_list.p, (int)h->nv_pad,
             h->ncs, (int)h->n_cha, (const cplx_t<R> *)h->csm.p, (int)h->n_vox, (R)scale,
             mixed(h) ? slot(h, 1) : nullptr,
-            (cplx_t<R> *)xm_out, (R)lambda, (const cplx_t<R> *)x_in);
+            (cplx_t<R> *)xm_out, (R)lambda, (const cplx_t<R> *)x_in, sc.t_lo * 256);
     });
     return hipGetLastError();
 }
 
 hipError_t run_adjoint_contract(Handle *h, double scale, void *xm_out, bool time_it) {
-    hipError_t e = adjoint_contract_only(h, time_it);
-    if (e != hipSuccess) return e;
-    return adjoint_reduce(h, scale, xm_out, 0.0f, nullptr);
+    return adjoint_passes(h, time_it, scale, xm_out, 0.0, nullptr);
 }
 
 hipError_t run_normal_masked(Handle *h, const void *xm_in, void *xm_out, double scale, double lambda,
                              bool time_it, bool have_max) {
     // slot 0 / slot 1 hold the fragment scales of the mixed path
     if (!have_max && !h->f64) zero_slots(h);
-    hipError_t e = forward_from_xm(h, xm_in, time_it, have_max);
-    if (e != hipSuccess) return e;
-    e = launch_reduce_fwd_ybuf(h, 1.0, true, nullptr);
-    if (e != hipSuccess) return e;
-    e = adjoint_contract_only(h, time_it);
+    hipError_t e = forward_from_xm(h, xm_in, time_it, have_max, [&](const Sched &sc) {
+        return launch_reduce_fwd_ybuf(h, sc, 1.0, true, nullptr);
+    });
     if (e != hipSuccess) return e;
     // FP32 backends: lambda enters the reduction as float, as it always has
     const double lam = h->f64 ? lambda : (double)(float)lambda;
-    return adjoint_reduce(h, scale, xm_out, lam, lambda != 0.0 ? xm_in : nullptr);
+    return adjoint_passes(h, time_it, scale, xm_out, lam, lambda != 0.0 ? xm_in : nullptr);
 }
 
 // Mixed backends with 32 coils: Z and residual fragments (slot 0 = max|xm|,

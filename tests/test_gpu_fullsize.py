@@ -16,6 +16,10 @@ Bars: 3e-5 relative L2 for the sampled rows and entries (the Float32 GPU-operato
 bar of test/test_explicit_gpu.jl:51) and 1e-5 for the identities; the BF16 hi/lo
 backend (about 16 significant bits per product) is held to the 1e-4 mixed-precision
 target for both.
+
+The device memory an operator holds after the four products (the drop in free device
+memory from before its construction) stays under 3 GB at both sizes: partial sums that
+would pass 512 MiB per contraction run as passes over tile ranges instead.
 """
 import numpy as np
 import pytest
@@ -49,17 +53,26 @@ def _picks(n, seed):
 BARS = {"mixed_fp16": (3e-5, 1e-5), "direct": (3e-5, 1e-5), "mixed_bf16": (1e-4, 1e-4)}
 
 
+def _free_device_bytes():
+    import torch
+    torch.cuda.synchronize()
+    return torch.cuda.mem_get_info(0)[0]
+
+
 @pytest.mark.parametrize("backend", ["mixed_fp16", "direct", "mixed_bf16"])
 def test_full_size_products(workload, backend, gpu_lib):
     w, c, x, y, n, ns, nc = workload
     print(f"workload {w} {backend}: {n} voxels, {ns} samples, {nc} coils", flush=True)
     mask = c.get("mask")
+    free0 = _free_device_bytes()
     with hoe.HighOrderExplicitOp(c["grid"], c["kspha"], c["times"], fieldmap=c["fieldmap"], csm=c["csm"],
                                  mask=mask, backend=backend) as op:
         ex = np.asarray(op * x)
         ehy = np.asarray(op.H * y)
         nx = np.asarray(hoe.normalOperator(op) * x)
         ehex = np.asarray(op.H * ex)
+        held = free0 - _free_device_bytes()
+    print(f"workload {w} {backend}: operator holds {held / 1e9:.2f} GB of device memory", flush=True)
     nv = n if mask is None else int(np.count_nonzero(mask))
 
     # sampled rows of E x
@@ -85,6 +98,7 @@ def test_full_size_products(workload, backend, gpu_lib):
     print(f"workload {w} {backend}: E x rows vs FP64 {e_fwd:.2e}, Eᴴy entries vs FP64 {e_adj:.2e}, "
           f"adjoint identity {e_dot:.2e}, EᴴE vs Eᴴ(E x) {e_norm:.2e}", flush=True)
     bar, bar_id = BARS[backend]
+    assert held < 3e9, held
     assert np.count_nonzero(ehy[~full]) == 0
     assert e_fwd < bar, e_fwd
     assert e_adj < bar, e_adj
