@@ -334,13 +334,22 @@ __global__ __launch_bounds__(256) void k_direct(
 // (coil data, precomputed per application in exactly the B-register layout,
 // FP16/BF16 hi+lo, scaled by a power of two) is read with 16-byte loads.
 // Three products per K step: Ahi*Bhi + Alo*Bhi + Ahi*Blo, FP32 accumulate.
-// Output columns: tile 0 = real part, tile 1 = imaginary part of 32 coils.
-template <typename T16, bool FWD, bool HX, int L, int NB>
+// Output columns (PK = 0): tile 0 = real part, tile 1 = imaginary part of 32 coils.
+// Up to 16 coils share one B tile (PK = 1): columns 0..15 the real part, 16..31 the
+// imaginary part of coils 0..15, so each product yields both (6 MFMAs per K step
+// instead of 12, two B fragments instead of four). Up to 8 coils (PK = 2): columns
+// [re hi | im hi | re lo | im lo] of coils 0..7, so Ahi*B + Alo*B holds all three
+// products (and the lo*lo term) in 4 MFMAs and one B fragment; column c + 16 is added
+// to column c in the epilogue.
+template <typename T16, bool FWD, bool HX, int L, int NB, int PK = 0>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (FWD ? HOE_WAVES_FWD : HOE_WAVES_ADJ) : 2))) void k_mixed(
     const double *__restrict__ coef, int sk, const double *__restrict__ basis, int sb,
     const uint4 *__restrict__ frag, int n_ks, float2 *__restrict__ part, int ncs,
     const int4 *__restrict__ segs, const int *__restrict__ chunk_off, int cpx, HornerArgs hx) {
     typedef typename mfma_t<T16>::vec vec_t;
+    static_assert(PK == 0 || NB == 1, "packed coil layouts hold one group");
+    constexpr int NQ = PK == 0 ? 4 : (PK == 1 ? 2 : 1);   // B fragments (uint4) per lane and K step
+    constexpr int NA = PK == 0 ? 2 : 1;                   // accumulator tiles per row tile
     const int chunk = chunk_id(cpx);
     const int grp = blockIdx.y;
     const int lane = threadIdx.x & 63;
@@ -358,14 +367,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (
     for (int sgi = sg0; sgi < sg1; ++sgi) {
         const int4 d = segs[sgi];
         const int row = d.x * 256 + wave * 64 + lane;
-        f16x_t acc[2][NB][2];
+        f16x_t acc[2][NB][NA];
 #pragma unroll
         for (int a = 0; a < 2; ++a)
 #pragma unroll
-            for (int b = 0; b < NB; ++b) {
-                acc[a][b][0] = f16x_t{};
-                acc[a][b][1] = f16x_t{};
-            }
+            for (int b = 0; b < NB; ++b)
+#pragma unroll
+                for (int i = 0; i < NA; ++i) acc[a][b][i] = f16x_t{};
         // K loop over this segment; phases_of(ks, ph) fills 8 phases per lane:
         // with swap, those of the lane's own row for the 8 streamed rows of K
         // step ks (moved into the A layout by permlane32 swaps); without, the
@@ -374,7 +382,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (
         auto kloop = [&](auto phases_of, auto swap) {
             // one K step: phases of the 8 streamed rows, exp(i phi) split hi/lo into the
             // A layout, 12 MFMAs per coil group against the B fragments bfr
-            auto step = [&](int ks, const uint4 (&bfr)[NB][4]) {
+            auto step = [&](int ks, const uint4 (&bfr)[NB][NQ]) {
                 double ph[8];
                 phases_of(ks, ph);
                 unsigned hp[8], lp[8];
@@ -400,20 +408,38 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (
                 ahi[1] = __builtin_bit_cast(vec_t, make_uint4(hp[4], hp[5], hp[6], hp[7]));
                 alo[0] = __builtin_bit_cast(vec_t, make_uint4(lp[0], lp[1], lp[2], lp[3]));
                 alo[1] = __builtin_bit_cast(vec_t, make_uint4(lp[4], lp[5], lp[6], lp[7]));
+                if constexpr (PK == 0) {
 #pragma unroll
-                for (int b = 0; b < NB; ++b) {
-                    const vec_t bhr = __builtin_bit_cast(vec_t, bfr[b][0]);
-                    const vec_t bhi = __builtin_bit_cast(vec_t, bfr[b][1]);
-                    const vec_t blr = __builtin_bit_cast(vec_t, bfr[b][2]);
-                    const vec_t bli = __builtin_bit_cast(vec_t, bfr[b][3]);
+                    for (int b = 0; b < NB; ++b) {
+                        const vec_t bhr = __builtin_bit_cast(vec_t, bfr[b][0]);
+                        const vec_t bhi = __builtin_bit_cast(vec_t, bfr[b][1]);
+                        const vec_t blr = __builtin_bit_cast(vec_t, bfr[b][2]);
+                        const vec_t bli = __builtin_bit_cast(vec_t, bfr[b][3]);
+#pragma unroll
+                        for (int rt = 0; rt < 2; ++rt) {
+                            acc[rt][b][0] = mfma_t<T16>::run(ahi[rt], bhr, acc[rt][b][0]);
+                            acc[rt][b][NA - 1] = mfma_t<T16>::run(ahi[rt], bhi, acc[rt][b][NA - 1]);
+                            acc[rt][b][0] = mfma_t<T16>::run(alo[rt], bhr, acc[rt][b][0]);
+                            acc[rt][b][NA - 1] = mfma_t<T16>::run(alo[rt], bhi, acc[rt][b][NA - 1]);
+                            acc[rt][b][0] = mfma_t<T16>::run(ahi[rt], blr, acc[rt][b][0]);
+                            acc[rt][b][NA - 1] = mfma_t<T16>::run(ahi[rt], bli, acc[rt][b][NA - 1]);
+                        }
+                    }
+                } else if constexpr (PK == 1) {
+                    const vec_t bh = __builtin_bit_cast(vec_t, bfr[0][0]);
+                    const vec_t bl = __builtin_bit_cast(vec_t, bfr[0][NQ - 1]);
 #pragma unroll
                     for (int rt = 0; rt < 2; ++rt) {
-                        acc[rt][b][0] = mfma_t<T16>::run(ahi[rt], bhr, acc[rt][b][0]);
-                        acc[rt][b][1] = mfma_t<T16>::run(ahi[rt], bhi, acc[rt][b][1]);
-                        acc[rt][b][0] = mfma_t<T16>::run(alo[rt], bhr, acc[rt][b][0]);
-                        acc[rt][b][1] = mfma_t<T16>::run(alo[rt], bhi, acc[rt][b][1]);
-                        acc[rt][b][0] = mfma_t<T16>::run(ahi[rt], blr, acc[rt][b][0]);
-                        acc[rt][b][1] = mfma_t<T16>::run(ahi[rt], bli, acc[rt][b][1]);
+                        acc[rt][0][0] = mfma_t<T16>::run(ahi[rt], bh, acc[rt][0][0]);
+                        acc[rt][0][0] = mfma_t<T16>::run(alo[rt], bh, acc[rt][0][0]);
+                        acc[rt][0][0] = mfma_t<T16>::run(ahi[rt], bl, acc[rt][0][0]);
+                    }
+                } else {
+                    const vec_t bb = __builtin_bit_cast(vec_t, bfr[0][0]);
+#pragma unroll
+                    for (int rt = 0; rt < 2; ++rt) {
+                        acc[rt][0][0] = mfma_t<T16>::run(ahi[rt], bb, acc[rt][0][0]);
+                        acc[rt][0][0] = mfma_t<T16>::run(alo[rt], bb, acc[rt][0][0]);
                     }
                 }
             };
@@ -425,14 +451,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (
                 // so the reads are conflict-free ds_read_b128.
                 const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
                 auto issue = [&](int c0, int bsel) {
-                    for (int i = wv; i < S * 4; i += 4) {
-                        const int kk = c0 + (i >> 2), q = i & 3;
+                    for (int i = wv; i < S * NQ; i += 4) {
+                        const int kk = c0 + i / NQ, q = i % NQ;
                         if (kk < d.z)
                             __builtin_amdgcn_global_load_lds(
                                 (const __attribute__((address_space(1))) void *)(
-                                    frag + ((((size_t)grp * n_ks + kk) * 64 + lane) << 2) + q),
-                                (__attribute__((address_space(3))) void *)&fbuf[bsel][(i >> 2) * 256 + q * 64], 16, 0,
-                                0);
+                                    frag + (((size_t)grp * n_ks + kk) * 64 + lane) * NQ + q),
+                                (__attribute__((address_space(3))) void *)&fbuf[bsel][(i / NQ) * 64 * NQ + q * 64],
+                                16, 0, 0);
                     }
                 };
                 issue(d.y, 0);
@@ -443,10 +469,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (
                     if (c0 + S < d.z) issue(c0 + S, bsel ^ 1);
                     const int c1 = min(c0 + S, d.z);
                     for (int ks = c0; ks < c1; ++ks) {
-                        uint4 bfr[NB][4];
-                        const uint4 *lb = &fbuf[bsel][(ks - c0) * 256 + lane];
+                        uint4 bfr[NB][NQ];
+                        const uint4 *lb = &fbuf[bsel][(ks - c0) * 64 * NQ + lane];
 #pragma unroll
-                        for (int q = 0; q < 4; ++q) bfr[0][q] = lb[q * 64];
+                        for (int q = 0; q < NQ; ++q) bfr[0][q] = lb[q * 64];
                         step(ks, bfr);
                     }
                     __builtin_amdgcn_s_waitcnt(0);
@@ -455,12 +481,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (
                 }
             } else {
                 for (int ks = d.y; ks < d.z; ++ks) {
-                    uint4 bfr[NB][4];
+                    uint4 bfr[NB][NQ];
 #pragma unroll
                     for (int b = 0; b < NB; ++b) {
-                        const uint4 *p = frag + ((((size_t)(grp * NB + b) * n_ks + ks) * 64 + lane) << 2);
+                        const uint4 *p = frag + (((size_t)(grp * NB + b) * n_ks + ks) * 64 + lane) * NQ;
 #pragma unroll
-                        for (int q = 0; q < 4; ++q) bfr[b][q] = p[q];
+                        for (int q = 0; q < NQ; ++q) bfr[b][q] = p[q];
                     }
                     step(ks, bfr);
                 }
@@ -660,17 +686,38 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (
         // row 8*(r/4) + 4*(l/32) + r%4, column l%32.
         const int col = lane & 31;
         const int hi32 = lane >> 5;
+        if constexpr (PK == 0) {
 #pragma unroll
-        for (int rt = 0; rt < 2; ++rt)
+            for (int rt = 0; rt < 2; ++rt)
 #pragma unroll
-            for (int b = 0; b < NB; ++b) {
-                const int coil = (grp * NB + b) * 32 + col;
+                for (int b = 0; b < NB; ++b) {
+                    const int coil = (grp * NB + b) * 32 + col;
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) {
+                        const int rr = wave * 64 + rt * 32 + 8 * (r >> 2) + 4 * hi32 + (r & 3);
+                        part[((size_t)sgi * 256 + rr) * ncs + coil] =
+                            make_float2(acc[rt][b][0][r], acc[rt][b][NA - 1][r]);
+                    }
+                }
+        } else {
+            // one FP32 component per lane: PK = 1 column c is (coil c % 16, part c / 16);
+            // PK = 2 lanes with column c < 16 add column c + 16 (the lo half) and hold
+            // (coil c % 8, part c / 8)
+            float *pf = reinterpret_cast<float *>(part);
+            const int coil = PK == 1 ? (col & 15) : (col & 7);
+            const int cmp = PK == 1 ? (col >> 4) : ((col >> 3) & 1);
+#pragma unroll
+            for (int rt = 0; rt < 2; ++rt)
 #pragma unroll
                 for (int r = 0; r < 16; ++r) {
+                    float v = acc[rt][0][0][r];
+                    // lane ^ 16 within each 32-lane half: ds_swizzle bit mode, and 0x1f, xor 0x10
+                    if constexpr (PK == 2)
+                        v += __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x401f));
                     const int rr = wave * 64 + rt * 32 + 8 * (r >> 2) + 4 * hi32 + (r & 3);
-                    part[((size_t)sgi * 256 + rr) * ncs + coil] = make_float2(acc[rt][b][0][r], acc[rt][b][1][r]);
+                    if (PK == 1 || col < 16) pf[(((size_t)sgi * 256 + rr) * ncs + coil) * 2 + cmp] = v;
                 }
-            }
+        }
     }
 }
 
@@ -1102,11 +1149,33 @@ __global__ void k_build_z(const cplx_t<R> *__restrict__ csm, const cplx_t<R> *__
     z[i] = mkc<R>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
+// Packed B fragment (PK = 1, 2: see k_mixed) of one lane from its four (re tile,
+// im tile) pairs: PK = 1 lane column c takes the re (c < 16) or im tile of coil c % 16,
+// hi and lo in two uint4; PK = 2 column c takes tile (c / 8) % 2 of coil c % 8, hi
+// (c < 16) or lo, in one uint4.
+template <typename T16, int PK>
+__device__ __forceinline__ void store_packed(uint4 *o, int col, const float (&re)[4][2], const float (&im)[4][2]) {
+    unsigned h[4], l[4];
+    const bool take_im = PK == 1 ? (col >= 16) : (((col >> 3) & 1) != 0);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const float a = take_im ? im[j][0] : re[j][0], b = take_im ? im[j][1] : re[j][1];
+        split2<T16>(a, b, h[j], l[j]);
+    }
+    if constexpr (PK == 1) {
+        o[0] = make_uint4(h[0], h[1], h[2], h[3]);
+        o[1] = make_uint4(l[0], l[1], l[2], l[3]);
+    } else {
+        o[0] = col < 16 ? make_uint4(h[0], h[1], h[2], h[3]) : make_uint4(l[0], l[1], l[2], l[3]);
+    }
+}
+
 // MFMA B fragments of z = csm .* xm for the forward (K = voxels):
 // frag[gb][ks][lane] = {hi_re, hi_im, lo_re, lo_im}, each 8 x T16:
 //   lane l: coil 32*gb + l%32, voxels 8*ks + 4*(l/32) + i/2 (i = 0..7);
 //   re tile: (Zr, -Zi) pairs, im tile: (Zi, Zr) pairs.
-template <typename T16>
+// PK = 1, 2: the packed layouts of k_mixed, 2 or 1 uint4 per lane (coils 0..15 / 0..7).
+template <typename T16, int PK = 0>
 __global__ void k_build_zfrag(const float2 *__restrict__ csm, const float2 *__restrict__ xm, uint4 *__restrict__ frag,
                               int n_ks, int ncs, int n_gb, const unsigned *__restrict__ maxslot, float csm_max) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1116,6 +1185,23 @@ __global__ void k_build_zfrag(const float2 *__restrict__ csm, const float2 *__re
     const int ks = (int)(rest % n_ks);
     const int gb = (int)(rest / n_ks);
     const float scale = pow2_scale(__uint_as_float(*maxslot) * csm_max);
+    if constexpr (PK != 0) {
+        const int col = lane & 31, c = PK == 1 ? (col & 15) : (col & 7);
+        float re[4][2], im[4][2];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int v = ks * 8 + 4 * (lane >> 5) + j;
+            const float2 a = csm[(size_t)v * ncs + c], b = xm[v];
+            const float zr = (a.x * b.x - a.y * b.y) * scale;
+            const float zi = (a.x * b.y + a.y * b.x) * scale;
+            re[j][0] = zr;
+            re[j][1] = -zi;
+            im[j][0] = zi;
+            im[j][1] = zr;
+        }
+        store_packed<T16, PK>(frag + t * (PK == 1 ? 2 : 1), col, re, im);
+        return;
+    }
     const int c = gb * 32 + (lane & 31);
     unsigned hr[4], hi[4], lr[4], li[4];
 #pragma unroll
@@ -1137,8 +1223,8 @@ __global__ void k_build_zfrag(const float2 *__restrict__ csm, const float2 *__re
 }
 
 // MFMA B fragments of the weighted signal y[s][c] for the adjoint (K = samples):
-//   re tile: (Yr, Yi) pairs, im tile: (Yi, -Yr) pairs.
-template <typename T16>
+//   re tile: (Yr, Yi) pairs, im tile: (Yi, -Yr) pairs (PK as in k_build_zfrag).
+template <typename T16, int PK = 0>
 __global__ void k_build_yfrag(const float2 *__restrict__ y, uint4 *__restrict__ frag, int n_ks, int ncs, int n_gb,
                               const unsigned *__restrict__ maxslot) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1148,6 +1234,22 @@ __global__ void k_build_yfrag(const float2 *__restrict__ y, uint4 *__restrict__ 
     const int ks = (int)(rest % n_ks);
     const int gb = (int)(rest / n_ks);
     const float scale = pow2_scale(__uint_as_float(*maxslot));
+    if constexpr (PK != 0) {
+        const int col = lane & 31, c = PK == 1 ? (col & 15) : (col & 7);
+        float re[4][2], im[4][2];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int s = ks * 8 + 4 * (lane >> 5) + j;
+            const float2 a = y[(size_t)s * ncs + c];
+            const float yr = a.x * scale, yi = a.y * scale;
+            re[j][0] = yr;
+            re[j][1] = yi;
+            im[j][0] = yi;
+            im[j][1] = -yr;
+        }
+        store_packed<T16, PK>(frag + t * (PK == 1 ? 2 : 1), col, re, im);
+        return;
+    }
     const int c = gb * 32 + (lane & 31);
     unsigned hr[4], hi[4], lr[4], li[4];
 #pragma unroll
@@ -1176,25 +1278,32 @@ __global__ void k_build_yfrag(const float2 *__restrict__ y, uint4 *__restrict__ 
 // when scale_slot is given (undoing the fragment scaling of the mixed path).
 template <typename R>
 __device__ __forceinline__ cplx_t<R> sum_segments(const cplx_t<R> *__restrict__ part, const int *__restrict__ list,
-                                                  int n, int r, int c, int ncs) {
+                                                  int n, int r, int c, int ncs, int i0 = 0, int step = 1) {
     cplx_t<R> a[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) a[q] = mkc<R>(R(0), R(0));
-    int i = 0;
-    for (; i + 3 < n; i += 4) {
+    int i = i0;
+    for (; i + 3 * step < n; i += 4 * step) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const cplx_t<R> t = part[((size_t)list[i + q] * 256 + r) * ncs + c];
+            const cplx_t<R> t = part[((size_t)list[i + q * step] * 256 + r) * ncs + c];
             a[q].x += t.x;
             a[q].y += t.y;
         }
     }
-    for (; i < n; ++i) {
+    for (; i < n; i += step) {
         const cplx_t<R> t = part[((size_t)list[i] * 256 + r) * ncs + c];
         a[0].x += t.x;
         a[0].y += t.y;
     }
     return mkc<R>((a[0].x + a[1].x) + (a[2].x + a[3].x), (a[0].y + a[1].y) + (a[2].y + a[3].y));
+}
+
+// Coil columns that one 32-lane row of a reduction covers: nc rounded up to a power of
+// two when nc <= 16 (the 32 / that lanes of a coil then share its segment list, each
+// summing every (32 / ncl)-th segment, and a fixed butterfly adds them), else 32.
+__device__ __forceinline__ int reduce_cols(int nc) {
+    return nc > 16 ? 32 : (nc <= 1 ? 1 : 1 << (32 - __clz(nc - 1)));
 }
 
 template <typename R>
@@ -1222,29 +1331,35 @@ __global__ __launch_bounds__(256) void k_reduce_fwd(
         __syncthreads();
     }
     // four rows per thread and two segments per step, in separate sums, so that 8
-    // loads are in flight
+    // loads are in flight; lanes of coils beyond nc read nothing (the contraction
+    // kernels need not write those columns). With nc <= 16 the lanes of one coil split
+    // its segments (see reduce_cols); lane tc < ncl then holds coil tc's total and the
+    // others, whose columns lie beyond nc, are discarded below.
+    const int ncl = c0 == 0 ? reduce_cols(nc) : 32, nsub = 32 / ncl;
+    const int cc = c0 + (tc & (ncl - 1));
     cplx_t<R> rows[4], rows2[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) rows[q] = rows2[q] = mkc<R>(R(0), R(0));
-    int j = 0;
-    for (; j + 2 <= nseg; j += 2) {
-        const size_t b0 = (size_t)list[j] * 256, b1 = (size_t)list[j + 1] * 256;
+    const int nsr = cc < nc ? nseg : 0;
+    int j = tc / ncl;
+    for (; j + nsub < nsr; j += 2 * nsub) {
+        const size_t b0 = (size_t)list[j] * 256, b1 = (size_t)list[j + nsub] * 256;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int r = (s0 + tr + 8 * q) & 255;
-            const cplx_t<R> t0 = part[(b0 + r) * ncs + c0 + tc];
-            const cplx_t<R> t1 = part[(b1 + r) * ncs + c0 + tc];
+            const cplx_t<R> t0 = part[(b0 + r) * ncs + cc];
+            const cplx_t<R> t1 = part[(b1 + r) * ncs + cc];
             rows[q].x += t0.x;
             rows[q].y += t0.y;
             rows2[q].x += t1.x;
             rows2[q].y += t1.y;
         }
     }
-    if (j < nseg) {
+    if (j < nsr) {
         const size_t b0 = (size_t)list[j] * 256;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const cplx_t<R> t = part[(b0 + ((s0 + tr + 8 * q) & 255)) * ncs + c0 + tc];
+            const cplx_t<R> t = part[(b0 + ((s0 + tr + 8 * q) & 255)) * ncs + cc];
             rows[q].x += t.x;
             rows[q].y += t.y;
         }
@@ -1253,6 +1368,10 @@ __global__ __launch_bounds__(256) void k_reduce_fwd(
     for (int q = 0; q < 4; ++q) {
         rows[q].x += rows2[q].x;
         rows[q].y += rows2[q].y;
+        for (int o = ncl; o < 32; o <<= 1) {
+            rows[q].x += __shfl_xor(rows[q].x, o, 32);
+            rows[q].y += __shfl_xor(rows[q].y, o, 32);
+        }
     }
     float m = 0.0f;
 #pragma unroll
@@ -1342,8 +1461,10 @@ __global__ __launch_bounds__(256) void k_reduce_adj(
         const int tile = v >> 8;
         const int *list = tile_list + tile_off[tile];
         const int nseg = tile_off[tile + 1] - tile_off[tile];
-        for (int c = cl; c < nc; c += 32) {
-            const cplx_t<R> acc = sum_segments<R>(part, list, nseg, v & 255, c, ncs);
+        // with nc <= 16 the 32 / ncl lanes of a coil split its segments (reduce_cols)
+        const int ncl = reduce_cols(nc);
+        for (int c = cl & (ncl - 1); c < nc; c += 32) {
+            const cplx_t<R> acc = sum_segments<R>(part, list, nseg, v & 255, c, ncs, cl / ncl, 32 / ncl);
             const cplx_t<R> s = csm[(size_t)v * ncs + c];
             ar += s.x * acc.x + s.y * acc.y;   // conj(s) * acc
             ai += s.x * acc.y - s.y * acc.x;
@@ -1653,19 +1774,25 @@ static void visit_direct(int L, int cb, F &&f) {
         default: visit_direct_cb<FWD, HX, 20, DT, R>(cb, f); break;
     }
 }
+// shape: coil groups per wave (1, 2) for 32-coil groups, or 3 / 4 for the packed layouts
+// of up to 16 / 8 coils (see k_mixed)
 template <typename T16, bool FWD, bool HX, int L, class F>
-static void visit_mixed_nb(int nb, F &&f) {
-    if (nb == 1) f(k_mixed<T16, FWD, HX, L, 1>);
-    else f(k_mixed<T16, FWD, HX, L, 2>);
+static void visit_mixed_nb(int shape, F &&f) {
+    switch (shape) {
+        case 1: f(k_mixed<T16, FWD, HX, L, 1>); break;
+        case 3: f(k_mixed<T16, FWD, HX, L, 1, 1>); break;
+        case 4: f(k_mixed<T16, FWD, HX, L, 1, 2>); break;
+        default: f(k_mixed<T16, FWD, HX, L, 2>); break;
+    }
 }
 template <typename T16, bool FWD, bool HX, class F>
-static void visit_mixed(int L, int nb, F &&f) {
+static void visit_mixed(int L, int shape, F &&f) {
     switch (L) {
-        case 6: visit_mixed_nb<T16, FWD, HX, 6>(nb, f); break;
-        case 10: visit_mixed_nb<T16, FWD, HX, 10>(nb, f); break;
-        case 12: visit_mixed_nb<T16, FWD, HX, 12>(nb, f); break;
-        case 16: visit_mixed_nb<T16, FWD, HX, 16>(nb, f); break;
-        default: visit_mixed_nb<T16, FWD, HX, 20>(nb, f); break;
+        case 6: visit_mixed_nb<T16, FWD, HX, 6>(shape, f); break;
+        case 10: visit_mixed_nb<T16, FWD, HX, 10>(shape, f); break;
+        case 12: visit_mixed_nb<T16, FWD, HX, 12>(shape, f); break;
+        case 16: visit_mixed_nb<T16, FWD, HX, 16>(shape, f); break;
+        default: visit_mixed_nb<T16, FWD, HX, 20>(shape, f); break;
     }
 }
 template <int L, class F>
@@ -1733,23 +1860,36 @@ static void visit_contract_direct(Handle *h, F &&f) {
         visit_direct<false, false>(h->LK, direct_cb(h), f);
     }
 }
+// Packed MFMA coil layout of the mixed backends: 2 for up to 8 coils, 1 for up to 16,
+// 0 for groups of 32 coils.
+#ifndef HOE_PACK8
+#define HOE_PACK8 1
+#endif
+static inline int mixed_pack(const Handle *h) {
+    return (HOE_PACK8 && h->n_cha <= 8) ? 2 : (h->n_cha <= 16 ? 1 : 0);
+}
+int mixed_nb(const Handle *h);
+static inline int mixed_shape(const Handle *h) {
+    const int pk = mixed_pack(h);
+    return pk ? 2 + pk : mixed_nb(h);
+}
 template <bool FWD, class F>
 static void visit_contract_mixed(Handle *h, int backend, F &&f) {
     if constexpr (FWD) {
         if (backend == HOE_BACKEND_MIXED_FP16) {
-            if (h->horner) visit_mixed<_Float16, true, true>(h->LK, mixed_nb(h), f);
-            else visit_mixed<_Float16, true, false>(h->LK, mixed_nb(h), f);
+            if (h->horner) visit_mixed<_Float16, true, true>(h->LK, mixed_shape(h), f);
+            else visit_mixed<_Float16, true, false>(h->LK, mixed_shape(h), f);
         } else {
-            if (h->horner) visit_mixed<__bf16, true, true>(h->LK, mixed_nb(h), f);
-            else visit_mixed<__bf16, true, false>(h->LK, mixed_nb(h), f);
+            if (h->horner) visit_mixed<__bf16, true, true>(h->LK, mixed_shape(h), f);
+            else visit_mixed<__bf16, true, false>(h->LK, mixed_shape(h), f);
         }
     } else {
         if (backend == HOE_BACKEND_MIXED_FP16) {
-            if (h->horner_adj) visit_mixed<_Float16, false, true>(h->LK, mixed_nb(h), f);
-            else visit_mixed<_Float16, false, false>(h->LK, mixed_nb(h), f);
+            if (h->horner_adj) visit_mixed<_Float16, false, true>(h->LK, mixed_shape(h), f);
+            else visit_mixed<_Float16, false, false>(h->LK, mixed_shape(h), f);
         } else {
-            if (h->horner_adj) visit_mixed<__bf16, false, true>(h->LK, mixed_nb(h), f);
-            else visit_mixed<__bf16, false, false>(h->LK, mixed_nb(h), f);
+            if (h->horner_adj) visit_mixed<__bf16, false, true>(h->LK, mixed_shape(h), f);
+            else visit_mixed<__bf16, false, false>(h->LK, mixed_shape(h), f);
         }
     }
 }
@@ -2190,12 +2330,15 @@ static hipError_t forward_from_xm(Handle *h, const void *xmv, bool time_it, bool
             k_absmax2<<<blocks_for(h->nv_pad), 256, 0, h->stream>>>(xm, (int)h->nv_pad, slot(h, 0));
         }
         const int64_t nt = (int64_t)n_gb * n_ks * 64;
+        auto build = [&](auto k) {
+            k<<<blocks_for(nt), 256, 0, h->stream>>>((const float2 *)h->csm.p, xm, (uint4 *)h->zfrag.p, n_ks, h->ncs,
+                                                      n_gb, slot(h, 0), h->csm_max);
+        };
+        const int pk = mixed_pack(h);
         if (h->backend == HOE_BACKEND_MIXED_FP16)
-            k_build_zfrag<_Float16><<<blocks_for(nt), 256, 0, h->stream>>>(
-                (const float2 *)h->csm.p, xm, (uint4 *)h->zfrag.p, n_ks, h->ncs, n_gb, slot(h, 0), h->csm_max);
+            build(pk == 2 ? k_build_zfrag<_Float16, 2> : (pk == 1 ? k_build_zfrag<_Float16, 1> : k_build_zfrag<_Float16, 0>));
         else
-            k_build_zfrag<__bf16><<<blocks_for(nt), 256, 0, h->stream>>>(
-                (const float2 *)h->csm.p, xm, (uint4 *)h->zfrag.p, n_ks, h->ncs, n_gb, slot(h, 0), h->csm_max);
+            build(pk == 2 ? k_build_zfrag<__bf16, 2> : (pk == 1 ? k_build_zfrag<__bf16, 1> : k_build_zfrag<__bf16, 0>));
         return contract_passes<true>(h, h->zfrag.p, h->part_f.p, time_it, 0, red);
     }
 }
@@ -2281,14 +2424,15 @@ static hipError_t adjoint_passes(Handle *h, bool time_it, double scale, void *xm
         const int n_ks = (int)(h->ns_pad / 8);
         const int n_gb = frag_groups(h);
         const int64_t nt = (int64_t)n_gb * n_ks * 64;
+        auto build = [&](auto k) {
+            k<<<blocks_for(nt), 256, 0, h->stream>>>((const float2 *)h->ybuf.p, (uint4 *)h->yfrag.p, n_ks, h->ncs,
+                                                      n_gb, slot(h, 1));
+        };
+        const int pk = mixed_pack(h);
         if (h->backend == HOE_BACKEND_MIXED_FP16)
-            k_build_yfrag<_Float16><<<blocks_for(nt), 256, 0, h->stream>>>((const float2 *)h->ybuf.p,
-                                                                            (uint4 *)h->yfrag.p, n_ks, h->ncs, n_gb,
-                                                                            slot(h, 1));
+            build(pk == 2 ? k_build_yfrag<_Float16, 2> : (pk == 1 ? k_build_yfrag<_Float16, 1> : k_build_yfrag<_Float16, 0>));
         else
-            k_build_yfrag<__bf16><<<blocks_for(nt), 256, 0, h->stream>>>((const float2 *)h->ybuf.p,
-                                                                          (uint4 *)h->yfrag.p, n_ks, h->ncs, n_gb,
-                                                                          slot(h, 1));
+            build(pk == 2 ? k_build_yfrag<__bf16, 2> : (pk == 1 ? k_build_yfrag<__bf16, 1> : k_build_yfrag<__bf16, 0>));
         return contract_passes<false>(h, h->yfrag.p, h->part_a.p, time_it, 2, red);
     }
 }

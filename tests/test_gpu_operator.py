@@ -72,7 +72,8 @@ def test_fixture_matches_dense(backend, gpu_lib):
 
 @pytest.mark.parametrize("backend", BACKENDS)
 @pytest.mark.parametrize("shape,nt,nc", [((5, 4, 1), 9, 3), ((4, 3, 2), 16, 33), ((4, 5, 1), 16, 5),
-                                         ((3, 4, 2), 9, 33), ((6, 5, 1), 16, 96), ((3, 3, 3), 16, 130)])
+                                         ((3, 4, 2), 9, 33), ((6, 5, 1), 16, 96), ((3, 3, 3), 16, 130),
+                                         ((4, 5, 1), 16, 8), ((5, 3, 2), 9, 12), ((3, 4, 2), 16, 16)])
 def test_independent_fp64_reference(backend, shape, nt, nc, gpu_lib):
     TOL32 = TOLS[backend][0]
     c = _random_case(shape, nt, nc, 1234 + nt + nc)
@@ -148,10 +149,12 @@ def test_recon_terms_and_concomitant(backend, gpu_lib):
 
 
 @pytest.mark.parametrize("backend", BACKENDS)
-def test_config0_matches_cpu_reference(backend, gpu_lib):
-    """Full workload 0 (64x64, 8 coils, 6000-sample spiral) vs the restated CPU
-    Array path (<= 1e-5 relative L2) and the FP64 oracle."""
-    c = synth.make(0)
+@pytest.mark.parametrize("n_cha", [8, 16])
+def test_config0_matches_cpu_reference(backend, n_cha, gpu_lib):
+    """Full workload 0 (64x64, 8 coils, 6000-sample spiral; and with 16 coils) vs the
+    restated CPU Array path (<= 1e-5 relative L2) and the FP64 oracle. The mixed
+    backends pack up to 8 and up to 16 coils into one MFMA tile."""
+    c = synth.make(0, n_cha=n_cha)
     m = _model(c)
     x = c["image"]
     y_ref = O.cpu_ref_forward(m, x)
@@ -159,7 +162,7 @@ def test_config0_matches_cpu_reference(backend, gpu_lib):
     with _op(c, backend) as op:
         y = op * x
         e_ref, e64 = O.relerr(y, y_ref), O.relerr(y, y64)
-        print(f"config0 {backend}: fwd vs cpu_ref {e_ref:.2e}, vs fp64 {e64:.2e}, cpu_ref vs fp64 "
+        print(f"config0 {backend} {n_cha} coils: fwd vs cpu_ref {e_ref:.2e}, vs fp64 {e64:.2e}, cpu_ref vs fp64 "
               f"{O.relerr(y_ref, y64):.2e}")
         tol = TOLS[backend][1]
         assert e_ref < tol
