@@ -19,6 +19,14 @@
   10 * amp * e_op, e_op being the GPU normal operator's own measured error against
   the FP64 normal on the same problem (with a floor of 1e-7 for e_op).
 
+- Workload 1 in full (200x200, 32 coils, 16 000 samples): 30 CGNR iterations of every
+  FP32 backend (hoe_cg) against the same textbook CG run in FP64 over the FP64 backend's
+  normal operator, whose products match the FP64 oracle at 1e-11 (test_gpu_fp64.py, a
+  slice of this workload included): the FP64 oracle cannot run 30 normals of this size
+  in seconds. The amplification is measured the same way over that FP64 normal, the
+  bar is 10 * amp * e_op, and the mixed and FP32 images must agree to within the sum
+  of their bars.
+
 The y data of the full-size cases comes from the GPU forward with perturbed coefficients
 (1 % order 1, 5 % orders 2-3) and a 3 us delay; it is data only, the gradient formula
 holds for any y. Bars: 1e-4 relative (the north-star gradient bar) for every backend.
@@ -198,3 +206,54 @@ def test_cg30_workload1_slice(cg_slice, backend, gpu_lib):
           f"bar {bar:.2e}; residual {log[-1]:.4e} vs {log64[-1]:.4e}", flush=True)
     assert err <= bar
     assert abs(log[-1] - log64[-1]) <= 5e-2 * log64[-1]
+
+
+@pytest.fixture(scope="module")
+def cg_full():
+    """30-iteration FP64 CG over the FP64 backend's normal at workload 1, and its
+    measured amplification."""
+    iters = 30
+    c = synth.make(1)
+    g64 = hoe.Grid(c["grid"].nX, c["grid"].nY, c["grid"].nZ, float(c["grid"].dx), float(c["grid"].dy),
+                   float(c["grid"].dz), dtype=np.float64)
+    op64 = hoe.HighOrderExplicitOp(g64, c["kspha"].astype(np.float64), c["times"].astype(np.float64),
+                                   fieldmap=c["fieldmap"].astype(np.float64), csm=c["csm"].astype(np.complex128),
+                                   backend="fp64")
+    y32 = np.asarray(op64 * c["image"].astype(np.complex128).reshape(-1)).astype(np.complex64)
+    rhs = np.asarray(op64.H * y32.astype(np.complex128))
+    n64 = hoe.normalOperator(op64)
+    nf = lambda v: np.asarray(n64 * v)
+    x64, log64 = O.cgnr(nf, rhs, iters)
+    delta = 1e-6
+    s = 1.0 + delta * np.random.default_rng(9).uniform(-1.0, 1.0, rhs.size)
+    xp, _ = O.cgnr(lambda v: s * nf(s * v), rhs, iters)
+    amp = O.relerr(xp, x64) / delta
+    print(f"workload 1 CG: FP64 30-iteration residual {log64[-1]:.3e}, amplification {amp:.2f}", flush=True)
+    yield c, y32, x64, log64, amp, nf, iters
+    op64.close()
+
+
+def test_cg30_workload1_full(cg_full, gpu_lib):
+    c, y32, x64, log64, amp, nf, iters = cg_full
+    ns, nc = c["kspha"].shape[1], c["csm"].shape[-1]
+    data = y32.reshape(nc, ns).T
+    params = {"solver": hoe.CGNR, "reg": hoe.L2Regularization(0.0), "reconSize": c["grid"].matrixSize,
+              "iterations": iters}
+    rng = np.random.default_rng(4)
+    v = (rng.standard_normal(c["grid"].n) + 1j * rng.standard_normal(c["grid"].n)).astype(np.complex64)
+    nv = nf(v.astype(np.complex128))
+    imgs, bars = {}, {}
+    for backend in BACKENDS:
+        with _op(c, backend) as op:
+            img, log = hoe.recon_HOOp(op, data, np.ones(ns, np.complex64), params, return_log=True)
+            e_op = O.relerr(np.asarray(hoe.normalOperator(op) * v), nv)
+        imgs[backend] = img.reshape(-1, order="F")
+        err = O.relerr(imgs[backend], x64)
+        bars[backend] = 10.0 * amp * max(e_op, 1e-7)
+        print(f"workload 1 CG {backend}: image vs FP64 CG {err:.2e}, operator error {e_op:.2e}, amplification "
+              f"{amp:.2f}, bar {bars[backend]:.2e}; residual {log[-1]:.4e} vs {log64[-1]:.4e}", flush=True)
+        assert err <= bars[backend]
+        assert abs(log[-1] - log64[-1]) <= 5e-2 * log64[-1]
+    d = O.relerr(imgs["mixed_fp16"], imgs["direct"])
+    print(f"workload 1 CG: mixed vs FP32 image {d:.2e}", flush=True)
+    assert d <= bars["mixed_fp16"] + bars["direct"]

@@ -20,6 +20,47 @@ import oracle as O
 from hoe.grid import Grid, basisfunc_spha
 
 
+@pytest.mark.parametrize("shape,res,dtype", [((4, 3, 2), (0.7, 0.9, 1.2), np.float32), ((5, 1, 3), (0.01, 0.02, 0.03),
+                                               np.float64), ((6, 5, 4), (2e-3, 3e-3, 5e-3), np.float32)])
+@pytest.mark.parametrize("opts", [{}, {"exchange_xy": True}, {"reverse_x": True, "reverse_z": True},
+                                  {"reverse_y": True}])
+def test_host_grid_matches_oracle_grid_and_closed_forms(shape, res, dtype, opts):
+    """The host API's Grid against the oracle's restatement of Grid.jl:63-95 (bit for bit)
+    and against closed forms: voxel (i, j, k), 0-based, sits at flattened index
+    i + nX (j + nY k) with coordinates ((i + 1 - (nX + 1)/2) dx, ...), so z takes the
+    values (k + 1 - (nZ + 1)/2) dz, symmetric about 0."""
+    nX, nY, nZ = shape
+    T = np.dtype(dtype).type
+    dx, dy, dz = (T(v) for v in res)
+    g = Grid(nX, nY, nZ, dx, dy, dz, dtype=dtype, **opts)
+    o = O.OracleGrid(nX, nY, nZ, dx, dy, dz, dtype=dtype, **opts)
+    for a in "xyz":
+        assert getattr(g, a).dtype == np.dtype(dtype)
+        np.testing.assert_array_equal(getattr(g, a), getattr(o, a))
+    assert g.matrixSize == o.matrixSize == shape and g.n == o.n == nX * nY * nZ
+    i, j, k = np.meshgrid(np.arange(nX), np.arange(nY), np.arange(nZ), indexing="ij")
+    idx = (i + nX * (j + nY * k)).ravel()
+    cx = np.empty(g.n)
+    cy = np.empty(g.n)
+    cz = np.empty(g.n)
+    cx[idx] = (i.ravel() + 1 - (nX + 1) / 2) * float(dx)
+    cy[idx] = (j.ravel() + 1 - (nY + 1) / 2) * float(dy)
+    cz[idx] = (k.ravel() + 1 - (nZ + 1) / 2) * float(dz)
+    if opts.get("exchange_xy"):
+        cx, cy = cy, cx
+    if opts.get("reverse_x"):
+        cx = cx[::-1]
+    if opts.get("reverse_y"):
+        cy = cy[::-1]
+    if opts.get("reverse_z"):
+        cz = cz[::-1]
+    for a, c in (("x", cx), ("y", cy), ("z", cz)):
+        np.testing.assert_allclose(getattr(g, a).astype(np.float64), c, rtol=0, atol=4 * np.finfo(dtype).eps * max(res) * max(shape))
+    assert abs(float(np.sum(g.z.astype(np.float64)))) <= g.n * np.finfo(dtype).eps * float(dz) * nZ
+    if nZ == 2 and not opts.get("reverse_z"):
+        np.testing.assert_array_equal(np.unique(g.z), [-dz / 2, dz / 2])
+
+
 def test_basis_known_answer_term17():
     # test_Concomitant.jl:92
     np.testing.assert_array_equal(basisfunc_spha(np.array([1., 2.]), np.array([3., 4.]), np.array([5., 6.]), 17),
