@@ -23,9 +23,13 @@
   FP32 backend (hoe_cg) against the same textbook CG run in FP64 over the FP64 backend's
   normal operator, whose products match the FP64 oracle at 1e-11 (test_gpu_fp64.py, a
   slice of this workload included): the FP64 oracle cannot run 30 normals of this size
-  in seconds. The amplification is measured the same way over that FP64 normal, the
-  bar is 10 * amp * e_op, and the mixed and FP32 images must agree to within the sum
-  of their bars.
+  in seconds. Here the CG's own FP32 vectors dominate the difference: the same CG over
+  the exact FP64 normal, with x, r, p and q rounded to complex64 after every update
+  (FP64 inner products, as the device CG computes them), lands d32 = 2.5e-3 from the
+  FP64 image, while the operators' error alone moves it by less than 1e-3 (FP16, FP32).
+  The bar is d32 + 10 * amp * e_op, amp being the diagonal-perturbation amplification
+  per unit of measured operator change; the final residuals agree within 10 %, and the
+  mixed and FP32 images within 2 * d32.
 
 The y data of the full-size cases comes from the GPU forward with perturbed coefficients
 (1 % order 1, 5 % orders 2-3) and a 3 us delay; it is data only, the gradient formula
@@ -208,10 +212,30 @@ def test_cg30_workload1_slice(cg_slice, backend, gpu_lib):
     assert abs(log[-1] - log64[-1]) <= 5e-2 * log64[-1]
 
 
+def _cg_fp32_vectors(apply_normal, rhs, iters):
+    """The textbook CG with its vectors rounded to complex64 after every update and FP64
+    inner products (the arithmetic of the device CG), over an FP64 normal."""
+    c64 = np.complex64
+    x = np.zeros(rhs.size, c64)
+    r = rhs.astype(c64)
+    p = r.copy()
+    rr = np.vdot(r.astype(np.complex128), r).real
+    rr0 = rr
+    for _ in range(iters):
+        q = np.asarray(apply_normal(p.astype(np.complex128))).astype(c64)
+        a = rr / np.vdot(p.astype(np.complex128), q).real
+        x = (x + c64(a) * p).astype(c64)
+        r = (r - c64(a) * q).astype(c64)
+        rn = np.vdot(r.astype(np.complex128), r).real
+        p = (r + c64(rn / rr) * p).astype(c64)
+        rr = rn
+    return x.astype(np.complex128), np.sqrt(rr / rr0)
+
+
 @pytest.fixture(scope="module")
 def cg_full():
-    """30-iteration FP64 CG over the FP64 backend's normal at workload 1, and its
-    measured amplification."""
+    """30-iteration FP64 CG over the FP64 backend's normal at workload 1, its
+    amplification per unit of operator change, and the FP32-vector CG's deviation."""
     iters = 30
     c = synth.make(1)
     g64 = hoe.Grid(c["grid"].nX, c["grid"].nY, c["grid"].nZ, float(c["grid"].dx), float(c["grid"].dy),
@@ -224,36 +248,38 @@ def cg_full():
     n64 = hoe.normalOperator(op64)
     nf = lambda v: np.asarray(n64 * v)
     x64, log64 = O.cgnr(nf, rhs, iters)
-    delta = 1e-6
-    s = 1.0 + delta * np.random.default_rng(9).uniform(-1.0, 1.0, rhs.size)
-    xp, _ = O.cgnr(lambda v: s * nf(s * v), rhs, iters)
-    amp = O.relerr(xp, x64) / delta
-    print(f"workload 1 CG: FP64 30-iteration residual {log64[-1]:.3e}, amplification {amp:.2f}", flush=True)
-    yield c, y32, x64, log64, amp, nf, iters
+    rng = np.random.default_rng(4)
+    v = rng.standard_normal(c["grid"].n) + 1j * rng.standard_normal(c["grid"].n)
+    nv = nf(v)
+    s = 1.0 + 1e-6 * np.random.default_rng(9).uniform(-1.0, 1.0, rhs.size)
+    xp, _ = O.cgnr(lambda u: s * nf(s * u), rhs, iters)
+    amp = O.relerr(xp, x64) / O.relerr(s * nf(s * v), nv)
+    x32, res32 = _cg_fp32_vectors(nf, rhs, iters)
+    d32 = O.relerr(x32, x64)
+    print(f"workload 1 CG: FP64 residual {log64[-1]:.3e}, amplification per operator change {amp:.1f}, "
+          f"FP32-vector CG {d32:.2e} (residual {res32:.3e})", flush=True)
+    yield c, y32, x64, log64, amp, d32, v, nv, iters
     op64.close()
 
 
 def test_cg30_workload1_full(cg_full, gpu_lib):
-    c, y32, x64, log64, amp, nf, iters = cg_full
+    c, y32, x64, log64, amp, d32, v, nv, iters = cg_full
     ns, nc = c["kspha"].shape[1], c["csm"].shape[-1]
     data = y32.reshape(nc, ns).T
     params = {"solver": hoe.CGNR, "reg": hoe.L2Regularization(0.0), "reconSize": c["grid"].matrixSize,
               "iterations": iters}
-    rng = np.random.default_rng(4)
-    v = (rng.standard_normal(c["grid"].n) + 1j * rng.standard_normal(c["grid"].n)).astype(np.complex64)
-    nv = nf(v.astype(np.complex128))
-    imgs, bars = {}, {}
+    imgs = {}
     for backend in BACKENDS:
         with _op(c, backend) as op:
             img, log = hoe.recon_HOOp(op, data, np.ones(ns, np.complex64), params, return_log=True)
-            e_op = O.relerr(np.asarray(hoe.normalOperator(op) * v), nv)
+            e_op = O.relerr(np.asarray(hoe.normalOperator(op) * v.astype(np.complex64)), nv)
         imgs[backend] = img.reshape(-1, order="F")
         err = O.relerr(imgs[backend], x64)
-        bars[backend] = 10.0 * amp * max(e_op, 1e-7)
-        print(f"workload 1 CG {backend}: image vs FP64 CG {err:.2e}, operator error {e_op:.2e}, amplification "
-              f"{amp:.2f}, bar {bars[backend]:.2e}; residual {log[-1]:.4e} vs {log64[-1]:.4e}", flush=True)
-        assert err <= bars[backend]
-        assert abs(log[-1] - log64[-1]) <= 5e-2 * log64[-1]
+        bar = d32 + 10.0 * amp * max(e_op, 1e-7)
+        print(f"workload 1 CG {backend}: image vs FP64 CG {err:.2e}, operator error {e_op:.2e}, bar {bar:.2e}; "
+              f"residual {log[-1]:.4e} vs {log64[-1]:.4e}", flush=True)
+        assert err <= bar
+        assert abs(log[-1] - log64[-1]) <= 0.1 * log64[-1]
     d = O.relerr(imgs["mixed_fp16"], imgs["direct"])
     print(f"workload 1 CG: mixed vs FP32 image {d:.2e}", flush=True)
-    assert d <= bars["mixed_fp16"] + bars["direct"]
+    assert d <= 2 * d32
