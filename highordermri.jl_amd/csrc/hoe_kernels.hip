@@ -789,7 +789,10 @@ __global__ __launch_bounds__(256) void k_grad(
 //  1. U^T (32 voxels x 32 samples, per tile) on v_mfma_f32_32x32x16: A = Z
 //     fragments (voxel rows, K = 2*ncs real: [zr | zi] for Re, [zi | -zr] for
 //     Im), B = residual fragments (K = [rr | ri], sample columns), hi/lo split.
-//     The residual fragments of the wave's samples sit in LDS for the segment.
+//     The residual fragments of the wave's samples sit in LDS for the segment;
+//     the Z fragments of one voxel block are copied to LDS once for the 4 waves
+//     (direct-to-LDS loads) and the next block's copy runs during stages 2 and 3
+//     (measured 0.93 -> 0.89 ms at 200^2, 32 coils, 16 000 samples).
 //  2. Phases with lane = sample (as in the forward: wave-uniform voxel
 //     records), then v_permlane32_swap moves (cos, sin) of voxel pairs (j, j+4)
 //     into the D layout of U^T: register r of lane l = voxel 8(r/4)+4(l/32)+r%4,
@@ -824,6 +827,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
     (void)sz;
     (void)sr;
     uint4 *my = rlds + (size_t)wave * n_kc * 4 * 64 + lane;
+    // Z fragments of the current voxel block, shared by the 4 waves: [kc][q][lane]
+    uint4 *zl = rlds + (size_t)4 * n_kc * 4 * 64;
+    const int wv = __builtin_amdgcn_readfirstlane(wave);
+    // direct-to-LDS copy of block kb's Z fragments, 4 of the 16 (kc, q) rows per wave
+    auto zfetch = [&](int kb) {
+#pragma unroll
+        for (int i = wv; i < n_kc * 4; i += 4)
+            __builtin_amdgcn_global_load_lds(
+                (const __attribute__((address_space(1))) void *)(zfr + ((((size_t)kb * n_kc + (i >> 2)) * 64 + lane) << 2) +
+                                                                 (i & 3)),
+                (__attribute__((address_space(3))) void *)&zl[i * 64], 16, 0, 0);
+    };
     const int sg0 = chunk_off[chunk], sg1 = chunk_off[chunk + 1];
     for (int sgi = sg0; sgi < sg1; ++sgi) {
         const int4 d = segs[sgi];
@@ -849,16 +864,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 #pragma unroll
             for (int i = 0; i <= L; ++i) st[i] = coef[(size_t)row * sk + i];
         }
+        zfetch(d.y);
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
         for (int kb = d.y; kb < d.z; ++kb) {
+            // basis fragments of both half blocks, in flight during stage 1
+            uint4 bq[2][2];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const uint4 *bp = bfr + ((((size_t)kb * 2 + h) * 64 + lane) << 1);
+                bq[h][0] = bp[0];
+                bq[h][1] = bp[1];
+            }
             // ---- stage 1: U^T for both sample tiles ----
             f16x_t ure[2] = {f16x_t{}, f16x_t{}}, uim[2] = {f16x_t{}, f16x_t{}};
 #pragma unroll
             for (int kc = 0; kc < n_kc; ++kc) {
-                const uint4 *zp = zfr + ((((size_t)kb * n_kc + kc) * 64 + lane) << 2);
+                const uint4 *zp = zl + kc * 4 * 64 + lane;
                 const vec_t zrh = __builtin_bit_cast(vec_t, zp[0]);
-                const vec_t zrl = __builtin_bit_cast(vec_t, zp[1]);
-                const vec_t zih = __builtin_bit_cast(vec_t, zp[2]);
-                const vec_t zil = __builtin_bit_cast(vec_t, zp[3]);
+                const vec_t zrl = __builtin_bit_cast(vec_t, zp[64]);
+                const vec_t zih = __builtin_bit_cast(vec_t, zp[128]);
+                const vec_t zil = __builtin_bit_cast(vec_t, zp[192]);
 #pragma unroll
                 for (int t = 0; t < 2; ++t) {
                     const vec_t rh = __builtin_bit_cast(vec_t, my[((kc * 2 + t) * 2 + 0) * 64]);
@@ -871,6 +897,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                     uim[t] = mfma_t<T16>::run(zih, rl, uim[t]);
                 }
             }
+            // every wave has read block kb's Z: fetch block kb + 1 into the same buffer
+            // while the phases of block kb run
+            __syncthreads();
+            if (kb + 1 < d.z) zfetch(kb + 1);
             // ---- stages 2 and 3, per half block of 16 voxels ----
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
@@ -928,9 +958,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                     sn[j] = __uint_as_float(s2[0]);
                     sn[j + 4] = __uint_as_float(s2[1]);
                 }
-                const uint4 *bp = bfr + ((((size_t)kb * 2 + h) * 64 + lane) << 1);
-                const vec_t bh = __builtin_bit_cast(vec_t, bp[0]);
-                const vec_t bl = __builtin_bit_cast(vec_t, bp[1]);
+                const vec_t bh = __builtin_bit_cast(vec_t, bq[h][0]);
+                const vec_t bl = __builtin_bit_cast(vec_t, bq[h][1]);
 #pragma unroll
                 for (int t = 0; t < 2; ++t) {
                     unsigned wh[4], wl[4];
@@ -948,6 +977,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                     g[t] = mfma_t<T16>::run(ah, bl, g[t]);
                 }
             }
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
         }
         // D layout: register r of lane l = sample 8(r/4)+4(l/32)+r%4 of the tile, term l%32
         const int col = lane & 31, hi32 = lane >> 5;
@@ -2207,7 +2238,9 @@ hipError_t prepare_grad(Handle *h) {
     return grow(h->part_g, (size_t)sg.n_seg * nb * 256 * h->LK * sizeof(double));
 }
 
-static constexpr int kGradLds = 4 * 4 * 4 * 64 * 16;   // waves x chunks x (tile, hi/lo) x lanes x uint4
+// residual fragments (waves x chunks x (tile, hi/lo) x lanes x uint4) and one voxel block's
+// Z fragments (chunks x 4 x lanes x uint4)
+static constexpr int kGradLds = 4 * 4 * 4 * 64 * 16 + 4 * 4 * 64 * 16;
 
 bool grad_on_mfma(const Handle *h) { return h->backend != HOE_BACKEND_DIRECT && h->ncs == 32; }
 
@@ -2216,10 +2249,14 @@ static hipError_t prepare_grad_mfma(Handle *h) {
     Sched &sg = h->sched_gm;
     if (!sg.built) {
         int occ = 1;
+        hipError_t ea = hipSuccess;
         visit_gmixed(h, [&](auto k) {
+            ea = hipFuncSetAttribute(reinterpret_cast<const void *>(k), hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     kGradLds);
             int nb = 0;
             if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k, 256, kGradLds) == hipSuccess && nb > 0) occ = nb;
         });
+        if (ea != hipSuccess) return ea;
         const int slots = std::max(8, h->cu_count * occ);
         if ((e = build_schedule(sg, (int)(h->ns_pad / 256), (int)(h->nv_pad / 32), slots, 0)) != hipSuccess) return e;
         if ((e = grow(h->part_gm, (size_t)sg.n_seg * 256 * 32 * sizeof(float))) != hipSuccess) return e;
