@@ -134,8 +134,8 @@ struct Handle {
 };
 
 // ---- coil batching and schedules (hoe_kernels.hip) ----
-int direct_cb(const Handle *h);        // coils per direct-kernel batch: 8, 16 or 32
-int direct_batches(const Handle *h);
+int direct_cb(const Handle *h, bool fwd);   // coils per direct-kernel batch: 8, 16 or 32
+int direct_batches(const Handle *h, bool fwd);
 int mixed_nb(const Handle *h);         // 32-coil groups per wave in the MFMA kernels: 1 or 2
 int mixed_groups(const Handle *h);
 hipError_t build_schedule(Sched &sc, int tiles, int n_ks, int slots, int kmax,

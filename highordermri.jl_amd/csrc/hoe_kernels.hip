@@ -1874,21 +1874,21 @@ static void visit_gmixed(Handle *h, F &&f) {
 template <bool FWD, class F>
 static void visit_contract_direct(Handle *h, F &&f) {
     if (h->f64) {
-        if (FWD && h->horner) visit_direct<FWD, true, false, double>(h->LK, direct_cb(h), f);
-        else visit_direct<FWD, false, false, double>(h->LK, direct_cb(h), f);
+        if (FWD && h->horner) visit_direct<FWD, true, false, double>(h->LK, direct_cb(h, FWD), f);
+        else visit_direct<FWD, false, false, double>(h->LK, direct_cb(h, FWD), f);
         return;
     }
     if constexpr (FWD) {
         if (h->dt_on) {
-            if (h->horner) visit_direct<true, true, true>(h->LK, direct_cb(h), f);
-            else visit_direct<true, false, true>(h->LK, direct_cb(h), f);
+            if (h->horner) visit_direct<true, true, true>(h->LK, direct_cb(h, true), f);
+            else visit_direct<true, false, true>(h->LK, direct_cb(h, true), f);
         } else if (h->horner) {
-            visit_direct<true, true>(h->LK, direct_cb(h), f);
+            visit_direct<true, true>(h->LK, direct_cb(h, true), f);
         } else {
-            visit_direct<true, false>(h->LK, direct_cb(h), f);
+            visit_direct<true, false>(h->LK, direct_cb(h, true), f);
         }
     } else {
-        visit_direct<false, false>(h->LK, direct_cb(h), f);
+        visit_direct<false, false>(h->LK, direct_cb(h, false), f);
     }
 }
 // Packed MFMA coil layout of the mixed backends: 2 for up to 8 coils, 1 for up to 16,
@@ -1931,11 +1931,22 @@ static void visit_contract(Handle *h, int backend, F &&f) {
 }
 
 // ---- coil batching ----
-int direct_cb(const Handle *h) {
+// Coils per batch of the FP32/FP64 direct kernels. The FP32 Horner forward (cheap phases,
+// the streamed coil row read through scalar loads) takes batches of 16 above 16 coils: a
+// 32-coil row (64 SGPRs) leaves no registers to keep the next row's loads in flight, and
+// the forward waited on them (200^2, 32 coils, 16 000 samples: 3.96 ms with 32-coil
+// batches, 3.17 ms with two 16-coil batches). The adjoint and the gradient, whose phases
+// cost more, are faster with one phase per 32 coils (3.11 vs 3.20 ms).
+int direct_cb(const Handle *h, bool fwd) {
     if (h->f64) return h->n_cha <= 8 ? 8 : 16;   // 4 FP64 registers per complex accumulator
-    return h->n_cha <= 8 ? 8 : (h->n_cha <= 16 ? 16 : 32);
+    if (h->n_cha <= 8) return 8;
+    if (h->n_cha <= 16) return 16;
+    return (fwd && h->horner && !h->dt_on) ? 16 : 32;
 }
-int direct_batches(const Handle *h) { return (int)((h->n_cha + direct_cb(h) - 1) / direct_cb(h)); }
+int direct_batches(const Handle *h, bool fwd) {
+    const int cb = direct_cb(h, fwd);
+    return (int)((h->n_cha + cb - 1) / cb);
+}
 int mixed_nb(const Handle *h) { return h->ncs / 32 >= 2 ? 2 : 1; }
 int mixed_groups(const Handle *h) { return (h->ncs / 32 + mixed_nb(h) - 1) / mixed_nb(h); }
 // fragment groups of 32 coils, rounded up to whole per-wave batches
@@ -2200,12 +2211,13 @@ hipError_t prepare_backend(Handle *h, int backend) {
     std::vector<Sched> &sf = h->sched_f[backend], &sa = h->sched_a[backend];
     if (!sf.empty() && !sa.empty()) return hipSuccess;
     const bool is_mixed = backend == HOE_BACKEND_MIXED_FP16 || backend == HOE_BACKEND_MIXED_BF16;
-    const int groups = is_mixed ? mixed_groups(h) : direct_batches(h);
+    const int groups_f = is_mixed ? mixed_groups(h) : direct_batches(h, true);
+    const int groups_a = is_mixed ? mixed_groups(h) : direct_batches(h, false);
     int occ_f = 1, occ_a = 1;
     visit_contract<true>(h, backend, [&](auto k) { occ_f = blocks_per_cu(k); });
     visit_contract<false>(h, backend, [&](auto k) { occ_a = blocks_per_cu(k); });
-    const int slots_f = std::max(8, h->cu_count * occ_f / groups);
-    const int slots_a = std::max(8, h->cu_count * occ_a / groups);
+    const int slots_f = std::max(8, h->cu_count * occ_f / groups_f);
+    const int slots_a = std::max(8, h->cu_count * occ_a / groups_a);
     hipError_t e;
     const int kmax_a = is_mixed ? HOE_MFMA_ADJ_KMAX : 0;
     const int64_t seg_bytes = (int64_t)256 * h->ncs * h->cbytes, budget = part_budget();
@@ -2230,8 +2242,8 @@ hipError_t prepare_grad(Handle *h) {
     Sched &sg = h->sched_g;
     if (sg.built) return hipSuccess;
     int occ = 1;
-    visit_grad(h->LK, direct_cb(h), [&](auto k) { occ = blocks_per_cu(k); });
-    const int nb = direct_batches(h);
+    visit_grad(h->LK, direct_cb(h, false), [&](auto k) { occ = blocks_per_cu(k); });
+    const int nb = direct_batches(h, false);
     const int slots = std::max(8, h->cu_count * occ / nb);
     hipError_t e = build_schedule(sg, (int)(h->ns_pad / 256), (int)(h->nv_pad / 8), slots, 0);
     if (e != hipSuccess) return e;
@@ -2308,7 +2320,7 @@ static void launch_contract(Handle *h, const Sched &sc, const void *operand, voi
     const int *coff = (const int *)sc.chunk_off.p;
     HornerArgs hx{(const double2 *)h->hrec.p, (const int *)h->vrun.p, (const double *)h->beta.p, h->t_index};
     if (backend == HOE_BACKEND_DIRECT || backend == HOE_BACKEND_FP64) {
-        dim3 grid(sc.n_chunks, direct_batches(h));
+        dim3 grid(sc.n_chunks, direct_batches(h, FWD));
         const int n_stream = (int)(FWD ? h->n_vox : h->n_sam);
         visit_contract_direct<FWD>(h, [&](auto k) {
             launch_direct(k, grid, h->stream, coef, h->sk, basis, h->sb, operand, part, h->ncs, n_stream, segs, coff,
@@ -2551,13 +2563,13 @@ hipError_t run_grad_contract(Handle *h, bool time_it) {
     hipError_t e = launch_build_zbuf(h, h->xm.p);
     if (e != hipSuccess) return e;
     const Sched &sc = h->sched_g;
-    const int nb = direct_batches(h);
+    const int nb = direct_batches(h, false);
     dim3 grid(sc.n_chunks, nb);
     const double *coef = (const double *)h->coef.p, *basis = (const double *)h->basis.p;
     const float2 *z = (const float2 *)h->zbuf.p, *res = (const float2 *)h->ybuf.p;
     double *part = (double *)h->part_g.p;
     if (time_it) (void)hipEventRecord(h->ev[0], h->stream);
-    visit_grad(h->LK, direct_cb(h), [&](auto k) {
+    visit_grad(h->LK, direct_cb(h, false), [&](auto k) {
         k<<<grid, 256, 0, h->stream>>>(coef, h->sk, basis, h->sb, z, res, part, h->ncs, (int)h->n_vox,
                                        (const int4 *)sc.seg.p, (const int *)sc.chunk_off.p, sc.cpx);
     });
