@@ -231,6 +231,9 @@ int reschedule_direct(Handle *h, bool prev_dt) {
     return set_err(h, e == hipErrorOutOfMemory ? HOE_ENOMEM : HOE_EHIP, msg);
 }
 
+// Without a mask the masked voxel order is the grid order (mask_idx is the identity).
+bool unmasked(const Handle *h) { return h->n_vox == h->n_grid; }
+
 // The derivative model (kspha_dt) defines only the forward product.
 int reject_dt(Handle *h) {
     return h->dt_on ? set_err(h, HOE_EINVAL,
@@ -809,9 +812,14 @@ int hoe_adjoint(hoe_handle *hh, const void *y, void *x, int flags) {
     if ((rc = stage_in(h, flags, y, h->io_y, (size_t)h->cbytes * h->n_sam * h->n_cha, &dy))) return rc;
     void *dx = (flags & HOE_DEVICE_PTRS) ? x : h->io_x.p;
     HOE_TRY_HIP(h, hoe::launch_load_ybuf(h, dy, false));
-    HOE_TRY_HIP(h, hoe::run_adjoint_contract(h, prec_scale(h, 1.0 / std::sqrt((double)h->n_vox)), h->xm.p,
-                                             !(flags & HOE_ASYNC)));
-    HOE_TRY_HIP(h, hoe::launch_scatter(h, h->xm.p, dx, 1.0));
+    const double sc = prec_scale(h, 1.0 / std::sqrt((double)h->n_vox));
+    if (unmasked(h)) {
+        // the masked order is the grid order: the reduction writes the output directly
+        HOE_TRY_HIP(h, hoe::run_adjoint_contract(h, sc, dx, !(flags & HOE_ASYNC), (int)h->n_vox));
+    } else {
+        HOE_TRY_HIP(h, hoe::run_adjoint_contract(h, sc, h->xm.p, !(flags & HOE_ASYNC)));
+        HOE_TRY_HIP(h, hoe::launch_scatter(h, h->xm.p, dx, 1.0));
+    }
     if ((rc = finish(h, flags, x, dx, (size_t)h->cbytes * h->n_grid))) return rc;
     if (!(flags & HOE_ASYNC)) record_times(h, false, true);
     return HOE_OK;
@@ -843,9 +851,15 @@ int hoe_normal(hoe_handle *hh, const float *D, int64_t n_d, const void *x, void 
     const void *dx;
     if ((rc = stage_in(h, flags, x, h->io_x, (size_t)h->cbytes * h->n_grid, &dx))) return rc;
     void *dout = (flags & HOE_DEVICE_PTRS) ? out : h->io_x2.p;
-    void *tmp = h->cg_vec.p;   // masked output
-    HOE_TRY_HIP(h, hoe::run_normal_full(h, dx, tmp, prec_scale(h, 1.0 / (double)h->n_vox), !(flags & HOE_ASYNC)));
-    HOE_TRY_HIP(h, hoe::launch_scatter(h, tmp, dout, 1.0));
+    const double sc = prec_scale(h, 1.0 / (double)h->n_vox);
+    if (unmasked(h)) {
+        // x is consumed by the gather before the reduction writes the output (in place is fine)
+        HOE_TRY_HIP(h, hoe::run_normal_full(h, dx, dout, sc, !(flags & HOE_ASYNC), (int)h->n_vox));
+    } else {
+        void *tmp = h->cg_vec.p;   // masked output
+        HOE_TRY_HIP(h, hoe::run_normal_full(h, dx, tmp, sc, !(flags & HOE_ASYNC)));
+        HOE_TRY_HIP(h, hoe::launch_scatter(h, tmp, dout, 1.0));
+    }
     if ((rc = finish(h, flags, out, dout, (size_t)h->cbytes * h->n_grid))) return rc;
     if (!(flags & HOE_ASYNC)) record_times(h, true, true);
     return HOE_OK;
@@ -961,9 +975,14 @@ int hoe_normal_noalloc(hoe_handle *hh, const void *x, void *out, int flags) {
             off += align256(sizes[i]);
         }
         if (!(flags & HOE_ASYNC)) (void)hipEventRecord(h->ev[4], h->stream);
-        void *tmp = h->cg_vec.p;
-        HOE_TRY_HIP(h, hoe::run_normal_full(h, x, tmp, prec_scale(h, 1.0 / (double)h->n_vox), !(flags & HOE_ASYNC)));
-        HOE_TRY_HIP(h, hoe::launch_scatter(h, tmp, out, 1.0));
+        const double sc = prec_scale(h, 1.0 / (double)h->n_vox);
+        if (unmasked(h)) {
+            HOE_TRY_HIP(h, hoe::run_normal_full(h, x, out, sc, !(flags & HOE_ASYNC), (int)h->n_vox));
+        } else {
+            void *tmp = h->cg_vec.p;
+            HOE_TRY_HIP(h, hoe::run_normal_full(h, x, tmp, sc, !(flags & HOE_ASYNC)));
+            HOE_TRY_HIP(h, hoe::launch_scatter(h, tmp, out, 1.0));
+        }
     }
     if ((rc = finish(h, flags, out, out, sizeof(float2) * h->n_grid))) return rc;
     if (!(flags & HOE_ASYNC)) record_times(h, true, true);

@@ -163,15 +163,17 @@ hipError_t run_forward_ybuf(Handle *h, const void *x_full, double scale, bool we
 // Load a user signal (complex [c][s]) into ybuf (coil fastest), optionally weighted.
 hipError_t launch_load_ybuf(Handle *h, const void *y_user, bool weights);
 // Adjoint contraction from ybuf -> part_a and coil reduction into xm (masked) with scale,
-// pass by pass.
-hipError_t run_adjoint_contract(Handle *h, double scale, void *xm_out, bool time_it);
+// pass by pass. Entries at nv_write and beyond are not written (-1: all nv_pad, the
+// padding as zeros); without a mask a caller's full-grid output takes nv_write = n_vox.
+hipError_t run_adjoint_contract(Handle *h, double scale, void *xm_out, bool time_it, int nv_write = -1);
 // Normal operator on masked vectors: out_m = scale * E^H D E x_m (+ lambda x_m).
 // have_max: slot 0 already holds max|xm_in| and the other slots are clear (launch from a
 // gather that computed it).
 hipError_t run_normal_masked(Handle *h, const void *xm_in, void *xm_out, double scale, double lambda,
-                             bool time_it, bool have_max = false);
+                             bool time_it, bool have_max = false, int nv_write = -1);
 // Gather of a full-grid x (with max|x| for the fragment scale) and the normal operator.
-hipError_t run_normal_full(Handle *h, const void *x_full, void *xm_out, double scale, bool time_it);
+hipError_t run_normal_full(Handle *h, const void *x_full, void *xm_out, double scale, bool time_it,
+                           int nv_write = -1);
 // Gradient partial kernel: uses zbuf (csm .* x) and ybuf (residual, coil fastest) -> gk.
 hipError_t run_grad_contract(Handle *h, bool time_it);
 // zbuf = csm .* xm

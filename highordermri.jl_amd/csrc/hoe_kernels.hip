@@ -1480,7 +1480,7 @@ __global__ __launch_bounds__(256) void k_reduce_adj(
     const cplx_t<R> *__restrict__ part, const int *__restrict__ tile_off, const int *__restrict__ tile_list,
     int nv_pad, int ncs, int nc, const cplx_t<R> *__restrict__ csm, int nv, R scale,
     const unsigned *__restrict__ scale_slot, cplx_t<R> *__restrict__ xm_out, R lambda,
-    const cplx_t<R> *__restrict__ x_in, int v_base) {
+    const cplx_t<R> *__restrict__ x_in, int v_base, int nv_write) {
     const int v = v_base + ((blockIdx.x * blockDim.x + threadIdx.x) >> 5);   // half-wave id = voxel
     const int cl = threadIdx.x & 31;
     R sc = scale;
@@ -1513,7 +1513,7 @@ __global__ __launch_bounds__(256) void k_reduce_adj(
             r.x += lambda * t.x;
             r.y += lambda * t.y;
         }
-        xm_out[v] = r;
+        if (v < nv_write) xm_out[v] = r;
     }
 }
 
@@ -2392,10 +2392,10 @@ static hipError_t forward_from_xm(Handle *h, const void *xmv, bool time_it, bool
     }
 }
 
-hipError_t run_normal_full(Handle *h, const void *x_full, void *xm_out, double scale, bool time_it) {
+hipError_t run_normal_full(Handle *h, const void *x_full, void *xm_out, double scale, bool time_it, int nv_write) {
     hipError_t e = launch_gather_max(h, x_full, h->xm.p);
     if (e != hipSuccess) return e;
-    return run_normal_masked(h, h->xm.p, xm_out, scale, 0.0, time_it, true);
+    return run_normal_masked(h, h->xm.p, xm_out, scale, 0.0, time_it, true, nv_write);
 }
 
 static inline bool mixed(Handle *h) { return !direct_path(h->backend); }
@@ -2460,13 +2460,14 @@ hipError_t launch_load_ybuf(Handle *h, const void *y_user, bool weights) {
 }
 
 static hipError_t adjoint_reduce(Handle *h, const Sched &sc, double scale, void *xm_out, double lambda,
-                                 const void *x_in);
+                                 const void *x_in, int nv_write);
 
 // Adjoint contraction from ybuf (slot 1 must hold |ybuf|max for the mixed path) and its
 // reduction into xm_out (+ lambda x_in), pass by pass.
 static hipError_t adjoint_passes(Handle *h, bool time_it, double scale, void *xm_out, double lambda,
-                                 const void *x_in) {
-    auto red = [&](const Sched &sc) { return adjoint_reduce(h, sc, scale, xm_out, lambda, x_in); };
+                                 const void *x_in, int nv_write) {
+    if (nv_write < 0) nv_write = (int)h->nv_pad;
+    auto red = [&](const Sched &sc) { return adjoint_reduce(h, sc, scale, xm_out, lambda, x_in, nv_write); };
     if (direct_path(h->backend)) {
         return contract_passes<false>(h, h->ybuf.p, h->part_a.p, time_it, 2, red);
     } else {
@@ -2486,9 +2487,10 @@ static hipError_t adjoint_passes(Handle *h, bool time_it, double scale, void *xm
     }
 }
 
-// Reduction of one adjoint pass (the voxels of its tiles).
+// Reduction of one adjoint pass (the voxels of its tiles); entries at nv_write and beyond
+// are not written (a caller's grid-sized output has no padding).
 static hipError_t adjoint_reduce(Handle *h, const Sched &sc, double scale, void *xm_out, double lambda,
-                                 const void *x_in) {
+                                 const void *x_in, int nv_write) {
     const int64_t threads = (int64_t)(sc.t_hi - sc.t_lo) * 256 * 32;
     by_prec(h, [&](auto r) {
         using R = decltype(r);
@@ -2496,17 +2498,17 @@ static hipError_t adjoint_reduce(Handle *h, const Sched &sc, double scale, void 
             (const cplx_t<R> *)h->part_a.p, (const int *)sc.tile_off.p, (const int *)sc.tile_list.p, (int)h->nv_pad,
             h->ncs, (int)h->n_cha, (const cplx_t<R> *)h->csm.p, (int)h->n_vox, (R)scale,
             mixed(h) ? slot(h, 1) : nullptr,
-            (cplx_t<R> *)xm_out, (R)lambda, (const cplx_t<R> *)x_in, sc.t_lo * 256);
+            (cplx_t<R> *)xm_out, (R)lambda, (const cplx_t<R> *)x_in, sc.t_lo * 256, nv_write);
     });
     return hipGetLastError();
 }
 
-hipError_t run_adjoint_contract(Handle *h, double scale, void *xm_out, bool time_it) {
-    return adjoint_passes(h, time_it, scale, xm_out, 0.0, nullptr);
+hipError_t run_adjoint_contract(Handle *h, double scale, void *xm_out, bool time_it, int nv_write) {
+    return adjoint_passes(h, time_it, scale, xm_out, 0.0, nullptr, nv_write);
 }
 
 hipError_t run_normal_masked(Handle *h, const void *xm_in, void *xm_out, double scale, double lambda,
-                             bool time_it, bool have_max) {
+                             bool time_it, bool have_max, int nv_write) {
     // slot 0 / slot 1 hold the fragment scales of the mixed path
     if (!have_max && !h->f64) zero_slots(h);
     hipError_t e = forward_from_xm(h, xm_in, time_it, have_max, [&](const Sched &sc) {
@@ -2515,7 +2517,7 @@ hipError_t run_normal_masked(Handle *h, const void *xm_in, void *xm_out, double 
     if (e != hipSuccess) return e;
     // FP32 backends: lambda enters the reduction as float, as it always has
     const double lam = h->f64 ? lambda : (double)(float)lambda;
-    return adjoint_passes(h, time_it, scale, xm_out, lam, lambda != 0.0 ? xm_in : nullptr);
+    return adjoint_passes(h, time_it, scale, xm_out, lam, lambda != 0.0 ? xm_in : nullptr, nv_write);
 }
 
 // Mixed backends with 32 coils: Z and residual fragments (slot 0 = max|xm|,

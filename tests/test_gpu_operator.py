@@ -248,3 +248,38 @@ def test_out_vectors_are_checked_and_written_in_place(gpu_lib):
             op.ctprod(y_ref, out=np.empty(op.ncol, dtype=np.complex128))
         with pytest.raises(hoe.ArgumentError):
             hoe.normalOperator(op).prod(xd, out=np.empty(op.ncol, dtype=np.complex64))
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_unmasked_outputs_written_directly(backend, gpu_lib):
+    """Without a mask the adjoint reduction writes the caller's grid-sized output directly
+    (no scatter): every entry is written (outputs pre-filled with NaN), the adjoint and the
+    normal match the FP64 oracle, and an in-place normal (out = x, device) equals the
+    out-of-place one bit for bit."""
+    import ctypes as C
+    import torch
+    from hoe import _lib
+    c = _random_case((7, 5, 3), 16, 5, 91)
+    c["mask"] = None
+    TOL32 = TOLS[backend][0]
+    m = _model(c)
+    n = c["grid"].n
+    dev = torch.device("cuda:0")
+    x = torch.from_numpy(c["x"]).to(dev)
+    y = torch.from_numpy(c["y"]).to(dev)
+    with _op(c, backend) as op:
+        h = op._handle()
+        f = _lib.HOE_DEVICE_PTRS
+        xa = torch.full((n,), float("nan"), dtype=torch.complex64, device=dev)
+        _lib.check(op._lib.hoe_adjoint(h, C.c_void_p(y.data_ptr()), C.c_void_p(xa.data_ptr()), f), h)
+        nx = torch.full((n,), float("nan"), dtype=torch.complex64, device=dev)
+        _lib.check(op._lib.hoe_normal(h, None, 0, C.c_void_p(x.data_ptr()), C.c_void_p(nx.data_ptr()), f), h)
+        xi = x.clone()
+        _lib.check(op._lib.hoe_normal(h, None, _lib.HOE_CURRENT_WEIGHTS, C.c_void_p(xi.data_ptr()),
+                                      C.c_void_p(xi.data_ptr()), f), h)
+        torch.cuda.synchronize()
+        xa, nx, xi = xa.cpu().numpy(), nx.cpu().numpy(), xi.cpu().numpy()
+    assert np.all(np.isfinite(xa)) and np.all(np.isfinite(nx))
+    assert O.relerr(xa, O.adjoint_f64(m, c["y"])) < TOL32
+    assert O.relerr(nx, O.normal_f64(m, c["x"])) < TOL32
+    np.testing.assert_array_equal(xi, nx)
