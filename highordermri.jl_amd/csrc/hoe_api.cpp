@@ -935,7 +935,7 @@ void noalloc_plan(const Handle *h, size_t sizes[9]) {
     sizes[2] = sizeof(float2) * (size_t)h->nv_pad;                                // cg_vec: masked output
     sizes[3] = sizeof(float2) * (size_t)h->ns_pad * h->ncs;                       // ybuf
     sizes[4] = sizeof(float2) * 256 * (size_t)h->ncs * max_segs(h->sched_f[h->backend]);   // part_f
-    sizes[5] = sizeof(float2) * 256 * (size_t)h->ncs * max_segs(h->sched_a[h->backend]);   // part_a
+    sizes[5] = sizeof(float2) * 256 * (size_t)hoe::adj_groups(h, h->backend) * max_segs(h->sched_a[h->backend]);   // part_a
     sizes[6] = mixed ? n_gb * (size_t)(h->nv_pad / 8) * 64 * 64 : 0;             // zfrag
     sizes[7] = mixed ? n_gb * (size_t)(h->ns_pad / 8) * 64 * 64 : 0;             // yfrag
     sizes[8] = mixed ? 0 : sizeof(float2) * (size_t)h->nv_pad * h->ncs;          // zbuf

@@ -95,7 +95,7 @@ struct Handle {
     DeviceBuf ybuf;       // float2 [ns_pad][ncs]   weighted signal, coil fastest
     DeviceBuf yfrag;      // MFMA B fragments of the weighted signal (mixed path)
     DeviceBuf part_f;     // float2 [n_seg][256][ncs]   forward partial sums
-    DeviceBuf part_a;     // float2 [n_seg][256][ncs]   adjoint partial sums
+    DeviceBuf part_a;     // float2 [n_seg][groups][256] adjoint partial sums, coils combined per group
     DeviceBuf part_g;     // double [n_seg][batches][256][L] gradient partials
     DeviceBuf dvec;       // float  [ns_pad][ncs] normal weights (coil fastest)
     DeviceBuf io_x;       // float2 [n_grid]  staging for host vectors
@@ -138,6 +138,9 @@ int direct_cb(const Handle *h, bool fwd);   // coils per direct-kernel batch: 8,
 int direct_batches(const Handle *h, bool fwd);
 int mixed_nb(const Handle *h);         // 32-coil groups per wave in the MFMA kernels: 1 or 2
 int mixed_groups(const Handle *h);
+// Coil groups (grid rows) of a backend's adjoint contraction: one partial sum per voxel,
+// segment and group.
+int adj_groups(const Handle *h, int backend);
 hipError_t build_schedule(Sched &sc, int tiles, int n_ks, int slots, int kmax,
                           const std::vector<double> *tile_w = nullptr, const std::vector<double> *k_w = nullptr);
 // Build the forward/adjoint schedules of a backend and size the partial buffers.

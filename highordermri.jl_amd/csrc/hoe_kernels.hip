@@ -53,9 +53,10 @@ namespace hoe {
 // Largest partial-sum buffer of one contraction, in bytes. A schedule whose segments
 // would need more runs as several passes over disjoint tile ranges (each scheduled over
 // the whole device and reduced before the next starts), which share one buffer of the
-// largest pass's size. A segment holds 256 x ncs complex values (64 KiB at 32 FP32
-// coils), and an MFMA adjoint tile has 8 * ceil(nSam / 8 / 8 / 1024) segments, so one
-// schedule of a 96 x 96 x 48 grid at 384 000 samples would hold 5.4 GB. The environment
+// largest pass's size. A forward segment holds 256 x ncs complex values (64 KiB at 32
+// FP32 coils); an adjoint segment holds 256 coil-combined values per coil group (2 KiB),
+// so an MFMA adjoint schedule of a 96 x 96 x 48 grid at 384 000 samples, with
+// 8 * ceil(nSam / 8 / 8 / 1024) segments per tile, holds about 170 MB. The environment
 // variable HOE_PART_BUDGET_MIB (a positive integer, read when a backend's schedules are
 // built) overrides it.
 #ifndef HOE_PART_BUDGET
@@ -223,7 +224,8 @@ struct mfma_t<__bf16> {
 // ---------------------------------------------------------------------------
 //
 // FWD:  part[seg][r][c] = sum_{v in seg} e(s,v) * z[v][c]          (s = 256*tile + r)
-// !FWD: part[seg][r][c] = sum_{s in seg} conj(e(s,v)) * y[s][c]    (v = 256*tile + r)
+// !FWD: part[seg][b][r] = sum_{c in batch b} conj(csm[v][c]) sum_{s in seg} conj(e(s,v)) * y[s][c]
+//                                                                  (v = 256*tile + r)
 // Each lane keeps CB complex accumulators; the streamed record and the CB
 // complex operands of the streamed row arrive through scalar loads. A K unit
 // is 8 streamed rows; rows at or beyond n_stream (padding) are skipped.
@@ -237,7 +239,7 @@ __global__ __launch_bounds__(256) void k_direct(
     const double *__restrict__ coef, int sk, const double *__restrict__ basis, int sb,
     const cplx_t<R> *__restrict__ src, cplx_t<R> *__restrict__ part, int ncs, int n_stream,
     const int4 *__restrict__ segs, const int *__restrict__ chunk_off, int cpx, HornerArgs hx,
-    const double *__restrict__ dtrec) {
+    const double *__restrict__ dtrec, const cplx_t<R> *__restrict__ csm) {
     static_assert(FWD || !DT, "the derivative product is a forward product");
     static_assert(!DT || std::is_same<R, float>::value, "the derivative product runs in FP32");
     const int chunk = chunk_id(cpx);
@@ -316,10 +318,54 @@ __global__ __launch_bounds__(256) void k_direct(
                 ai[c] = fmaT(sn, u.x, ai[c]);
             }
         }
-        cplx_t<R> *o = part + ((size_t)sgi * 256 + threadIdx.x) * ncs + cb0;
+        if constexpr (FWD) {
+            cplx_t<R> *o = part + ((size_t)sgi * 256 + threadIdx.x) * ncs + cb0;
 #pragma unroll
-        for (int c = 0; c < CB; ++c) o[c] = mkc<R>(ar[c], ai[c]);
+            for (int c = 0; c < CB; ++c) o[c] = mkc<R>(ar[c], ai[c]);
+        } else {
+            // coil combination of the batch, sum_c conj(csm[v][c]) * acc[c] (padding coils
+            // have zero maps and zero data): one complex per voxel, segment and batch
+            const cplx_t<R> *s = csm + (size_t)row * ncs + cb0;
+            R xr = R(0), xi = R(0);
+#pragma unroll
+            for (int c = 0; c < CB; ++c) {
+                const cplx_t<R> m = s[c];
+                xr = fmaT(m.x, ar[c], fmaT(m.y, ai[c], xr));
+                xi = fmaT(m.x, ai[c], fmaT(-m.y, ar[c], xi));
+            }
+            part[((size_t)sgi * gridDim.y + blockIdx.y) * 256 + threadIdx.x] = mkc<R>(xr, xi);
+        }
     }
+}
+
+// ---------------------------------------------------------------------------
+// cross-lane reduce-scatter of the MFMA adjoint's coil columns
+// ---------------------------------------------------------------------------
+//
+// x[k], k = 0..15, in every lane of a 16-lane row; afterwards lane l holds the sum of
+// x[l & 15] over the 16 lanes of its row. Each level pairs lane l with l ^ m (m = 15, 7,
+// 2, 1: DPP row_mirror, row_half_mirror, quad_perm [2,3,0,1], quad_perm [1,0,3,2]);
+// the lane whose bit b (b = 3, 2, 1, 0) is set keeps the upper half of its values and
+// sends the lower half, its partner the reverse. The order of the additions is fixed.
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, true));
+}
+template <int N, int CTRL>
+__device__ __forceinline__ void rs_level(float (&x)[16], bool up) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        const float keep = up ? x[k + N] : x[k];
+        const float send = up ? x[k] : x[k + N];
+        x[k] = keep + dpp_f<CTRL>(send);
+    }
+}
+__device__ __forceinline__ float reduce_scatter16(float (&x)[16], int lane) {
+    rs_level<8, 0x140>(x, (lane & 8) != 0);
+    rs_level<4, 0x141>(x, (lane & 4) != 0);
+    rs_level<2, 0x4e>(x, (lane & 2) != 0);
+    rs_level<1, 0xb1>(x, (lane & 1) != 0);
+    return x[0];
 }
 
 // ---------------------------------------------------------------------------
@@ -345,7 +391,8 @@ template <typename T16, bool FWD, bool HX, int L, int NB, int PK = 0>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (FWD ? HOE_WAVES_FWD : HOE_WAVES_ADJ) : 2))) void k_mixed(
     const double *__restrict__ coef, int sk, const double *__restrict__ basis, int sb,
     const uint4 *__restrict__ frag, int n_ks, float2 *__restrict__ part, int ncs,
-    const int4 *__restrict__ segs, const int *__restrict__ chunk_off, int cpx, HornerArgs hx) {
+    const int4 *__restrict__ segs, const int *__restrict__ chunk_off, int cpx, HornerArgs hx,
+    const float2 *__restrict__ csm) {
     typedef typename mfma_t<T16>::vec vec_t;
     static_assert(PK == 0 || NB == 1, "packed coil layouts hold one group");
     constexpr int NQ = PK == 0 ? 4 : (PK == 1 ? 2 : 1);   // B fragments (uint4) per lane and K step
@@ -686,7 +733,85 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (
         // row 8*(r/4) + 4*(l/32) + r%4, column l%32.
         const int col = lane & 31;
         const int hi32 = lane >> 5;
-        if constexpr (PK == 0) {
+        if constexpr (!FWD) {
+            // Adjoint: the coil combination sum_c conj(csm[v][c]) * acc[v][c] of the
+            // workgroup's coils, then one complex per voxel into part[seg][grp][row].
+            // Lane l of a half-wave ends up with row index k = l % 32 of its half:
+            // tile rt = k / 16, register r = k % 16.
+            float xr[2][16], xi[2][16];
+            if constexpr (PK == 0) {
+                // column = coil (grp * NB + b) * 32 + col; tile 0 real, tile 1 imaginary part
+#pragma unroll
+                for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+                    for (int r = 0; r < 16; ++r) {
+                        const int v = d.x * 256 + wave * 64 + rt * 32 + 8 * (r >> 2) + 4 * hi32 + (r & 3);
+                        float sr = 0.0f, si = 0.0f;
+#pragma unroll
+                        for (int b = 0; b < NB; ++b) {
+                            const float2 m = csm[(size_t)v * ncs + (grp * NB + b) * 32 + col];
+                            const float ar = acc[rt][b][0][r], ai = acc[rt][b][NA - 1][r];
+                            sr = fmaf(m.x, ar, fmaf(m.y, ai, sr));
+                            si = fmaf(m.x, ai, fmaf(-m.y, ar, si));
+                        }
+                        xr[rt][r] = sr;
+                        xi[rt][r] = si;
+                    }
+                // first level over the 32 coil lanes (l and l ^ 16) by row swaps: afterwards
+                // row 0 of each half holds tile 0 and row 1 tile 1, each summed over the pair
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    auto a = __builtin_amdgcn_permlane16_swap(__float_as_uint(xr[0][r]), __float_as_uint(xr[1][r]),
+                                                              false, false);
+                    auto b = __builtin_amdgcn_permlane16_swap(__float_as_uint(xi[0][r]), __float_as_uint(xi[1][r]),
+                                                              false, false);
+                    xr[0][r] = __uint_as_float(a[0]) + __uint_as_float(a[1]);
+                    xi[0][r] = __uint_as_float(b[0]) + __uint_as_float(b[1]);
+                }
+            } else {
+                // one FP32 component per column. A row swap of the two tiles' registers puts
+                // tile 0 in row 0 and tile 1 in row 1 of each half: PK = 1 column c = (coil
+                // c % 16, part c / 16), so the swap pairs each coil's real and imaginary
+                // parts; PK = 2 columns c and c + 16 are the hi and lo halves of one
+                // product, added by the swap, and lanes c % 16 = 0..7 hold the real parts of
+                // coils 0..7, lanes 8..15 the imaginary parts (one row rotation by 8 pairs
+                // them; the lanes 8..15 copies get zero weight).
+                const bool row1 = (lane & 16) != 0;
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    auto a = __builtin_amdgcn_permlane16_swap(__float_as_uint(acc[0][0][0][r]),
+                                                              __float_as_uint(acc[1][0][0][r]), false, false);
+                    float re, im;
+                    if constexpr (PK == 1) {
+                        re = __uint_as_float(a[0]);
+                        im = __uint_as_float(a[1]);
+                    } else {
+                        const float t = __uint_as_float(a[0]) + __uint_as_float(a[1]);
+                        const float u = dpp_f<0x128>(t);   // row_ror:8
+                        re = (lane & 8) ? u : t;
+                        im = (lane & 8) ? t : u;
+                    }
+                    xr[0][r] = re;
+                    xi[0][r] = im;
+                }
+                const int coil = PK == 1 ? (lane & 15) : (lane & 7);
+                const bool live = PK == 1 || (lane & 8) == 0;
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const int v = d.x * 256 + wave * 64 + (row1 ? 32 : 0) + 8 * (r >> 2) + 4 * hi32 + (r & 3);
+                    float2 m = csm[(size_t)v * ncs + coil];
+                    if (!live) m = make_float2(0.0f, 0.0f);
+                    const float ar = xr[0][r], ai = xi[0][r];
+                    xr[0][r] = fmaf(m.x, ar, m.y * ai);
+                    xi[0][r] = fmaf(m.x, ai, -m.y * ar);
+                }
+            }
+            const float sr = reduce_scatter16(xr[0], lane);
+            const float si = reduce_scatter16(xi[0], lane);
+            const int k = lane & 31;
+            const int rr = wave * 64 + (k >> 4) * 32 + 8 * ((k & 15) >> 2) + 4 * hi32 + (k & 3);
+            part[((size_t)sgi * gridDim.y + grp) * 256 + rr] = make_float2(sr, si);
+        } else if constexpr (PK == 0) {
 #pragma unroll
             for (int rt = 0; rt < 2; ++rt)
 #pragma unroll
@@ -1470,51 +1595,53 @@ __global__ __launch_bounds__(256) void k_load_ybuf(const cplx_t<R> *__restrict__
     if (maxslot) wave_absmax_to(maxslot, m);
 }
 
-// Adjoint reduction: xm_out[v] = sc * sum_{c < nc} conj(csm[v][c]) * sum_segments part[seg][v%256][c]
+// Adjoint reduction: xm_out[v] = sc * sum_{seg} sum_{g < ngrp} part[seg][g][v % 256]
 //                               (+ lambda * x_in[v]).
-// Half-wave per voxel: lanes over coils, shuffle reduction over 32 lanes. Only the nc real
-// coils are read: the FP32/FP64 direct kernels write partial sums for their coil batches
-// only, so the padding columns beyond them hold no data.
+// The contraction kernels combine the coils of their group (conj(csm) weighting) in their
+// epilogues, so one complex per voxel, segment and coil group remains. One thread per
+// voxel; the segments of its tile are added in list order, the groups in index order.
 template <typename R>
 __global__ __launch_bounds__(256) void k_reduce_adj(
     const cplx_t<R> *__restrict__ part, const int *__restrict__ tile_off, const int *__restrict__ tile_list,
-    int nv_pad, int ncs, int nc, const cplx_t<R> *__restrict__ csm, int nv, R scale,
-    const unsigned *__restrict__ scale_slot, cplx_t<R> *__restrict__ xm_out, R lambda,
-    const cplx_t<R> *__restrict__ x_in, int v_base, int nv_write) {
-    const int v = v_base + ((blockIdx.x * blockDim.x + threadIdx.x) >> 5);   // half-wave id = voxel
-    const int cl = threadIdx.x & 31;
+    int ngrp, int nv_pad, int nv, R scale, const unsigned *__restrict__ scale_slot, cplx_t<R> *__restrict__ xm_out,
+    R lambda, const cplx_t<R> *__restrict__ x_in, int v_base, int v_end, int nv_write) {
+    const int v = v_base + blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= v_end) return;
     R sc = scale;
     if constexpr (std::is_same<R, float>::value) {
         if (scale_slot) sc = scale / pow2_scale(__uint_as_float(*scale_slot));
     }
-    R ar = R(0), ai = R(0);
-    if (v < nv_pad) {
-        const int tile = v >> 8;
-        const int *list = tile_list + tile_off[tile];
-        const int nseg = tile_off[tile + 1] - tile_off[tile];
-        // with nc <= 16 the 32 / ncl lanes of a coil split its segments (reduce_cols)
-        const int ncl = reduce_cols(nc);
-        for (int c = cl & (ncl - 1); c < nc; c += 32) {
-            const cplx_t<R> acc = sum_segments<R>(part, list, nseg, v & 255, c, ncs, cl / ncl, 32 / ncl);
-            const cplx_t<R> s = csm[(size_t)v * ncs + c];
-            ar += s.x * acc.x + s.y * acc.y;   // conj(s) * acc
-            ai += s.x * acc.y - s.y * acc.x;
+    const int tile = v >> 8, r = v & 255;
+    const int *list = tile_list + tile_off[tile];
+    const int n = (tile_off[tile + 1] - tile_off[tile]) * ngrp;
+    // entry i = (segment i / ngrp, group i % ngrp); four sums in flight
+    auto at = [&](int i) { return part[((size_t)list[i / ngrp] * ngrp + i % ngrp) * 256 + r]; };
+    cplx_t<R> a[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) a[q] = mkc<R>(R(0), R(0));
+    int i = 0;
+    for (; i + 3 < n; i += 4) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const cplx_t<R> t = at(i + q);
+            a[q].x += t.x;
+            a[q].y += t.y;
         }
     }
-    for (int o = 16; o > 0; o >>= 1) {
-        ar += __shfl_xor(ar, o, 32);
-        ai += __shfl_xor(ai, o, 32);
+    for (; i < n; ++i) {
+        const cplx_t<R> t = at(i);
+        a[0].x += t.x;
+        a[0].y += t.y;
     }
-    if (cl == 0 && v < nv_pad) {
-        cplx_t<R> r = mkc<R>(ar * sc, ai * sc);
-        if (v >= nv) r = mkc<R>(R(0), R(0));
-        if (x_in && v < nv) {
-            const cplx_t<R> t = x_in[v];
-            r.x += lambda * t.x;
-            r.y += lambda * t.y;
-        }
-        if (v < nv_write) xm_out[v] = r;
+    cplx_t<R> res = mkc<R>(((a[0].x + a[1].x) + (a[2].x + a[3].x)) * sc, ((a[0].y + a[1].y) + (a[2].y + a[3].y)) * sc);
+    if (v >= nv) res = mkc<R>(R(0), R(0));
+    if (x_in && v < nv) {
+        const cplx_t<R> t = x_in[v];
+        res.x += lambda * t.x;
+        res.y += lambda * t.y;
     }
+    if (v < nv_write) xm_out[v] = res;
+    (void)nv_pad;
 }
 
 // gk[l][s] = scale * sum_segments sum_batches part[seg][batch][s%256][l]  (FP64)
@@ -2220,7 +2347,9 @@ hipError_t prepare_backend(Handle *h, int backend) {
     const int slots_a = std::max(8, h->cu_count * occ_a / groups_a);
     hipError_t e;
     const int kmax_a = is_mixed ? HOE_MFMA_ADJ_KMAX : 0;
+    // forward segments hold every coil; adjoint segments one coil-combined value per group
     const int64_t seg_bytes = (int64_t)256 * h->ncs * h->cbytes, budget = part_budget();
+    const int64_t seg_bytes_a = (int64_t)256 * groups_a * h->cbytes;
     // the Horner forward (MFMA and FP32) weighs its run-crossing K units
     const bool fweighted = h->horner && !h->dt_on &&
                            std::any_of(h->fwd_k_w.begin(), h->fwd_k_w.end(), [](double w) { return w != 1.0; });
@@ -2230,12 +2359,17 @@ hipError_t prepare_backend(Handle *h, int backend) {
     const bool weighted = is_mixed &&
                           std::any_of(h->adj_tile_w.begin(), h->adj_tile_w.end(), [](double w) { return w != 1.0; });
     if ((e = build_passes(sa, (int)(h->nv_pad / 256), (int)(h->ns_pad / 8), slots_a, kmax_a,
-                          weighted ? &h->adj_tile_w : nullptr, nullptr, seg_bytes, budget)) != hipSuccess)
+                          weighted ? &h->adj_tile_w : nullptr, nullptr, seg_bytes_a, budget)) != hipSuccess)
         return e;
     h->occ_f[backend] = occ_f;
     h->occ_a[backend] = occ_a;
     if ((e = grow(h->part_f, (size_t)max_segments(sf) * seg_bytes)) != hipSuccess) return e;
-    return grow(h->part_a, (size_t)max_segments(sa) * seg_bytes);
+    return grow(h->part_a, (size_t)max_segments(sa) * seg_bytes_a);
+}
+
+int adj_groups(const Handle *h, int backend) {
+    const bool is_mixed = backend == HOE_BACKEND_MIXED_FP16 || backend == HOE_BACKEND_MIXED_BF16;
+    return is_mixed ? mixed_groups(h) : direct_batches(h, false);
 }
 
 hipError_t prepare_grad(Handle *h) {
@@ -2304,12 +2438,12 @@ static hipError_t prepare_grad_mfma(Handle *h) {
 // from the instance (float2 or double2).
 template <typename C>
 static void launch_direct(void (*k)(const double *, int, const double *, int, const C *, C *, int, int, const int4 *,
-                                    const int *, int, HornerArgs, const double *),
+                                    const int *, int, HornerArgs, const double *, const C *),
                           dim3 grid, hipStream_t st, const double *coef, int sk, const double *basis, int sb,
                           const void *operand, void *part, int ncs, int n_stream, const int4 *segs, const int *coff,
-                          int cpx, HornerArgs hx, const double *dtrec) {
+                          int cpx, HornerArgs hx, const double *dtrec, const void *csm) {
     k<<<grid, 256, 0, st>>>(coef, sk, basis, sb, (const C *)operand, (C *)part, ncs, n_stream, segs, coff, cpx, hx,
-                            dtrec);
+                            dtrec, (const C *)csm);
 }
 
 template <bool FWD>
@@ -2324,13 +2458,13 @@ static void launch_contract(Handle *h, const Sched &sc, const void *operand, voi
         const int n_stream = (int)(FWD ? h->n_vox : h->n_sam);
         visit_contract_direct<FWD>(h, [&](auto k) {
             launch_direct(k, grid, h->stream, coef, h->sk, basis, h->sb, operand, part, h->ncs, n_stream, segs, coff,
-                          sc.cpx, hx, (const double *)h->dtrec.p);
+                          sc.cpx, hx, (const double *)h->dtrec.p, h->csm.p);
         });
     } else {
         dim3 grid(sc.n_chunks, mixed_groups(h));
         visit_contract_mixed<FWD>(h, backend, [&](auto k) {
             k<<<grid, 256, 0, h->stream>>>(coef, h->sk, basis, h->sb, (const uint4 *)operand, sc.n_ks,
-                                           (float2 *)part, h->ncs, segs, coff, sc.cpx, hx);
+                                           (float2 *)part, h->ncs, segs, coff, sc.cpx, hx, (const float2 *)h->csm.p);
         });
     }
 }
@@ -2491,14 +2625,13 @@ static hipError_t adjoint_passes(Handle *h, bool time_it, double scale, void *xm
 // are not written (a caller's grid-sized output has no padding).
 static hipError_t adjoint_reduce(Handle *h, const Sched &sc, double scale, void *xm_out, double lambda,
                                  const void *x_in, int nv_write) {
-    const int64_t threads = (int64_t)(sc.t_hi - sc.t_lo) * 256 * 32;
+    const int64_t threads = (int64_t)(sc.t_hi - sc.t_lo) * 256;
     by_prec(h, [&](auto r) {
         using R = decltype(r);
         k_reduce_adj<R><<<blocks_for(threads), 256, 0, h->stream>>>(
-            (const cplx_t<R> *)h->part_a.p, (const int *)sc.tile_off.p, (const int *)sc.tile_list.p, (int)h->nv_pad,
-            h->ncs, (int)h->n_cha, (const cplx_t<R> *)h->csm.p, (int)h->n_vox, (R)scale,
-            mixed(h) ? slot(h, 1) : nullptr,
-            (cplx_t<R> *)xm_out, (R)lambda, (const cplx_t<R> *)x_in, sc.t_lo * 256, nv_write);
+            (const cplx_t<R> *)h->part_a.p, (const int *)sc.tile_off.p, (const int *)sc.tile_list.p,
+            adj_groups(h, h->backend), (int)h->nv_pad, (int)h->n_vox, (R)scale, mixed(h) ? slot(h, 1) : nullptr,
+            (cplx_t<R> *)xm_out, (R)lambda, (const cplx_t<R> *)x_in, sc.t_lo * 256, sc.t_hi * 256, nv_write);
     });
     return hipGetLastError();
 }
