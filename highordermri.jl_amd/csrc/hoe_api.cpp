@@ -305,7 +305,7 @@ void release(Handle *h) {
                               &h->part_a,   &h->part_g, &h->dvec,   &h->io_x,    &h->io_x2,  &h->io_y,   &h->io_y2,
                               &h->scal,     &h->cg_vec, &h->red,    &h->gk,      &h->cg_log, &h->gzfrag, &h->grfrag,
                               &h->gbfrag,   &h->gbscale, &h->part_gm, &h->ghbuf, &h->growmap, &h->dtrec,
-                              &h->traces_d, &h->times_d,  &h->act_d};
+                              &h->traces_d, &h->times_d,  &h->act_d,   &h->units,   &h->uvox};
     for (auto *b : bufs) dfree(*b);
     for (int b = 0; b < 4; ++b) {
         hoe::drop_passes(h->sched_f[b]);
@@ -548,6 +548,8 @@ int create_impl(const Desc *d, hoe_handle **out) {
     std::vector<double> hrec((size_t)h->nv_pad * 2, 0.0);
     std::vector<int32_t> vrun((size_t)h->nv_pad, 0);
     std::vector<double> beta;
+    std::vector<hoe::HoeUnit> units;
+    std::vector<hoe::HoeUnitVox> uvox;
     {
         int run = -1;
         T py = T(0), pz = T(0);
@@ -611,6 +613,28 @@ int create_impl(const Desc *d, hoe_handle **out) {
         if (h->horner)
             for (int64_t k = 0; k < h->nv_pad / 8; ++k)
                 if (vrun[(size_t)(8 * k)] != vrun[(size_t)(8 * k + 7)]) h->fwd_k_w[(size_t)k] = HOE_FWD_CROSS_W;
+        // K units of the MFMA Horner forward: the anchor is the middle of the real voxels'
+        // x range (rounded to FP32) and the fieldmap of the first real voxel; padding
+        // voxels get u = dfm = 0
+        units.assign((size_t)(h->nv_pad / 8), hoe::HoeUnit{});
+        uvox.assign((size_t)(h->nv_pad / 8), hoe::HoeUnitVox{});
+        for (int64_t k = 0; k < h->nv_pad / 8; ++k) {
+            hoe::HoeUnit &a = units[(size_t)k];
+            const int64_t v0 = 8 * k, v1 = std::min<int64_t>(8 * k + 8, nv);
+            a.run = vrun[(size_t)v0] == vrun[(size_t)(8 * k + 7)] ? vrun[(size_t)v0] : -1;
+            if (a.run < 0 || v0 >= v1) continue;
+            double lo = hrec[2 * (size_t)v0], hi = lo;
+            for (int64_t v = v0; v < v1; ++v) {
+                lo = std::min(lo, hrec[2 * (size_t)v]);
+                hi = std::max(hi, hrec[2 * (size_t)v]);
+            }
+            a.xa = (float)(0.5 * (lo + hi));
+            a.fma = (float)hrec[2 * (size_t)v0 + 1];
+            for (int64_t v = v0; v < v1; ++v) {
+                uvox[(size_t)k].ud[v - v0][0] = (float)(hrec[2 * (size_t)v] - (double)a.xa);
+                uvox[(size_t)k].ud[v - v0][1] = (float)(hrec[2 * (size_t)v + 1] - (double)a.fma);
+            }
+        }
     }
 
     hipDeviceProp_t prop;
@@ -653,6 +677,8 @@ int create_impl(const Desc *d, hoe_handle **out) {
         {&h->hrec, sizeof(double) * hrec.size()},
         {&h->vrun, sizeof(int32_t) * vrun.size()},
         {&h->beta, sizeof(double) * std::max<size_t>(beta.size(), 1)},
+        {&h->units, sizeof(hoe::HoeUnit) * units.size()},
+        {&h->uvox, sizeof(hoe::HoeUnitVox) * uvox.size()},
         {&h->xm, cb * (size_t)h->nv_pad},
         {&h->zbuf, cb * (size_t)h->nv_pad * h->ncs},
         {&h->zfrag, F64 ? 0 : (size_t)n_gb_pad * (size_t)(h->nv_pad / 8) * 64 * 64},
@@ -681,6 +707,10 @@ int create_impl(const Desc *d, hoe_handle **out) {
     if (e == hipSuccess) e = hipMemcpy(h->vrun.p, vrun.data(), sizeof(int32_t) * vrun.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess && !beta.empty())
         e = hipMemcpy(h->beta.p, beta.data(), sizeof(double) * beta.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(h->units.p, units.data(), sizeof(hoe::HoeUnit) * units.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(h->uvox.p, uvox.data(), sizeof(hoe::HoeUnitVox) * uvox.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess && !F64)
         e = hipMemcpy(h->times_d.p, h->times.data(), sizeof(float) * (size_t)ns, hipMemcpyHostToDevice);
     if (e == hipSuccess && !h->active.empty())

@@ -23,6 +23,22 @@ struct DeviceBuf {
     size_t bytes = 0;
 };
 
+// K unit (8 consecutive voxels) of the MFMA Horner forward. For a unit inside one run,
+// x_a (an FP32 value: the middle of its voxels' x range, rounded) and fm_a (the fieldmap
+// of its first voxel) anchor the phase: voxel j carries u_j = x_j - x_a and
+// dfm_j = fieldmap_j - fm_a in FP32 (HoeUnitVox), so that its phase is
+// P(x_a) + t * fm_a (FP64, reduced modulo one cycle) plus a cubic in u_j and t * dfm_j
+// (FP32). run = -1 marks a unit that crosses a run boundary.
+struct alignas(16) HoeUnit {
+    int32_t run;
+    float xa;
+    float fma;
+    float pad;
+};
+struct alignas(64) HoeUnitVox {
+    float ud[8][2];   // (u_j, dfm_j)
+};
+
 // Stream-K schedule of one contraction (see build_schedule in hoe_kernels.hip), or of one
 // pass of it: a pass covers the stationary tiles [t_lo, t_hi) over the whole K range.
 struct Sched {
@@ -85,6 +101,8 @@ struct Handle {
     DeviceBuf dtrec;      // double [ns_pad][sk] kspha_dt records (derivative product), t slot 0
     DeviceBuf csm;        // float2 [nv_pad][ncs]
     DeviceBuf hrec;       // double2 [nv_pad]: (x, fieldmap) of each voxel, for the Horner forward
+    DeviceBuf units;      // HoeUnit [nv_pad / 8]: K-unit anchors of the MFMA Horner forward
+    DeviceBuf uvox;       // HoeUnitVox [nv_pad / 8]: voxel offsets from the anchors
     DeviceBuf vrun;       // int [nv_pad]: run of each voxel
     DeviceBuf beta;       // double [n_runs][LK][4]: x-polynomial coefficients of each record entry
 

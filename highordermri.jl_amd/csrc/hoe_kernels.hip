@@ -99,6 +99,13 @@ __device__ __forceinline__ void sincos_cycles(double ph, float &sn, float &cs) {
     cs = __builtin_amdgcn_cosf(r);
 }
 
+// The same from an FP32 phase in cycles (any magnitude below 2^23): v_fract_f32 is exact.
+__device__ __forceinline__ void sincos_cycles(float ph, float &sn, float &cs) {
+    const float r = __builtin_amdgcn_fractf(ph);
+    sn = __builtin_amdgcn_sinf(r);
+    cs = __builtin_amdgcn_cosf(r);
+}
+
 // exp(2 pi i ph) in FP64 (the FP64 backend): ph - rint(ph) in [-1/2, 1/2] is exact, then
 // the device math library's sincospi of twice it.
 __device__ __forceinline__ void sincos_cycles(double ph, double &sn, double &cs) {
@@ -200,6 +207,8 @@ struct HornerArgs {
     const int *vrun;       // [nv_pad] run of each voxel
     const double *beta;    // [n_runs][L][4]
     int t_index;           // position of t in the sample record
+    const HoeUnit *units;     // [nv_pad / 8] K-unit anchors of the MFMA Horner forward
+    const HoeUnitVox *uvox;   // [nv_pad / 8] voxel offsets from the anchors
 };
 
 template <typename T16>
@@ -426,11 +435,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (
         // step ks (moved into the A layout by permlane32 swaps); without, the
         // A layout directly (rows lane%32 and 32 + lane%32, streamed rows
         // 4*(lane/32)..+3 each).
-        auto kloop = [&](auto phases_of, auto swap) {
-            // one K step: phases of the 8 streamed rows, exp(i phi) split hi/lo into the
-            // A layout, 12 MFMAs per coil group against the B fragments bfr
+        auto kloop = [&](auto phases_of, auto swap, auto ptype) {
+            // one K step: phases of the 8 streamed rows (in cycles: FP64, or FP32 from the
+            // anchored Horner forward), exp(i phi) split hi/lo into the A layout, 12 MFMAs
+            // per coil group against the B fragments bfr
             auto step = [&](int ks, const uint4 (&bfr)[NB][NQ]) {
-                double ph[8];
+                decltype(ptype) ph[8];
                 phases_of(ks, ph);
                 unsigned hp[8], lp[8];
 #pragma unroll
@@ -566,27 +576,50 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (
                     }
                     ph[j] = p;
                 }
-            }, std::true_type{});
+            }, std::true_type{}, double{});
         };
         if constexpr (HX && FWD) {
-            // Horner forward: c_j set up per run, 4 FP64 FMAs per phase
-            // Runs are contiguous, so the 8 rows of a K step share one run when
-            // the first and the last do: then one run check per K step.
+            // Horner forward. Runs are contiguous, so the 8 rows of a K step share one run
+            // when the first and the last do: then one run check per K step, and the step's
+            // phases are anchored: P(x) = c0 + x (c1 + x (c2 + x c3)) of the run (c_j set up
+            // per run in FP64) and t * fieldmap are evaluated in FP64 at the unit's anchor
+            // (x_a, fm_a) and reduced modulo one cycle; the offsets of the 8 voxels,
+            //   P(x_a + u) - P(x_a) = u (d1 + u (d2 + u c3)),  d1 = P'(x_a), d2 = P''(x_a) / 2,
+            // and t * (fieldmap - fm_a), are added in FP32 (|u| <= half the unit's x range,
+            // so the FP32 terms stay within a few cycles): 4 FP32 FMAs per phase instead of
+            // 4 FP64 FMAs, a fract and a conversion.
             const double t = coef[(size_t)row * sk + hx.t_index];
+            const float tf = (float)t;
             double hc[4] = {0.0, 0.0, 0.0, 0.0};
+            float c1f = 0.0f, c2f = 0.0f, c2x2f = 0.0f, c3f = 0.0f, c3x3f = 0.0f;
             int cur = -1;
-            kloop([&](int ks, double *ph) {
+            // the anchor of the next K unit is fetched during this step, so that the step's
+            // branch does not wait on a scalar load before the unit's voxel offsets are fetched
+            HoeUnit an = hx.units[d.y];
+            kloop([&](int ks, float *ph) {
                 const int q0 = ks * 8;
-                const int r0 = hx.vrun[q0], r7 = hx.vrun[q0 + 7];
-                if (r0 == r7) {
+                const HoeUnit a = an;
+                const int r0 = a.run;
+                if (r0 >= 0) {
+                    const HoeUnitVox &uv = hx.uvox[ks];
                     if (r0 != cur) {
                         run_setup<L>(coef + (size_t)row * sk, hx.beta + (size_t)r0 * L * 4, hc);
+                        c1f = (float)hc[1];
+                        c2f = (float)hc[2];
+                        c2x2f = (float)(2.0 * hc[2]);
+                        c3f = (float)hc[3];
+                        c3x3f = (float)(3.0 * hc[3]);
                         cur = r0;
                     }
+                    const double xa = (double)a.xa;
+                    const double pa = fma(t, (double)a.fma, fma(fma(fma(hc[3], xa, hc[2]), xa, hc[1]), xa, hc[0]));
+                    const float base = (float)__builtin_amdgcn_fract(pa);
+                    const float d2 = fmaf(c3x3f, a.xa, c2f);
+                    const float d1 = fmaf(fmaf(c3x3f, a.xa, c2x2f), a.xa, c1f);
 #pragma unroll
                     for (int j = 0; j < 8; ++j) {
-                        const double2 xd = hx.hrec[q0 + j];
-                        ph[j] = fma(t, xd.y, fma(fma(fma(hc[3], xd.x, hc[2]), xd.x, hc[1]), xd.x, hc[0]));
+                        const float u = uv.ud[j][0], dfm = uv.ud[j][1];
+                        ph[j] = fmaf(tf, dfm, fmaf(fmaf(fmaf(c3f, u, d2), u, d1), u, base));
                     }
                 } else {
                     // a run boundary inside the K step: generic phase from the full records
@@ -600,10 +633,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (
                         double p = a[0];
 #pragma unroll
                         for (int i = 1; i <= L; ++i) p = fma(a[i], sm[i - 1], p);
-                        ph[j] = p;
+                        ph[j] = (float)__builtin_amdgcn_fract(p);
                     }
                 }
-            }, std::true_type{});
+                an = hx.units[min(ks + 1, n_ks - 1)];
+            }, std::true_type{}, float{});
         } else if constexpr (HX && !FWD) {
             // A wave whose 64 voxels span three or more runs (short rows of a mask)
             // takes the generic phase; the decision is uniform over the wave.
@@ -724,7 +758,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (
                         ph[4 + i] = fma(t, h1.y, horner(c, h1.x));
                     }
                 }
-            }, std::false_type{});
+            }, std::false_type{}, double{});
             }
         } else {
             generic_path();
@@ -2452,7 +2486,8 @@ static void launch_contract(Handle *h, const Sched &sc, const void *operand, voi
     const double *coef = (const double *)h->coef.p, *basis = (const double *)h->basis.p;
     const int4 *segs = (const int4 *)sc.seg.p;
     const int *coff = (const int *)sc.chunk_off.p;
-    HornerArgs hx{(const double2 *)h->hrec.p, (const int *)h->vrun.p, (const double *)h->beta.p, h->t_index};
+    HornerArgs hx{(const double2 *)h->hrec.p, (const int *)h->vrun.p, (const double *)h->beta.p, h->t_index,
+                  (const HoeUnit *)h->units.p, (const HoeUnitVox *)h->uvox.p};
     if (backend == HOE_BACKEND_DIRECT || backend == HOE_BACKEND_FP64) {
         dim3 grid(sc.n_chunks, direct_batches(h, FWD));
         const int n_stream = (int)(FWD ? h->n_vox : h->n_sam);
@@ -2676,7 +2711,8 @@ static hipError_t run_grad_mfma(Handle *h, bool time_it) {
         k_build_grfrag<__bf16><<<blocks_for(nr), 256, 0, h->stream>>>(
             (const float2 *)h->ybuf.p, (uint4 *)h->grfrag.p, n_st, nkc, h->ncs, slots, h->csm_max);
     }
-    HornerArgs hx{(const double2 *)h->hrec.p, (const int *)h->vrun.p, (const double *)h->beta.p, h->t_index};
+    HornerArgs hx{(const double2 *)h->hrec.p, (const int *)h->vrun.p, (const double *)h->beta.p, h->t_index,
+                  (const HoeUnit *)h->units.p, (const HoeUnitVox *)h->uvox.p};
     if (time_it) (void)hipEventRecord(h->ev[0], h->stream);
     visit_gmixed(h, [&](auto k) {
         k<<<sc.n_chunks, 256, kGradLds, h->stream>>>(
