@@ -1016,6 +1016,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         // Horner: t of the lane's sample; generic: its full record
         double st[HX ? 1 : L + 1];
         double hc[4] = {0.0, 0.0, 0.0, 0.0};
+        float c1f = 0.0f, c2f = 0.0f, c2x2f = 0.0f, c3f = 0.0f, c3x3f = 0.0f;
         int cur = -1;
         if (HX) {
             st[0] = coef[(size_t)row * sk + hx.t_index];
@@ -1066,18 +1067,38 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                 const int q0 = kb * 32 + h * 16;
                 float cs[16], sn[16];
                 if constexpr (HX) {
-                    const int r0 = hx.vrun[q0], r15 = hx.vrun[q0 + 15];
-                    if (r0 == r15) {
+                    // the half block is K units 2 (q0 / 16) and 2 (q0 / 16) + 1; inside one run
+                    // their phases are anchored as in the MFMA forward (FP64 cubic and
+                    // fieldmap term at each unit's anchor, FP32 offsets of its 8 voxels)
+                    const HoeUnit a0 = hx.units[q0 / 8], a1 = hx.units[q0 / 8 + 1];
+                    const int r0 = a0.run;
+                    if (r0 >= 0 && a1.run == r0) {
                         if (r0 != cur) {
                             run_setup<L>(coef + (size_t)row * sk, hx.beta + (size_t)r0 * L * 4, hc);
+                            c1f = (float)hc[1];
+                            c2f = (float)hc[2];
+                            c2x2f = (float)(2.0 * hc[2]);
+                            c3f = (float)hc[3];
+                            c3x3f = (float)(3.0 * hc[3]);
                             cur = r0;
                         }
+                        const float tf = (float)st[0];
 #pragma unroll
-                        for (int j = 0; j < 16; ++j) {
-                            const double2 xd = hx.hrec[q0 + j];
-                            sincos_cycles(
-                                fma(st[0], xd.y, fma(fma(fma(hc[3], xd.x, hc[2]), xd.x, hc[1]), xd.x, hc[0])),
-                                sn[j], cs[j]);
+                        for (int uh = 0; uh < 2; ++uh) {
+                            const HoeUnit &a = uh ? a1 : a0;
+                            const HoeUnitVox &uv = hx.uvox[q0 / 8 + uh];
+                            const double xa = (double)a.xa;
+                            const double pa =
+                                fma(st[0], (double)a.fma, fma(fma(fma(hc[3], xa, hc[2]), xa, hc[1]), xa, hc[0]));
+                            const float base = (float)__builtin_amdgcn_fract(pa);
+                            const float d2 = fmaf(c3x3f, a.xa, c2f);
+                            const float d1 = fmaf(fmaf(c3x3f, a.xa, c2x2f), a.xa, c1f);
+#pragma unroll
+                            for (int j = 0; j < 8; ++j) {
+                                const float u = uv.ud[j][0], dfm = uv.ud[j][1];
+                                sincos_cycles(fmaf(tf, dfm, fmaf(fmaf(fmaf(c3f, u, d2), u, d1), u, base)),
+                                              sn[8 * uh + j], cs[8 * uh + j]);
+                            }
                         }
                     } else {
                         // a run boundary inside the half block: full records (generic phase)
