@@ -305,7 +305,7 @@ void release(Handle *h) {
                               &h->part_a,   &h->part_g, &h->dvec,   &h->io_x,    &h->io_x2,  &h->io_y,   &h->io_y2,
                               &h->scal,     &h->cg_vec, &h->red,    &h->gk,      &h->cg_log, &h->gzfrag, &h->grfrag,
                               &h->gbfrag,   &h->gbscale, &h->part_gm, &h->ghbuf, &h->growmap, &h->dtrec,
-                              &h->traces_d, &h->times_d,  &h->act_d,   &h->units,   &h->uvox};
+                              &h->traces_d, &h->times_d,  &h->act_d,   &h->units,   &h->uvox,    &h->aanc,    &h->aud};
     for (auto *b : bufs) dfree(*b);
     for (int b = 0; b < 4; ++b) {
         hoe::drop_passes(h->sched_f[b]);
@@ -550,6 +550,8 @@ int create_impl(const Desc *d, hoe_handle **out) {
     std::vector<double> beta;
     std::vector<hoe::HoeUnit> units;
     std::vector<hoe::HoeUnitVox> uvox;
+    std::vector<hoe::HoeAdjAnchor> aanc;
+    std::vector<float> aud;
     {
         int run = -1;
         T py = T(0), pz = T(0);
@@ -616,6 +618,33 @@ int create_impl(const Desc *d, hoe_handle **out) {
         // K units of the MFMA Horner forward: the anchor is the middle of the real voxels'
         // x range (rounded to FP32) and the fieldmap of the first real voxel; padding
         // voxels get u = dfm = 0
+        // group anchors of the MFMA Horner adjoint (waves spanning at most two runs)
+        aanc.assign((size_t)(h->nv_pad / 64), hoe::HoeAdjAnchor{});
+        aud.assign((size_t)h->nv_pad * 2, 0.0f);
+        for (int64_t w = 0; w < h->nv_pad / 64; ++w) {
+            const int r_lo = vrun[(size_t)(64 * w)];
+            if (vrun[(size_t)(64 * w + 63)] - r_lo > 1) continue;
+            double lo[4], hi[4];
+            bool seen[4] = {false, false, false, false};
+            for (int64_t v = 64 * w; v < std::min<int64_t>(64 * w + 64, nv); ++v) {
+                const int c = 2 * (vrun[(size_t)v] - r_lo) + (int)((v % 64) / 32);
+                const double x = hrec[2 * (size_t)v];
+                if (!seen[c]) {
+                    seen[c] = true;
+                    lo[c] = hi[c] = x;
+                    aanc[(size_t)w].fm[c] = (float)hrec[2 * (size_t)v + 1];
+                }
+                lo[c] = std::min(lo[c], x);
+                hi[c] = std::max(hi[c], x);
+            }
+            for (int c = 0; c < 4; ++c)
+                if (seen[c]) aanc[(size_t)w].xa[c] = (float)(0.5 * (lo[c] + hi[c]));
+            for (int64_t v = 64 * w; v < std::min<int64_t>(64 * w + 64, nv); ++v) {
+                const int c = 2 * (vrun[(size_t)v] - r_lo) + (int)((v % 64) / 32);
+                aud[2 * (size_t)v] = (float)(hrec[2 * (size_t)v] - (double)aanc[(size_t)w].xa[c]);
+                aud[2 * (size_t)v + 1] = (float)(hrec[2 * (size_t)v + 1] - (double)aanc[(size_t)w].fm[c]);
+            }
+        }
         units.assign((size_t)(h->nv_pad / 8), hoe::HoeUnit{});
         uvox.assign((size_t)(h->nv_pad / 8), hoe::HoeUnitVox{});
         for (int64_t k = 0; k < h->nv_pad / 8; ++k) {
@@ -679,6 +708,8 @@ int create_impl(const Desc *d, hoe_handle **out) {
         {&h->beta, sizeof(double) * std::max<size_t>(beta.size(), 1)},
         {&h->units, sizeof(hoe::HoeUnit) * units.size()},
         {&h->uvox, sizeof(hoe::HoeUnitVox) * uvox.size()},
+        {&h->aanc, sizeof(hoe::HoeAdjAnchor) * aanc.size()},
+        {&h->aud, sizeof(float) * aud.size()},
         {&h->xm, cb * (size_t)h->nv_pad},
         {&h->zbuf, cb * (size_t)h->nv_pad * h->ncs},
         {&h->zfrag, F64 ? 0 : (size_t)n_gb_pad * (size_t)(h->nv_pad / 8) * 64 * 64},
@@ -711,6 +742,9 @@ int create_impl(const Desc *d, hoe_handle **out) {
         e = hipMemcpy(h->units.p, units.data(), sizeof(hoe::HoeUnit) * units.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemcpy(h->uvox.p, uvox.data(), sizeof(hoe::HoeUnitVox) * uvox.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(h->aanc.p, aanc.data(), sizeof(hoe::HoeAdjAnchor) * aanc.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(h->aud.p, aud.data(), sizeof(float) * aud.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess && !F64)
         e = hipMemcpy(h->times_d.p, h->times.data(), sizeof(float) * (size_t)ns, hipMemcpyHostToDevice);
     if (e == hipSuccess && !h->active.empty())

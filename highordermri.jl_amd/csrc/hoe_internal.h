@@ -38,6 +38,13 @@ struct alignas(16) HoeUnit {
 struct alignas(64) HoeUnitVox {
     float ud[8][2];   // (u_j, dfm_j)
 };
+// Anchors of a 64-voxel wave of the MFMA Horner adjoint: group c = 2 g + h holds the wave's
+// voxels of run r_lo + g in wave half h (rows 32 h .. 32 h + 31); x_c is the middle of their
+// x range (rounded to FP32), fm_c the fieldmap of the group's first voxel.
+struct alignas(32) HoeAdjAnchor {
+    float xa[4];
+    float fm[4];
+};
 
 // Stream-K schedule of one contraction (see build_schedule in hoe_kernels.hip), or of one
 // pass of it: a pass covers the stationary tiles [t_lo, t_hi) over the whole K range.
@@ -103,6 +110,8 @@ struct Handle {
     DeviceBuf hrec;       // double2 [nv_pad]: (x, fieldmap) of each voxel, for the Horner forward
     DeviceBuf units;      // HoeUnit [nv_pad / 8]: K-unit anchors of the MFMA Horner forward
     DeviceBuf uvox;       // HoeUnitVox [nv_pad / 8]: voxel offsets from the anchors
+    DeviceBuf aanc;       // HoeAdjAnchor [nv_pad / 64]: group anchors of the MFMA Horner adjoint
+    DeviceBuf aud;        // float2 [nv_pad]: (x - x_c, fieldmap - fm_c) against the voxel's adjoint group
     DeviceBuf vrun;       // int [nv_pad]: run of each voxel
     DeviceBuf beta;       // double [n_runs][LK][4]: x-polynomial coefficients of each record entry
 

@@ -209,6 +209,8 @@ struct HornerArgs {
     int t_index;           // position of t in the sample record
     const HoeUnit *units;     // [nv_pad / 8] K-unit anchors of the MFMA Horner forward
     const HoeUnitVox *uvox;   // [nv_pad / 8] voxel offsets from the anchors
+    const HoeAdjAnchor *aanc; // [nv_pad / 64] group anchors of the MFMA Horner adjoint
+    const float2 *aud;        // [nv_pad] voxel offsets from the adjoint group anchors
 };
 
 template <typename T16>
@@ -647,118 +649,121 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (
                 generic_path();
             } else {
             // Horner adjoint. The wave's 64 voxels lie in at most two runs (r_lo, r_hi).
+            // Its voxel rows fall into up to four groups c = 2 g + h: run r_lo + g, wave half
+            // h (rows 0..31 or 32..63, the lane's two voxel rows). Each group has an anchor
+            // (x_c, fm_c) and its voxels carry u = x - x_c, dfm = fieldmap - fm_c (FP32).
             // For each pair of K steps, FP64 matrix products give every coefficient the
-            // pair needs: rows = its 16 streamed samples, columns = c_j(s, run) =
-            // sum_i A_s[i] beta[run][i][j] for j = 0..3, one product chain per run. The
-            // results and the samples' t_s go through a wave-private LDS slot; each lane
-            // reads the (c0..c3, t) of its samples and runs and spends 4 FP64 FMAs per phase
-            // on its two voxel rows, already in the A-operand layout.
+            // pair needs: rows = its 16 streamed samples, columns = the Taylor coefficients
+            // of the sample's run cubic at x_c, e_0 = P(x_c) + t_s fm_c, e_k = P^(k)(x_c) / k!
+            // (k = 1..3), one product chain per group, from the run's x-polynomial table
+            // re-centred at x_c (and fm_c in the t row). e_0 is reduced modulo one cycle in
+            // FP64; the coefficients and the samples' t_s go to a wave-private LDS slot in
+            // FP32, and each lane spends 4 FP32 FMAs per phase,
+            //   phi = e_0 + u (e_1 + u (e_2 + u e_3)) + t_s dfm,
+            // on its two voxel rows, already in the A-operand layout (|u| is at most half a
+            // group's x range, so the FP32 terms stay within a few cycles).
             // The products are v_mfma_f64_4x4x4f64 (4 blocks of 4 samples x 4 coefficients):
             // 10-17 cycles each on gfx950 against 70-80 for v_mfma_f64_16x16x4f64, whose
             // 16 columns this product would fill only half. They run over k blocks of 4
-            // record terms covering the coefficients 0..L-1; when L is not a multiple of 4
-            // that also covers t (record entry L at most), which is then taken from the
-            // loaded records, else t_s is loaded with them.
-            constexpr int NM = (L + 3) / 4;
-            constexpr bool TIN = (L % 4) != 0;
-            __shared__ double hl[4][288];       // per wave, 2 slots: c [2 steps][2 runs][2 halves][4][4], t [16]
-            __shared__ double hb[4][2][NM][16]; // per wave: B operands per run and k block (constant per segment)
+            // record terms covering entries 0..L, so t (record entry L at most) is always
+            // among them.
+            constexpr int NM = L / 4 + 1;
+            constexpr int SLOT = 272;             // floats: e [2 steps][4 groups][2 halves][4][4], t [16]
+            __shared__ float hl[4][2 * SLOT];     // per wave, 2 slots
+            __shared__ double hb[4][4][NM][16];   // per wave: B operands per group and k block (constant per segment)
             const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-            double *cl = hl[wv];
+            float *cl = hl[wv];
             const int base = d.x * 256 + wv * 64;
             const int r_lo = hx.vrun[base], r_hi = hx.vrun[base + 63];
             const bool two = r_hi != r_lo;
+            const HoeAdjAnchor &an = hx.aanc[base / 64];
             const int v0 = base + (lane & 31), v1 = v0 + 32;
-            const double2 h0 = hx.hrec[v0], h1 = hx.hrec[v1];
-            const int rs0 = hx.vrun[v0] - r_lo, rs1 = hx.vrun[v1] - r_lo;
+            const float2 o0 = hx.aud[v0], o1 = hx.aud[v1];
+            const int c0 = 2 * (hx.vrun[v0] - r_lo), c1 = 2 * (hx.vrun[v1] - r_lo) + 1;
             const int half = lane >> 5;
             const int kq = lane >> 4;
-            // B entry (k, j) of run g and k block m = beta[run g][4 m + k][j], the same for
+            // B entry (k, j) of group c and k block m: Taylor coefficient j at x_c of record
+            // term 4 m + k's x-polynomial (run r_lo + c / 2), plus fm_c at (t, 0); the same for
             // the 4 blocks of a product
-            for (int i = lane; i < 2 * NM * 16; i += 64) {
-                const int g = i / (NM * 16), m = (i / 16) % NM, e = i & 15;
-                const int term = 4 * m + (e >> 2);
-                const int rr = min(r_lo + g, r_hi);
-                hb[wv][g][m][e] = term < L ? hx.beta[((size_t)rr * L + term) * 4 + (e & 3)] : 0.0;
+            for (int i = lane; i < 4 * NM * 16; i += 64) {
+                const int c = i / (NM * 16), m = (i / 16) % NM, e = i & 15;
+                const int term = 4 * m + (e >> 2), j = e & 3;
+                const int rr = min(r_lo + (c >> 1), r_hi);
+                const double x = (double)an.xa[c];
+                double v = 0.0;
+                if (term < L) {
+                    const double *b = hx.beta + ((size_t)rr * L + term) * 4;
+                    v = j == 0 ? fma(fma(fma(b[3], x, b[2]), x, b[1]), x, b[0])
+                               : (j == 1 ? fma(fma(3.0 * b[3], x, 2.0 * b[2]), x, b[1])
+                                         : (j == 2 ? fma(3.0 * b[3], x, b[2]) : b[3]));
+                }
+                if (term == hx.t_index && j == 0) v += (double)an.fm[c];
+                hb[wv][c][m][e] = v;
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            auto horner = [](const double *c, double x) { return fma(fma(fma(c[3], x, c[2]), x, c[1]), x, c[0]); };
             // A operand of the pair starting at step kp: lane l holds record entry
             // l/16 + 4 m of row l%16; rows 0..7 = samples of kp, rows 8..15 = of kp + 1;
             // zero where the step is outside the segment (the address is then a valid row,
-            // so the load is in bounds whatever the compiler does)
-            auto load_a = [&](int kp, double *a, double &t) {
+            // so the load is in bounds whatever the compiler does; entries up to 4 NM - 1
+            // lie inside the sk >= L + 1 rounded up to 8 doubles of a record)
+            auto load_a = [&](int kp, double *a) {
                 const int r = lane & 15;
                 const bool valid = kp < d.z && (r < 8 || kp + 1 < d.z);
                 const double *ap = coef + ((size_t)(valid ? kp * 8 + r : d.y * 8)) * sk;
 #pragma unroll
                 for (int m = 0; m < NM; ++m) a[m] = valid ? ap[kq + 4 * m] : 0.0;
-                if constexpr (!TIN) t = valid ? ap[hx.t_index] : 0.0;
             };
             // setup products of one pair into an LDS slot. v_mfma_f64_4x4x4f64: A lane l =
             // row l%16, k l/16; B lane l = k l/16, column l%16 (block (l%16)/4, coefficient
             // l%4); D lane l = row R = 4 ((l%16)/4) + l/16, coefficient l%4. Row R is sample
-            // R%4 of half (R/4)%2 of step R/8 of the pair; t of row R goes to slot[128 + R].
+            // R%4 of half (R/4)%2 of step R/8 of the pair; t of row R goes to slot[256 + R].
             const int hbe = (kq << 2) | (lane & 3);
-            const int sidx = ((lane >> 3) & 1) * 64 + ((lane >> 2) & 1) * 16 + 4 * kq + (lane & 3);
-            auto setup = [&](const double *a, double t, double *slot) {
+            const int sidx = ((lane >> 3) & 1) * 128 + ((lane >> 2) & 1) * 16 + 4 * kq + (lane & 3);
+            auto setup = [&](const double *a, float *slot) {
 #pragma unroll
-                for (int g = 0; g < 2; ++g) {
-                    if (g == 1 && !two) break;
+                for (int c = 0; c < 4; ++c) {
+                    if (c == 2 && !two) break;
                     double acc = 0.0;
 #pragma unroll
                     for (int m = 0; m < NM; ++m)
-                        acc = __builtin_amdgcn_mfma_f64_4x4x4f64(a[m], hb[wv][g][m][hbe], acc, 0, 0, 0);
-                    slot[sidx + g * 32] = acc;
+                        acc = __builtin_amdgcn_mfma_f64_4x4x4f64(a[m], hb[wv][c][m][hbe], acc, 0, 0, 0);
+                    slot[sidx + c * 32] = (lane & 3) == 0 ? (float)__builtin_amdgcn_fract(acc) : (float)acc;
                 }
-                if constexpr (TIN) {
-                    const int ti = hx.t_index;
-                    double tv = 0.0;
+                const int ti = hx.t_index;
+                double tv = 0.0;
 #pragma unroll
-                    for (int m = 0; m < NM; ++m) tv = (m == (ti >> 2)) ? a[m] : tv;
-                    if (kq == (ti & 3)) slot[128 + (lane & 15)] = tv;
-                } else if (kq == 0) {
-                    slot[128 + (lane & 15)] = t;
-                }
+                for (int m = 0; m < NM; ++m) tv = (m == (ti >> 2)) ? a[m] : tv;
+                if (kq == (ti & 3)) slot[256 + (lane & 15)] = (float)tv;
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             };
             // software pipeline: the slot of pair p + 1 is filled during the second step of
             // pair p, and the sample records of pair p + 2 are fetched meanwhile
-            double anx[NM], tnx = 0.0;
-            load_a(d.y, anx, tnx);
-            if (d.y < d.z) setup(anx, tnx, cl);
-            load_a(d.y + 2, anx, tnx);
-            kloop([&](int ks, double *ph) {
+            double anx[NM];
+            load_a(d.y, anx);
+            if (d.y < d.z) setup(anx, cl);
+            load_a(d.y + 2, anx);
+            kloop([&](int ks, float *ph) {
                 const int rel = ks - d.y, k2 = rel & 1;
-                const double *cur = cl + ((rel >> 1) & 1) * 144;
+                const float *cur = cl + ((rel >> 1) & 1) * SLOT;
                 if (k2 == 1 && ks + 1 < d.z) {
-                    setup(anx, tnx, cl + (((rel >> 1) & 1) ^ 1) * 144);
-                    load_a(ks + 3, anx, tnx);
+                    setup(anx, cl + (((rel >> 1) & 1) ^ 1) * SLOT);
+                    load_a(ks + 3, anx);
                 }
-                const double *tp = cur + 128 + (k2 * 2 + half) * 4;
-                const double *c0p = cur + ((k2 * 2 + rs0) * 2 + half) * 16;
-                if (two) {
-                    const double *c1p = cur + ((k2 * 2 + rs1) * 2 + half) * 16;
+                const float *tp = cur + 256 + (k2 * 2 + half) * 4;
+                const float4 *e0p = reinterpret_cast<const float4 *>(cur + ((k2 * 4 + c0) * 2 + half) * 16);
+                const float4 *e1p = reinterpret_cast<const float4 *>(cur + ((k2 * 4 + c1) * 2 + half) * 16);
 #pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        const double t = tp[i];
-                        ph[i] = fma(t, h0.y, horner(c0p + 4 * i, h0.x));
-                        ph[4 + i] = fma(t, h1.y, horner(c1p + 4 * i, h1.x));
-                    }
-                } else {
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        const double t = tp[i];
-                        const double *c = c0p + 4 * i;
-                        ph[i] = fma(t, h0.y, horner(c, h0.x));
-                        ph[4 + i] = fma(t, h1.y, horner(c, h1.x));
-                    }
+                for (int i = 0; i < 4; ++i) {
+                    const float t = tp[i];
+                    const float4 e0 = e0p[i], e1 = e1p[i];
+                    ph[i] = fmaf(t, o0.y, fmaf(fmaf(fmaf(e0.w, o0.x, e0.z), o0.x, e0.y), o0.x, e0.x));
+                    ph[4 + i] = fmaf(t, o1.y, fmaf(fmaf(fmaf(e1.w, o1.x, e1.z), o1.x, e1.y), o1.x, e1.x));
                 }
-            }, std::false_type{}, double{});
+            }, std::false_type{}, float{});
             }
         } else {
             generic_path();
@@ -2508,7 +2513,8 @@ static void launch_contract(Handle *h, const Sched &sc, const void *operand, voi
     const int4 *segs = (const int4 *)sc.seg.p;
     const int *coff = (const int *)sc.chunk_off.p;
     HornerArgs hx{(const double2 *)h->hrec.p, (const int *)h->vrun.p, (const double *)h->beta.p, h->t_index,
-                  (const HoeUnit *)h->units.p, (const HoeUnitVox *)h->uvox.p};
+                  (const HoeUnit *)h->units.p, (const HoeUnitVox *)h->uvox.p, (const HoeAdjAnchor *)h->aanc.p,
+                  (const float2 *)h->aud.p};
     if (backend == HOE_BACKEND_DIRECT || backend == HOE_BACKEND_FP64) {
         dim3 grid(sc.n_chunks, direct_batches(h, FWD));
         const int n_stream = (int)(FWD ? h->n_vox : h->n_sam);
@@ -2733,7 +2739,8 @@ static hipError_t run_grad_mfma(Handle *h, bool time_it) {
             (const float2 *)h->ybuf.p, (uint4 *)h->grfrag.p, n_st, nkc, h->ncs, slots, h->csm_max);
     }
     HornerArgs hx{(const double2 *)h->hrec.p, (const int *)h->vrun.p, (const double *)h->beta.p, h->t_index,
-                  (const HoeUnit *)h->units.p, (const HoeUnitVox *)h->uvox.p};
+                  (const HoeUnit *)h->units.p, (const HoeUnitVox *)h->uvox.p, (const HoeAdjAnchor *)h->aanc.p,
+                  (const float2 *)h->aud.p};
     if (time_it) (void)hipEventRecord(h->ev[0], h->stream);
     visit_gmixed(h, [&](auto k) {
         k<<<sc.n_chunks, 256, kGradLds, h->stream>>>(
