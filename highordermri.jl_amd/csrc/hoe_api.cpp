@@ -664,6 +664,20 @@ int create_impl(const Desc *d, hoe_handle **out) {
                 uvox[(size_t)k].ud[v - v0][1] = (float)(hrec[2 * (size_t)v + 1] - (double)a.fma);
             }
         }
+        // Anchored phases add t * (fieldmap - anchor fieldmap) in FP32, whose rounding grows
+        // with its size (about 2e-7 of it); the FP32 offset along x is bounded by the grid
+        // (half a K unit or a wave half of voxels, times the phase change per voxel). Above
+        // kAnchorMaxCycles of fieldmap offset (a rough map over a long readout) the MFMA
+        // kernels keep the FP64 phase of every pair instead.
+        {
+            double tmax = 0.0, fmax_off = 0.0;
+            for (int64_t s = 0; s < ns; ++s) tmax = std::max(tmax, std::fabs((double)d->times[s]));
+            for (const hoe::HoeUnitVox &u : uvox)
+                for (int j = 0; j < 8; ++j) fmax_off = std::max(fmax_off, std::fabs((double)u.ud[j][1]));
+            for (int64_t v = 0; v < h->nv_pad; ++v) fmax_off = std::max(fmax_off, std::fabs((double)aud[2 * (size_t)v + 1]));
+            h->anchor_fm_cycles = tmax * fmax_off;
+            h->anchored = h->anchor_fm_cycles <= hoe::kAnchorMaxCycles;
+        }
     }
 
     hipDeviceProp_t prop;
@@ -790,7 +804,7 @@ int hoe_get_info(const hoe_handle *hh, hoe_info_t *info) {
     info->backend = h->backend;
     info->device = h->device;
     info->closed = h->closed ? 1 : 0;
-    info->horner = (h->horner ? 1 : 0) | (h->horner_adj ? 2 : 0);
+    info->horner = (h->horner ? 1 : 0) | (h->horner_adj ? 2 : 0) | (h->horner && h->anchored ? 4 : 0);
     info->n_runs = h->n_runs;
     return HOE_OK;
 }

@@ -35,6 +35,11 @@ struct alignas(16) HoeUnit {
     float fma;
     float pad;
 };
+// Largest fieldmap offset max|t| * max|fieldmap - anchor fieldmap|, in cycles, for which the
+// MFMA Horner kernels evaluate anchored phases: its FP32 rounding (about 3 * 2^-24 of it,
+// 1.4e-6 cycles at the limit) stays far below the FP16 hi/lo split's 2^-22.
+constexpr double kAnchorMaxCycles = 8.0;
+
 struct alignas(64) HoeUnitVox {
     float ud[8][2];   // (u_j, dfm_j)
 };
@@ -71,6 +76,8 @@ struct Handle {
     std::vector<int> active;          // 0-based term indices (>= 1) with a not identically zero basis
     bool horner = false;              // forward kernels evaluate the phase per run by Horner in x
     bool horner_adj = false;          // MFMA adjoint by Horner (per wave, when its 64 voxels span <= 2 runs)
+    bool anchored = false;            // MFMA Horner kernels evaluate anchored phases (FP32 offsets bounded)
+    double anchor_fm_cycles = 0.0;    // max |t| * max |fieldmap - anchor fieldmap| over the anchor groups
     int32_t n_runs = 0;               // runs: consecutive masked voxels sharing (y, z)
     int32_t t_index = 0;              // index of t in the sample record (= number of active terms + 1)
     int32_t n_term_in = 0;            // 9 or 16

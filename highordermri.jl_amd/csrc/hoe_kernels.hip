@@ -211,6 +211,7 @@ struct HornerArgs {
     const HoeUnitVox *uvox;   // [nv_pad / 8] voxel offsets from the anchors
     const HoeAdjAnchor *aanc; // [nv_pad / 64] group anchors of the MFMA Horner adjoint
     const float2 *aud;        // [nv_pad] voxel offsets from the adjoint group anchors
+    int anchored;             // 0: the MFMA kernels take the FP64 generic phase on every step
 };
 
 template <typename T16>
@@ -602,7 +603,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (
                 const int q0 = ks * 8;
                 const HoeUnit a = an;
                 const int r0 = a.run;
-                if (r0 >= 0) {
+                if (r0 >= 0 && hx.anchored) {
                     const HoeUnitVox &uv = hx.uvox[ks];
                     if (r0 != cur) {
                         run_setup<L>(coef + (size_t)row * sk, hx.beta + (size_t)r0 * L * 4, hc);
@@ -624,7 +625,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (
                         ph[j] = fmaf(tf, dfm, fmaf(fmaf(fmaf(c3f, u, d2), u, d1), u, base));
                     }
                 } else {
-                    // a run boundary inside the K step: generic phase from the full records
+                    // a run boundary inside the K step (or anchored phases off for this
+                    // operator): generic phase from the full records
                     const double *ap = coef + (size_t)row * sk;
                     double a[L + 1];
 #pragma unroll
@@ -641,11 +643,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (
                 an = hx.units[min(ks + 1, n_ks - 1)];
             }, std::true_type{}, float{});
         } else if constexpr (HX && !FWD) {
-            // A wave whose 64 voxels span three or more runs (short rows of a mask)
+            // A wave whose 64 voxels span three or more runs (short rows of a mask), or any
+            // wave when the operator's fieldmap offsets are too large for anchored phases,
             // takes the generic phase; the decision is uniform over the wave.
             const int wv0 = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
             const int base0 = d.x * 256 + wv0 * 64;
-            if (hx.vrun[base0 + 63] - hx.vrun[base0] > 1) {
+            if (hx.vrun[base0 + 63] - hx.vrun[base0] > 1 || !hx.anchored) {
                 generic_path();
             } else {
             // Horner adjoint. The wave's 64 voxels lie in at most two runs (r_lo, r_hi).
@@ -1077,7 +1080,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                     // fieldmap term at each unit's anchor, FP32 offsets of its 8 voxels)
                     const HoeUnit a0 = hx.units[q0 / 8], a1 = hx.units[q0 / 8 + 1];
                     const int r0 = a0.run;
-                    if (r0 >= 0 && a1.run == r0) {
+                    if (r0 >= 0 && a1.run == r0 && hx.anchored) {
                         if (r0 != cur) {
                             run_setup<L>(coef + (size_t)row * sk, hx.beta + (size_t)r0 * L * 4, hc);
                             c1f = (float)hc[1];
@@ -1106,7 +1109,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                             }
                         }
                     } else {
-                        // a run boundary inside the half block: full records (generic phase)
+                        // a run boundary inside the half block (or anchored phases off):
+                        // full records (generic phase)
                         const double *ap = coef + (size_t)row * sk;
                         double a[L + 1];
 #pragma unroll
@@ -2514,7 +2518,7 @@ static void launch_contract(Handle *h, const Sched &sc, const void *operand, voi
     const int *coff = (const int *)sc.chunk_off.p;
     HornerArgs hx{(const double2 *)h->hrec.p, (const int *)h->vrun.p, (const double *)h->beta.p, h->t_index,
                   (const HoeUnit *)h->units.p, (const HoeUnitVox *)h->uvox.p, (const HoeAdjAnchor *)h->aanc.p,
-                  (const float2 *)h->aud.p};
+                  (const float2 *)h->aud.p, h->anchored ? 1 : 0};
     if (backend == HOE_BACKEND_DIRECT || backend == HOE_BACKEND_FP64) {
         dim3 grid(sc.n_chunks, direct_batches(h, FWD));
         const int n_stream = (int)(FWD ? h->n_vox : h->n_sam);
@@ -2740,7 +2744,7 @@ static hipError_t run_grad_mfma(Handle *h, bool time_it) {
     }
     HornerArgs hx{(const double2 *)h->hrec.p, (const int *)h->vrun.p, (const double *)h->beta.p, h->t_index,
                   (const HoeUnit *)h->units.p, (const HoeUnitVox *)h->uvox.p, (const HoeAdjAnchor *)h->aanc.p,
-                  (const float2 *)h->aud.p};
+                  (const float2 *)h->aud.p, h->anchored ? 1 : 0};
     if (time_it) (void)hipEventRecord(h->ev[0], h->stream);
     visit_gmixed(h, [&](auto k) {
         k<<<sc.n_chunks, 256, kGradLds, h->stream>>>(

@@ -248,7 +248,8 @@ class HighOrderExplicitOp:
         info = _lib.Info()
         check(lib.hoe_get_info(h, C.byref(info)), h)
         self.nVox = int(info.n_vox)
-        self.horner = int(info.horner)         # bit 0: Horner forward, bit 1: Horner MFMA adjoint
+        self.horner = int(info.horner)         # bit 0: Horner forward, bit 1: Horner MFMA adjoint,
+                                               # bit 2: anchored phases in the MFMA Horner kernels
         self.nrow, self.ncol = n_sam * n_cha, n_grid
         self.device = gpus[0]
         self.backend = backend

@@ -136,7 +136,8 @@ typedef struct {
     int32_t backend;
     int32_t device;
     int32_t closed;
-    int32_t horner;   /* phase evaluation: bit 0 = Horner runs in the forward kernels, bit 1 = in the MFMA adjoint */
+    int32_t horner;   /* phase evaluation: bit 0 = Horner runs in the forward kernels, bit 1 = in the MFMA adjoint,
+                         bit 2 = the MFMA kernels evaluate anchored phases (FP64 anchor, FP32 offsets) on them */
     int32_t n_runs;   /* runs of consecutive masked voxels sharing (y, z) */
 } hoe_info_t;
 

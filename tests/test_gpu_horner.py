@@ -113,3 +113,39 @@ def test_horner_record_buckets_vs_fp64(case, backend, gpu_lib):
         e_n = O.relerr(hoe.normalOperator(op) * x, n_ref)
     print(f"{case} {backend} (horner {op.horner}): E {e_f:.2e}  E^H {e_a:.2e}  E^H E {e_n:.2e}")
     assert e_f < 3e-5 and e_a < 3e-5 and e_n < 3e-5
+
+
+@pytest.mark.parametrize("fm_kind,expect_anchored", [("smooth", True), ("rough_small", True), ("rough_large", False)])
+@pytest.mark.parametrize("backend", ["mixed_fp16", "mixed_bf16"])
+def test_anchored_phase_bound(fm_kind, expect_anchored, backend, gpu_lib):
+    """The MFMA Horner kernels add t * (fieldmap - anchor fieldmap) in FP32 (anchored phases),
+    whose rounding grows with its size. hoe_create enables them only while
+    max|t| * max|fieldmap offset| <= 8 cycles, and otherwise keeps the FP64 phase of every
+    pair. On a 192x8 grid with a 20 ms readout:
+    - a smooth map (3 Gaussians, 120 Hz): anchored;
+    - an uncorrelated map of +-90 Hz (offsets up to 180 Hz: 3.6 cycles): anchored, at the
+      same accuracy;
+    - an uncorrelated map of +-3 kHz (120 cycles): FP64 phases.
+    Every product against the FP64 operator at 3e-5 (1e-4 for BF16)."""
+    grid, kspha, times, fm, csm, x, _ = _problem(192, 8, 1)
+    times = (np.arange(times.size) * 13e-6).astype(np.float32)   # 20 ms
+    rng = np.random.default_rng(29)
+    if fm_kind == "rough_small":
+        fm = rng.uniform(-90.0, 90.0, size=fm.shape).astype(np.float32)
+    elif fm_kind == "rough_large":
+        fm = rng.uniform(-3000.0, 3000.0, size=fm.shape).astype(np.float32)
+    m = O.make_model(grid, kspha, times, fm, csm, None)
+    y_ref = O.forward_f64(m, x)
+    probe = (np.random.default_rng(1).normal(size=y_ref.size)
+             + 1j * np.random.default_rng(2).normal(size=y_ref.size)).astype(np.complex64)
+    xa_ref = O.adjoint_f64(m, probe)
+    n_ref = O.normal_f64(m, x)
+    tol = TOLS[backend]
+    with hoe.HighOrderExplicitOp(grid, kspha, times, fieldmap=fm, csm=csm, backend=backend) as op:
+        assert op.horner & 3 == 3, "runs long enough for the Horner kernels"
+        assert bool(op.horner & 4) == expect_anchored
+        e_f = O.relerr(op * x, y_ref)
+        e_a = O.relerr(op.H * probe, xa_ref)
+        e_n = O.relerr(hoe.normalOperator(op) * x, n_ref)
+    print(f"{fm_kind} {backend} (horner {op.horner}): E {e_f:.2e}  E^H {e_a:.2e}  E^H E {e_n:.2e}")
+    assert e_f < tol and e_a < tol and e_n < tol
