@@ -126,7 +126,8 @@ def test_anchored_phase_bound(fm_kind, expect_anchored, backend, gpu_lib):
     - an uncorrelated map of +-90 Hz (offsets up to 180 Hz: 3.6 cycles): anchored, at the
       same accuracy;
     - an uncorrelated map of +-3 kHz (120 cycles): FP64 phases.
-    Every product against the FP64 operator at 3e-5 (1e-4 for BF16)."""
+    Every product against the FP64 operator at 3e-5 (1e-4 for BF16), and the MFMA coefficient
+    gradient (its Horner blocks anchored alike) against the FP64 gradient at 1e-4."""
     grid, kspha, times, fm, csm, x, _ = _problem(192, 8, 1)
     times = (np.arange(times.size) * 13e-6).astype(np.float32)   # 20 ms
     rng = np.random.default_rng(29)
@@ -141,11 +142,17 @@ def test_anchored_phase_bound(fm_kind, expect_anchored, backend, gpu_lib):
     xa_ref = O.adjoint_f64(m, probe)
     n_ref = O.normal_f64(m, x)
     tol = TOLS[backend]
+    # gradient data: the forward of a perturbed image, so that the residual is not small
+    y_g = O.forward_f64(m, 0.8 * x + 0.05).astype(np.complex64)
+    g64, _, _ = O.grad_f64(m, x, y_g.astype(np.complex128))
     with hoe.HighOrderExplicitOp(grid, kspha, times, fieldmap=fm, csm=csm, backend=backend) as op:
         assert op.horner & 3 == 3, "runs long enough for the Horner kernels"
         assert bool(op.horner & 4) == expect_anchored
         e_f = O.relerr(op * x, y_ref)
         e_a = O.relerr(op.H * probe, xa_ref)
         e_n = O.relerr(hoe.normalOperator(op) * x, n_ref)
-    print(f"{fm_kind} {backend} (horner {op.horner}): E {e_f:.2e}  E^H {e_a:.2e}  E^H E {e_n:.2e}")
+        g, _, _ = op.grad(x, y_g)
+        e_g = np.linalg.norm(g - g64) / np.linalg.norm(g64)
+    print(f"{fm_kind} {backend} (horner {op.horner}): E {e_f:.2e}  E^H {e_a:.2e}  E^H E {e_n:.2e}  grad {e_g:.2e}")
     assert e_f < tol and e_a < tol and e_n < tol
+    assert e_g < 1e-4
