@@ -862,8 +862,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (
 #pragma unroll
                     for (int r = 0; r < 16; ++r) {
                         const int rr = wave * 64 + rt * 32 + 8 * (r >> 2) + 4 * hi32 + (r & 3);
-                        part[((size_t)sgi * 256 + rr) * ncs + coil] =
-                            make_float2(acc[rt][b][0][r], acc[rt][b][NA - 1][r]);
+                        // streaming stores: the partial sums are read once, by the reduction,
+                        // and need not displace the K part's fragments from L2
+                        float2 *dst = &part[((size_t)sgi * 256 + rr) * ncs + coil];
+                        __builtin_nontemporal_store(acc[rt][b][0][r], &dst->x);
+                        __builtin_nontemporal_store(acc[rt][b][NA - 1][r], &dst->y);
                     }
                 }
         } else {
@@ -882,7 +885,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (
                     if constexpr (PK == 2)
                         v += __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x401f));
                     const int rr = wave * 64 + rt * 32 + 8 * (r >> 2) + 4 * hi32 + (r & 3);
-                    if (PK == 1 || col < 16) pf[(((size_t)sgi * 256 + rr) * ncs + coil) * 2 + cmp] = v;
+                    if (PK == 1 || col < 16)
+                        __builtin_nontemporal_store(v, &pf[(((size_t)sgi * 256 + rr) * ncs + coil) * 2 + cmp]);
                 }
         }
     }
@@ -1176,7 +1180,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
                 const int rr = wave * 64 + t * 32 + 8 * (r >> 2) + 4 * hi32 + (r & 3);
-                part[((size_t)sgi * 256 + rr) * 32 + col] = g[t][r];
+                __builtin_nontemporal_store(g[t][r], &part[((size_t)sgi * 256 + rr) * 32 + col]);
             }
     }
 }
