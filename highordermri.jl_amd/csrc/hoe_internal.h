@@ -61,6 +61,7 @@ struct Sched {
     int cpx = 0;        // chunks per XCD part
     int n_chunks = 0;   // workgroups launched (8 * cpx)
     int n_seg = 0;      // segments = partial-sum blocks
+    int max_tile_segs = 0;   // most segments of one tile
     DeviceBuf seg;        // int4 [n_seg]: tile, k0, k1, 0
     DeviceBuf chunk_off;  // int [n_chunks + 1]: first segment of each chunk
     DeviceBuf tile_off;   // int [tiles + 1]: CSR offsets into tile_list

@@ -1666,42 +1666,51 @@ __global__ __launch_bounds__(256) void k_load_ybuf(const cplx_t<R> *__restrict__
 // Adjoint reduction: xm_out[v] = sc * sum_{seg} sum_{g < ngrp} part[seg][g][v % 256]
 //                               (+ lambda * x_in[v]).
 // The contraction kernels combine the coils of their group (conj(csm) weighting) in their
-// epilogues, so one complex per voxel, segment and coil group remains. One thread per
-// voxel; the segments of its tile are added in list order, the groups in index order.
+// epilogues, so one complex per voxel, segment and coil group remains. NL consecutive lanes
+// share a voxel (NL = 1..16, a power of two chosen from the tile's segment count, so that
+// small problems, whose tiles are cut into many segments, keep few loads per lane): lane j
+// adds the entries j, j + NL, ... of the voxel's (segment, group) list, two in flight, and a
+// fixed butterfly adds the lanes.
 template <typename R>
 __global__ __launch_bounds__(256) void k_reduce_adj(
     const cplx_t<R> *__restrict__ part, const int *__restrict__ tile_off, const int *__restrict__ tile_list,
-    int ngrp, int nv_pad, int nv, R scale, const unsigned *__restrict__ scale_slot, cplx_t<R> *__restrict__ xm_out,
+    int ngrp, int nl, int nv, R scale, const unsigned *__restrict__ scale_slot, cplx_t<R> *__restrict__ xm_out,
     R lambda, const cplx_t<R> *__restrict__ x_in, int v_base, int v_end, int nv_write) {
-    const int v = v_base + blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= v_end) return;
+    const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int v = v_base + (int)(gid / nl), j = (int)(gid % nl);
+    const bool live = v < v_end;
     R sc = scale;
     if constexpr (std::is_same<R, float>::value) {
         if (scale_slot) sc = scale / pow2_scale(__uint_as_float(*scale_slot));
     }
-    const int tile = v >> 8, r = v & 255;
-    const int *list = tile_list + tile_off[tile];
-    const int n = (tile_off[tile + 1] - tile_off[tile]) * ngrp;
-    // entry i = (segment i / ngrp, group i % ngrp); four sums in flight
-    auto at = [&](int i) { return part[((size_t)list[i / ngrp] * ngrp + i % ngrp) * 256 + r]; };
-    cplx_t<R> a[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) a[q] = mkc<R>(R(0), R(0));
-    int i = 0;
-    for (; i + 3 < n; i += 4) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const cplx_t<R> t = at(i + q);
-            a[q].x += t.x;
-            a[q].y += t.y;
+    cplx_t<R> a = mkc<R>(R(0), R(0)), b = mkc<R>(R(0), R(0));
+    if (live) {
+        const int tile = v >> 8, r = v & 255;
+        const int *list = tile_list + tile_off[tile];
+        const int n = (tile_off[tile + 1] - tile_off[tile]) * ngrp;
+        // entry i = (segment i / ngrp, group i % ngrp)
+        auto at = [&](int i) { return part[((size_t)list[i / ngrp] * ngrp + i % ngrp) * 256 + r]; };
+        int i = j;
+        for (; i + nl < n; i += 2 * nl) {
+            const cplx_t<R> t0 = at(i), t1 = at(i + nl);
+            a.x += t0.x;
+            a.y += t0.y;
+            b.x += t1.x;
+            b.y += t1.y;
+        }
+        if (i < n) {
+            const cplx_t<R> t = at(i);
+            a.x += t.x;
+            a.y += t.y;
         }
     }
-    for (; i < n; ++i) {
-        const cplx_t<R> t = at(i);
-        a[0].x += t.x;
-        a[0].y += t.y;
+    R sx = a.x + b.x, sy = a.y + b.y;
+    for (int o = 1; o < nl; o <<= 1) {
+        sx += __shfl_xor(sx, o);
+        sy += __shfl_xor(sy, o);
     }
-    cplx_t<R> res = mkc<R>(((a[0].x + a[1].x) + (a[2].x + a[3].x)) * sc, ((a[0].y + a[1].y) + (a[2].y + a[3].y)) * sc);
+    if (!live || j != 0) return;
+    cplx_t<R> res = mkc<R>(sx * sc, sy * sc);
     if (v >= nv) res = mkc<R>(R(0), R(0));
     if (x_in && v < nv) {
         const cplx_t<R> t = x_in[v];
@@ -1709,7 +1718,6 @@ __global__ __launch_bounds__(256) void k_reduce_adj(
         res.y += lambda * t.y;
     }
     if (v < nv_write) xm_out[v] = res;
-    (void)nv_pad;
 }
 
 // gk[l][s] = scale * sum_segments sum_batches part[seg][batch][s%256][l]  (FP64)
@@ -2276,6 +2284,9 @@ static hipError_t upload_plan(Sched &sc, const SchedPlan &pl, int tiles, int n_k
     sc.cpx = pl.cpx;
     sc.n_chunks = pl.n_chunks;
     sc.n_seg = pl.n_seg;
+    sc.max_tile_segs = 0;
+    for (int t = 0; t + 1 < (int)pl.tile_off.size(); ++t)
+        sc.max_tile_segs = std::max(sc.max_tile_segs, pl.tile_off[(size_t)t + 1] - pl.tile_off[(size_t)t]);
     hipError_t e;
     if ((e = upload_ints(sc.seg, pl.seg)) != hipSuccess) return e;
     if ((e = upload_ints(sc.chunk_off, pl.chunk_off)) != hipSuccess) return e;
@@ -2695,13 +2706,17 @@ static hipError_t adjoint_passes(Handle *h, bool time_it, double scale, void *xm
 // are not written (a caller's grid-sized output has no padding).
 static hipError_t adjoint_reduce(Handle *h, const Sched &sc, double scale, void *xm_out, double lambda,
                                  const void *x_in, int nv_write) {
-    const int64_t threads = (int64_t)(sc.t_hi - sc.t_lo) * 256;
+    // lanes per voxel: about 4 entries per lane, at most 16 lanes
+    const int ngrp = adj_groups(h, h->backend);
+    int nl = 1;
+    while (nl < 16 && nl * 4 < sc.max_tile_segs * ngrp) nl <<= 1;
+    const int64_t threads = (int64_t)(sc.t_hi - sc.t_lo) * 256 * nl;
     by_prec(h, [&](auto r) {
         using R = decltype(r);
         k_reduce_adj<R><<<blocks_for(threads), 256, 0, h->stream>>>(
-            (const cplx_t<R> *)h->part_a.p, (const int *)sc.tile_off.p, (const int *)sc.tile_list.p,
-            adj_groups(h, h->backend), (int)h->nv_pad, (int)h->n_vox, (R)scale, mixed(h) ? slot(h, 1) : nullptr,
-            (cplx_t<R> *)xm_out, (R)lambda, (const cplx_t<R> *)x_in, sc.t_lo * 256, sc.t_hi * 256, nv_write);
+            (const cplx_t<R> *)h->part_a.p, (const int *)sc.tile_off.p, (const int *)sc.tile_list.p, ngrp, nl,
+            (int)h->n_vox, (R)scale, mixed(h) ? slot(h, 1) : nullptr, (cplx_t<R> *)xm_out, (R)lambda,
+            (const cplx_t<R> *)x_in, sc.t_lo * 256, sc.t_hi * 256, nv_write);
     });
     return hipGetLastError();
 }
