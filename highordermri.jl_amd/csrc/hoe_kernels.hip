@@ -1531,14 +1531,18 @@ __device__ __forceinline__ int reduce_cols(int nc) {
 }
 
 template <typename R>
-__global__ __launch_bounds__(256) void k_reduce_fwd(
+__global__ __launch_bounds__(1024) void k_reduce_fwd(
     const cplx_t<R> *__restrict__ part, const int *__restrict__ tile_off, const int *__restrict__ tile_list, int ncs,
     int ns, int nc, R scale, const unsigned *__restrict__ scale_slot, float scale_mul, int mode,
     cplx_t<R> *__restrict__ out, const R *__restrict__ w, const cplx_t<R> *__restrict__ y_data, unsigned *maxslot,
     int s_base) {
     __shared__ cplx_t<R> tileb[32][33];
+    __shared__ cplx_t<R> other[3][4][256];   // row sums of slices 1..H-1
+    const int H = blockDim.x >> 8;              // segment slices of 256 threads (1, 2 or 4)
     const int s0 = s_base + blockIdx.x * 32, c0 = blockIdx.y * 32;
-    const int tc = threadIdx.x & 31, tr = threadIdx.x >> 5;   // 8 row groups
+    const int hf = threadIdx.x >> 8;
+    const int tl = threadIdx.x & 255;
+    const int tc = tl & 31, tr = tl >> 5;   // 8 row groups
     const int tile = s0 >> 8;
     const int *list = tile_list + tile_off[tile];
     const int nseg = tile_off[tile + 1] - tile_off[tile];
@@ -1546,28 +1550,28 @@ __global__ __launch_bounds__(256) void k_reduce_fwd(
     if constexpr (std::is_same<R, float>::value) {
         if (scale_slot) sc = scale / pow2_scale(__uint_as_float(*scale_slot) * scale_mul);
     }
-    if (mode == 1 && y_data) {
+    if (mode == 1 && y_data && hf == 0) {
         // stage the data tile transposed: y_data[c*ns + s]
         for (int k = tr; k < 32; k += 8) {
             const int c = c0 + k, s = s0 + tc;
             tileb[k][tc] = (c < nc && s < ns) ? y_data[(size_t)c * ns + s] : mkc<R>(R(0), R(0));
         }
-        __syncthreads();
     }
-    // four rows per thread and two segments per step, in separate sums, so that 8
-    // loads are in flight; lanes of coils beyond nc read nothing (the contraction
-    // kernels need not write those columns). With nc <= 16 the lanes of one coil split
-    // its segments (see reduce_cols); lane tc < ncl then holds coil tc's total and the
-    // others, whose columns lie beyond nc, are discarded below.
+    // four rows per thread; slice hf of the block takes every H-th group of the thread's
+    // segments, two per step in separate sums (8 loads in flight); lanes of coils beyond
+    // nc read nothing (the contraction kernels need not write those columns). With nc <= 16
+    // the lanes of one coil also split its segments (see reduce_cols); lane tc < ncl then
+    // holds coil tc's total and the others, whose columns lie beyond nc, are discarded below.
     const int ncl = c0 == 0 ? reduce_cols(nc) : 32, nsub = 32 / ncl;
+    const int S = H * nsub;   // segment stride of one thread
     const int cc = c0 + (tc & (ncl - 1));
     cplx_t<R> rows[4], rows2[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) rows[q] = rows2[q] = mkc<R>(R(0), R(0));
     const int nsr = cc < nc ? nseg : 0;
-    int j = tc / ncl;
-    for (; j + nsub < nsr; j += 2 * nsub) {
-        const size_t b0 = (size_t)list[j] * 256, b1 = (size_t)list[j + nsub] * 256;
+    int j = tc / ncl + hf * nsub;
+    for (; j + S < nsr; j += 2 * S) {
+        const size_t b0 = (size_t)list[j] * 256, b1 = (size_t)list[j + S] * 256;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int r = (s0 + tr + 8 * q) & 255;
@@ -1592,45 +1596,58 @@ __global__ __launch_bounds__(256) void k_reduce_fwd(
     for (int q = 0; q < 4; ++q) {
         rows[q].x += rows2[q].x;
         rows[q].y += rows2[q].y;
-        for (int o = ncl; o < 32; o <<= 1) {
-            rows[q].x += __shfl_xor(rows[q].x, o, 32);
-            rows[q].y += __shfl_xor(rows[q].y, o, 32);
-        }
+        if (hf > 0) other[hf - 1][q][tl] = rows[q];
     }
+    __syncthreads();   // the other slices' sums and the staged data tile
     float m = 0.0f;
+    if (hf == 0) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int k = tr + 8 * q;
-        const int s = s0 + k, c = c0 + tc;
-        cplx_t<R> acc = rows[q];
-        acc.x *= sc;
-        acc.y *= sc;
-        if (mode == 0) {
-            tileb[k][tc] = acc;
-        } else {
-            const bool valid = (s < ns) && (c < nc);
-            if (y_data) {
-                const cplx_t<R> d = tileb[tc][k];
-                acc.x -= d.x;
-                acc.y -= d.y;
+        for (int q = 0; q < 4; ++q) {
+            for (int h = 1; h < H; ++h) {
+                const cplx_t<R> o = other[h - 1][q][tl];
+                rows[q].x += o.x;
+                rows[q].y += o.y;
             }
-            if (w) {
-                const R ww = w[(size_t)s * ncs + c];
-                acc.x *= ww;
-                acc.y *= ww;
+            for (int k = ncl; k < 32; k <<= 1) {
+                rows[q].x += __shfl_xor(rows[q].x, k, 32);
+                rows[q].y += __shfl_xor(rows[q].y, k, 32);
             }
-            if (!valid) acc = mkc<R>(R(0), R(0));
-            out[(size_t)s * ncs + c] = acc;
-            m = fmaxf(m, fmaxf(fabsf((float)acc.x), fabsf((float)acc.y)));
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int k = tr + 8 * q;
+            const int s = s0 + k, c = c0 + tc;
+            cplx_t<R> acc = rows[q];
+            acc.x *= sc;
+            acc.y *= sc;
+            if (mode == 0) {
+                tileb[k][tc] = acc;
+            } else {
+                const bool valid = (s < ns) && (c < nc);
+                if (y_data) {
+                    const cplx_t<R> d = tileb[tc][k];
+                    acc.x -= d.x;
+                    acc.y -= d.y;
+                }
+                if (w) {
+                    const R ww = w[(size_t)s * ncs + c];
+                    acc.x *= ww;
+                    acc.y *= ww;
+                }
+                if (!valid) acc = mkc<R>(R(0), R(0));
+                out[(size_t)s * ncs + c] = acc;
+                m = fmaxf(m, fmaxf(fabsf((float)acc.x), fabsf((float)acc.y)));
+            }
         }
     }
     if (mode == 0) {
         __syncthreads();
-        for (int k = tr; k < 32; k += 8) {
-            const int c = c0 + k, s = s0 + tc;
-            if (c < nc && s < ns) out[(size_t)c * ns + s] = tileb[tc][k];
-        }
-    } else if (maxslot) {
+        if (hf == 0)
+            for (int k = tr; k < 32; k += 8) {
+                const int c = c0 + k, s = s0 + tc;
+                if (c < nc && s < ns) out[(size_t)c * ns + s] = tileb[tc][k];
+            }
+    } else if (maxslot && hf == 0) {
         wave_absmax_to(maxslot, m);
     }
 }
@@ -2615,14 +2632,25 @@ hipError_t run_normal_full(Handle *h, const void *x_full, void *xm_out, double s
 
 static inline bool mixed(Handle *h) { return !direct_path(h->backend); }
 
+// Segment slices of the forward reduction's blocks (1, 2 or 4 x 256 threads): about 3
+// segments per thread, counting the lanes that already split a coil's segments when the
+// operator has 16 coils or fewer.
+static int fwd_reduce_slices(const Handle *h, const Sched &sc) {
+    const int nsub = 32 / (h->n_cha > 16 ? 32 : (h->n_cha <= 1 ? 1 : 1 << (32 - __builtin_clz((unsigned)h->n_cha - 1))));
+    int H = 1;
+    while (H < 4 && H * nsub * 3 < sc.max_tile_segs) H <<= 1;
+    return H;
+}
+
 // Reduction of one forward pass (its 32-row blocks of samples) into the user signal
 // layout (complex [c][s]) with scale.
 static hipError_t launch_reduce_fwd_user(Handle *h, const Sched &sc, void *y_out, double scale) {
     const bool unscale = !direct_path(fwd_backend(h));
     dim3 grid((unsigned)((sc.t_hi - sc.t_lo) * 8), (unsigned)(h->ncs / 32));
+    const int thr = 256 * fwd_reduce_slices(h, sc);
     by_prec(h, [&](auto r) {
         using R = decltype(r);
-        k_reduce_fwd<R><<<grid, 256, 0, h->stream>>>(
+        k_reduce_fwd<R><<<grid, thr, 0, h->stream>>>(
             (const cplx_t<R> *)h->part_f.p, (const int *)sc.tile_off.p, (const int *)sc.tile_list.p, h->ncs,
             (int)h->n_sam, (int)h->n_cha, (R)scale, unscale ? slot(h, 0) : nullptr, h->csm_max, 0,
             (cplx_t<R> *)y_out, nullptr, nullptr, nullptr, sc.t_lo * 256);
@@ -2635,9 +2663,10 @@ static hipError_t launch_reduce_fwd_user(Handle *h, const Sched &sc, void *y_out
 static hipError_t launch_reduce_fwd_ybuf(Handle *h, const Sched &sc, double scale, bool weights,
                                          const void *y_data) {
     dim3 grid((unsigned)((sc.t_hi - sc.t_lo) * 8), (unsigned)(h->ncs / 32));
+    const int thr = 256 * fwd_reduce_slices(h, sc);
     by_prec(h, [&](auto r) {
         using R = decltype(r);
-        k_reduce_fwd<R><<<grid, 256, 0, h->stream>>>(
+        k_reduce_fwd<R><<<grid, thr, 0, h->stream>>>(
             (const cplx_t<R> *)h->part_f.p, (const int *)sc.tile_off.p, (const int *)sc.tile_list.p, h->ncs,
             (int)h->n_sam, (int)h->n_cha, (R)scale, mixed(h) ? slot(h, 0) : nullptr, h->csm_max, 1,
             (cplx_t<R> *)h->ybuf.p, weights ? (const R *)h->dvec.p : nullptr, (const cplx_t<R> *)y_data,
