@@ -87,6 +87,47 @@ def akima_tangents(x, y):
     return np.where(den > 0, t, mean)
 
 
+def _eval_columns(x, y, t, xq, mode):
+    """Cubic Hermite (``t``: node slopes) or linear interpolation of the float64 columns
+    ``y`` at ``xq``; zero outside ``[x[0], x[-1]]``. The basis weights are formed once per
+    query point and applied to all columns."""
+    n = x.size
+    inside = (xq >= x[0]) & (xq <= x[-1])
+    i = np.clip(np.searchsorted(x, xq, side="right") - 1, 0, n - 2)
+    h = x[i + 1] - x[i]
+    s = (xq - x[i]) / h
+    if mode == "linear":
+        w0, w1 = 1.0 - s, s
+        w0[~inside] = 0.0
+        w1[~inside] = 0.0
+        out = np.take(y, i, axis=0)
+        out *= w0[:, None]
+        tmp = np.take(y, i + 1, axis=0)
+        tmp *= w1[:, None]
+        out += tmp
+        return out
+    s2 = s * s
+    s3 = s2 * s
+    h00 = 2 * s3 - 3 * s2 + 1
+    h10 = (s3 - 2 * s2 + s) * h
+    h01 = -2 * s3 + 3 * s2
+    h11 = (s3 - s2) * h
+    for w in (h00, h10, h01, h11):
+        w[~inside] = 0.0
+    out = np.take(y, i, axis=0)
+    out *= h00[:, None]
+    tmp = np.take(t, i, axis=0)
+    tmp *= h10[:, None]
+    out += tmp
+    np.take(y, i + 1, axis=0, out=tmp)
+    tmp *= h01[:, None]
+    out += tmp
+    np.take(t, i + 1, axis=0, out=tmp)
+    tmp *= h11[:, None]
+    out += tmp
+    return out
+
+
 def interp_columns(x, y, xq, intermode="akima"):
     """Interpolate the columns of ``y`` ((n, m)) given at increasing ``x`` onto ``xq``;
     zero outside ``[x[0], x[-1]]`` (``extrapolate(itp, 0)``, InterpTraj.jl:41)."""
@@ -100,21 +141,29 @@ def interp_columns(x, y, xq, intermode="akima"):
         raise DimensionMismatch("interpolation nodes and values")
     if not np.all(np.diff(x) > 0):
         raise ArgumentError("interpolation nodes must be strictly increasing")
-    n = x.size
-    inside = (xq >= x[0]) & (xq <= x[-1])
-    i = np.clip(np.searchsorted(x, xq, side="right") - 1, 0, n - 2)
-    h = x[i + 1] - x[i]
-    s = ((xq - x[i]) / h)[:, None]
-    y0, y1 = y[i], y[i + 1]
-    if mode == "linear":
-        out = y0 + s * (y1 - y0)
-    else:
-        t = akima_tangents(x, y)
-        s2, s3 = s * s, s * s * s
-        out = ((2 * s3 - 3 * s2 + 1) * y0 + (s3 - 2 * s2 + s) * h[:, None] * t[i]
-               + (-2 * s3 + 3 * s2) * y1 + (s3 - s2) * h[:, None] * t[i + 1])
-    out[~inside] = 0.0
-    return out
+    t = akima_tangents(x, y) if mode == "akima" else None
+    return _eval_columns(x, y, t, xq, mode)
+
+
+class _Resampler:
+    """Field samples ``traj`` ((nFieldSample, nTerm)) at ``j*dt``, resampled at shifted
+    times: sample j sits at ``j*dt - delTime`` (InterpTraj.jl:18-44). A shift leaves
+    Akima's node slopes unchanged, so they are formed once on the unshifted nodes and
+    reused for every ``delTime`` (the delay loops resample the same samples once per update)."""
+
+    def __init__(self, traj, dt, intermode=None):
+        self.dtype = np.asarray(traj).dtype
+        self.y = np.asarray(traj, dtype=np.float64)
+        self.mode = _mode(intermode)
+        n = self.y.shape[0]
+        if n < 2:
+            raise ArgumentError("interpolation needs at least 2 field samples")
+        self.x0 = float(dt) * np.arange(n)
+        self.t = akima_tangents(self.x0, self.y) if self.mode == "akima" else None
+
+    def __call__(self, delTime, datatime):
+        xq = np.asarray(datatime, dtype=np.float64).reshape(-1)
+        return _eval_columns(self.x0 - float(delTime), self.y, self.t, xq, self.mode).astype(self.dtype)
 
 
 def InterpTrajTime(traj, dt, delTime, datatime=None, *, intermode=None):
@@ -146,8 +195,7 @@ def InterpTrajTime(traj, dt, delTime, datatime=None, *, intermode=None):
     n = traj.shape[0]
     if datatime is None:
         datatime = float(dt) * np.arange(n)
-    t_field = float(dt) * np.arange(n) - float(delTime)
-    return interp_columns(t_field, traj, datatime, intermode).astype(traj.dtype)
+    return _Resampler(traj, dt, intermode)(delTime, datatime)
 
 
 def kspha_derivative(kspha, dt):
@@ -279,13 +327,15 @@ def FindDelay(gridding, data, kspha, *args, JumpFact=3, Δτ_min=0.005, intermod
     rec = _rec_params(gridding, reg, iter_max, solver)
     yv = data.reshape(-1, order="F")
     loop = _Loop(JumpFact, Δτ_min, dt, max_updates)
+    if kspha.ndim != 2:
+        raise DimensionMismatch("traj must be (nFieldSample, nTerm) or (nShot, nFieldSample, nTerm)")
+    # InterpTrajTime of the same samples at every update: node slopes formed once
+    res_k, res_dt = _Resampler(kspha, dt, intermode), _Resampler(kspha_dt, dt, intermode)
     op = None
     try:
         while loop.running():
-            k_tau = InterpTrajTime(kspha, dt, loop.τ + float(StartTime), datatime,
-                                   intermode=intermode)[:nSample].T
-            kdt_tau = InterpTrajTime(kspha_dt, dt, loop.τ + float(StartTime), datatime,
-                                     intermode=intermode)[:nSample].T
+            k_tau = res_k(loop.τ + float(StartTime), datatime)[:nSample].T
+            kdt_tau = res_dt(loop.τ + float(StartTime), datatime)[:nSample].T
             w = _weights(weight, k_tau, nSample)
             if op is None:
                 op = HighOrderExplicitOp(gridding, k_tau, datatime, fieldmap=fieldmap, csm=csm,
@@ -358,11 +408,16 @@ def FindDelay_multishot(gridding, data, kspha, *args, JumpFact=3, Δτ_min=0.005
     same_times = np.array_equal(t_recon, t_first)
     yv = data.reshape(-1, order="F")
     loop = _Loop(JumpFact, Δτ_min, dt, max_updates)
+    if StartTime.size != n_shot:
+        raise AssertionError("Number of shots in trajectory must match length of delTime vector")
+    # InterpTrajTime of each shot's samples at every update: node slopes formed once per shot
+    res_k = [_Resampler(kspha[i], dt, intermode) for i in range(n_shot)]
+    res_dt = [_Resampler(kspha_dt[i], dt, intermode) for i in range(n_shot)]
     op = op_res = None
     try:
         while loop.running():
-            k_tau = InterpTrajTime(kspha, dt, loop.τ + StartTime, datatime, intermode=intermode)
-            kdt_tau = InterpTrajTime(kspha_dt, dt, loop.τ + StartTime, datatime, intermode=intermode)
+            k_tau = np.stack([res_k[i](loop.τ + StartTime[i], datatime[i]) for i in range(n_shot)])
+            kdt_tau = np.stack([res_dt[i](loop.τ + StartTime[i], datatime[i]) for i in range(n_shot)])
             k_r = k_tau.reshape(n_sam, n_term).T          # (nTerm, nADC*nShot), ADC fastest
             kdt_r = kdt_tau.reshape(n_sam, n_term).T
             w = _weights(weight, k_r, n_sam)
