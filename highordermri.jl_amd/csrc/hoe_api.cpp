@@ -231,6 +231,11 @@ int reschedule_direct(Handle *h, bool prev_dt) {
     return set_err(h, e == hipErrorOutOfMemory ? HOE_ENOMEM : HOE_EHIP, msg);
 }
 
+// The FP32 forward's schedules exist (the direct backend is or was selected): they follow dt_on.
+bool need_direct_resched(const Handle *h) {
+    return h->backend == HOE_BACKEND_DIRECT || !h->sched_f[HOE_BACKEND_DIRECT].empty();
+}
+
 // Without a mask the masked voxel order is the grid order (mask_idx is the identity).
 bool unmasked(const Handle *h) { return h->n_vox == h->n_grid; }
 
@@ -497,6 +502,13 @@ int create_impl(const Desc *d, hoe_handle **out) {
     h->active.clear();
     for (int l = 1; l < L; ++l)
         if (nonzero[l]) h->active.push_back(l);
+    // max |b| of each record entry (constant, active terms): the bound on psi that scales the
+    // MFMA derivative forward's A operand
+    h->rec_bmax.assign(h->active.size() + 1, 0.0);
+    h->rec_bmax[0] = 1.0;
+    for (int64_t v = 0; v < nv; ++v)
+        for (size_t i = 0; i < h->active.size(); ++i)
+            h->rec_bmax[1 + i] = std::max(h->rec_bmax[1 + i], std::fabs(bfull[(size_t)v * L + h->active[i]]));
     const int need = (int)h->active.size() + 1;     // active terms + fieldmap
     static const int kBuckets[] = {6, 10, 12, 16, 20};
     h->LK = 20;
@@ -1124,7 +1136,7 @@ int hoe_set_kspha_dt(hoe_handle *hh, const float *kspha_dt) {
         if (!h->dt_on) return HOE_OK;
         HOE_TRY_HIP(h, hipStreamSynchronize(h->stream));
         h->dt_on = false;
-        return reschedule_direct(h, true);
+        return need_direct_resched(h) ? reschedule_direct(h, true) : HOE_OK;
     }
     if (h->has_conc)
         return set_err(h, HOE_EINVAL,
@@ -1134,21 +1146,34 @@ int hoe_set_kspha_dt(hoe_handle *hh, const float *kspha_dt) {
     for (int64_t i = 0; i < ns * n_term; ++i)
         if (!std::isfinite(kspha_dt[i])) return set_err(h, HOE_EINVAL, "kspha_dt must be finite");
     // records [kdt_1, kdt of the active terms, 0 (t slot), 0...]: terms whose basis is
-    // identically zero on the masked voxels contribute nothing
+    // identically zero on the masked voxels contribute nothing. psi_max bounds
+    // |psi(s, v)| = |sum_i r_i(s) b_i(v)| over every pair.
     std::vector<double> rec((size_t)h->ns_pad * h->sk, 0.0);
+    double psi_max = 0.0;
     for (int64_t s = 0; s < ns; ++s) {
         double *r = &rec[(size_t)s * h->sk];
         const float *k = kspha_dt + (size_t)s * n_term;
         r[0] = (double)k[0];
         for (size_t i = 0; i < h->active.size(); ++i) r[1 + i] = (double)k[h->active[i]];
+        double b = 0.0;
+        for (size_t i = 0; i < h->rec_bmax.size(); ++i) b += std::fabs(r[i]) * h->rec_bmax[i];
+        psi_max = std::max(psi_max, b);
     }
     HOE_TRY_HIP(h, hipStreamSynchronize(h->stream));
     HOE_TRY_HIP(h, hipMemcpy(h->dtrec.p, rec.data(), rec.size() * sizeof(double), hipMemcpyHostToDevice));
+    // the MFMA forward's A operand is 2 pi sigma psi e^{i phi}: sigma, a power of two, keeps
+    // it within [-1, 1] like the plain forward's e^{i phi}; the reduction divides it out
+    const double m = 6.283185307179586 * psi_max;
+    int ex = 0;
+    if (m > 0.0) (void)std::frexp(m, &ex);   // m < 2^ex
+    h->dt_sigma = std::ldexp(1.0, -ex);
+    h->dt_w = 6.283185307179586 * h->dt_sigma;
     if (h->dt_on) return HOE_OK;
-    // the FP32 forward's schedule is sized by the occupancy of the kernel instance that will
-    // run it: the derivative one from now on
     h->dt_on = true;
-    return reschedule_direct(h, false);
+    // the FP32 forward's schedule is sized by the occupancy of the kernel instance that will
+    // run it: the derivative one from now on (the MFMA derivative forward keeps the plain
+    // forward's schedule, its instance having the same occupancy)
+    return need_direct_resched(h) ? reschedule_direct(h, false) : HOE_OK;
 }
 
 int hoe_grad(hoe_handle *hh, const void *x, const void *y, float *g_h, double *g_tau, double *loss, int flags) {

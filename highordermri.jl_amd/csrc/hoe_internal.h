@@ -93,6 +93,8 @@ struct Handle {
     bool closed = false;
     bool weights_unit = false;        // dvec holds unit weights
     bool dt_on = false;               // hoe_forward computes the derivative product B x (kspha_dt set)
+    double dt_sigma = 1.0;            // power of two with 2 pi dt_sigma max|psi| <= 1 (MFMA derivative forward)
+    double dt_w = 0.0;                // 2 pi dt_sigma
     bool has_conc = false;            // the model carries the 4 concomitant terms
     bool order[4] = {true, true, true, true};   // recon_terms flags: orders 0..3 kept
     float csm_max = 0.0f;             // max |Re|,|Im| bound of the masked coil maps: max |csm|
@@ -154,6 +156,7 @@ struct Handle {
     DeviceBuf times_d;    // float [n_sam]
     DeviceBuf act_d;      // int [active terms]: 0-based term index of each kernel record entry 1..
     std::vector<float> gbscale_h;
+    std::vector<double> rec_bmax;     // max |b| over the masked voxels of each record entry (entry 0: 1)
     std::vector<double> adj_tile_w;   // stream-K work weight of each 256-voxel tile in the MFMA adjoint
     std::vector<double> fwd_k_w;      // stream-K work weight of each K unit (8 voxels) in the Horner forward
     int gb_type = -1;     // backend whose element type gbfrag holds

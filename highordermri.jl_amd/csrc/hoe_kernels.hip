@@ -212,6 +212,8 @@ struct HornerArgs {
     const HoeAdjAnchor *aanc; // [nv_pad / 64] group anchors of the MFMA Horner adjoint
     const float2 *aud;        // [nv_pad] voxel offsets from the adjoint group anchors
     int anchored;             // 0: the MFMA kernels take the FP64 generic phase on every step
+    const double *dtrec;      // [ns_pad][sk] derivative records (MFMA derivative forward)
+    double dtw;               // 2 pi sigma: psi's factor in the MFMA derivative forward's A operand
 };
 
 template <typename T16>
@@ -399,7 +401,7 @@ __device__ __forceinline__ float reduce_scatter16(float (&x)[16], int lane) {
 // [re hi | im hi | re lo | im lo] of coils 0..7, so Ahi*B + Alo*B holds all three
 // products (and the lo*lo term) in 4 MFMAs and one B fragment; column c + 16 is added
 // to column c in the epilogue.
-template <typename T16, bool FWD, bool HX, int L, int NB, int PK = 0>
+template <typename T16, bool FWD, bool HX, int L, int NB, int PK = 0, bool DT = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (FWD ? HOE_WAVES_FWD : HOE_WAVES_ADJ) : 2))) void k_mixed(
     const double *__restrict__ coef, int sk, const double *__restrict__ basis, int sb,
     const uint4 *__restrict__ frag, int n_ks, float2 *__restrict__ part, int ncs,
@@ -407,6 +409,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (
     const float2 *__restrict__ csm) {
     typedef typename mfma_t<T16>::vec vec_t;
     static_assert(PK == 0 || NB == 1, "packed coil layouts hold one group");
+    static_assert(FWD || !DT, "the derivative product is a forward product");
     constexpr int NQ = PK == 0 ? 4 : (PK == 1 ? 2 : 1);   // B fragments (uint4) per lane and K step
     constexpr int NA = PK == 0 ? 2 : 1;                   // accumulator tiles per row tile
     const int chunk = chunk_id(cpx);
@@ -433,23 +436,32 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (
             for (int b = 0; b < NB; ++b)
 #pragma unroll
                 for (int i = 0; i < NA; ++i) acc[a][b][i] = f16x_t{};
-        // K loop over this segment; phases_of(ks, ph) fills 8 phases per lane:
+        // K loop over this segment; phases_of(ks, ph, ps) fills 8 phases per lane:
         // with swap, those of the lane's own row for the 8 streamed rows of K
         // step ks (moved into the A layout by permlane32 swaps); without, the
         // A layout directly (rows lane%32 and 32 + lane%32, streamed rows
-        // 4*(lane/32)..+3 each).
+        // 4*(lane/32)..+3 each). DT: ps receives 2 pi sigma psi of the same pairs,
+        // and the A operand is 2 pi sigma psi i e = 2 pi sigma psi (-sin + i cos)
+        // (|2 pi sigma psi| <= 1 by the host's power-of-two sigma, undone in the
+        // forward reduction).
         auto kloop = [&](auto phases_of, auto swap, auto ptype) {
             // one K step: phases of the 8 streamed rows (in cycles: FP64, or FP32 from the
             // anchored Horner forward), exp(i phi) split hi/lo into the A layout, 12 MFMAs
             // per coil group against the B fragments bfr
             auto step = [&](int ks, const uint4 (&bfr)[NB][NQ]) {
                 decltype(ptype) ph[8];
-                phases_of(ks, ph);
+                float ps[8];
+                phases_of(ks, ph, ps);
                 unsigned hp[8], lp[8];
 #pragma unroll
                 for (int j = 0; j < 8; ++j) {
                     float sn, cs;
                     sincos_cycles(ph[j], sn, cs);
+                    if constexpr (DT) {
+                        const float c2 = -sn * ps[j];
+                        sn = cs * ps[j];
+                        cs = c2;
+                    }
                     split2<T16>(cs, sn, hp[j], lp[j]);
                 }
                 if constexpr (decltype(swap)::value) {
@@ -553,15 +565,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (
             }
         };
         // generic phase: the lane's full record in registers, the streamed records
-        // from scalar loads, L FP64 FMAs per phase
+        // from scalar loads, L FP64 FMAs per phase (DT: and per psi)
         auto generic_path = [&]() {
-            double st[NST];
+            double st[NST], sd[DT ? L + 1 : 1];
             {
                 const double *p = FWD ? coef + (size_t)row * sk : basis + (size_t)row * sb;
 #pragma unroll
                 for (int i = 0; i < NST; ++i) st[i] = p[i];
+                if constexpr (DT) {
+#pragma unroll
+                    for (int i = 0; i <= L; ++i) sd[i] = hx.dtrec[(size_t)row * sk + i] * hx.dtw;
+                }
             }
-            kloop([&](int ks, double *ph) {
+            kloop([&](int ks, double *ph, float *ps) {
 #pragma unroll
                 for (int j = 0; j < 8; ++j) {
                     const int q = ks * 8 + j;
@@ -571,6 +587,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (
                         p = st[0];
 #pragma unroll
                         for (int i = 1; i <= L; ++i) p = fma(st[i], sm[i - 1], p);
+                        if constexpr (DT) {
+                            double q = sd[0];
+#pragma unroll
+                            for (int i = 1; i <= L; ++i) q = fma(sd[i], sm[i - 1], q);
+                            ps[j] = (float)q;
+                        }
                     } else {
                         p = sm[L] * st[L - 1];
 #pragma unroll
@@ -595,11 +617,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (
             const float tf = (float)t;
             double hc[4] = {0.0, 0.0, 0.0, 0.0};
             float c1f = 0.0f, c2f = 0.0f, c2x2f = 0.0f, c3f = 0.0f, c3x3f = 0.0f;
+            // DT: psi of the run, a cubic in x like P (scaled by 2 pi sigma), evaluated in FP32
+            // as a Taylor expansion at the unit's anchor (no reduction: psi is no phase)
+            float p0f = 0.0f, p1f = 0.0f, p2f = 0.0f, p2x2f = 0.0f, p3f = 0.0f, p3x3f = 0.0f;
             int cur = -1;
             // the anchor of the next K unit is fetched during this step, so that the step's
             // branch does not wait on a scalar load before the unit's voxel offsets are fetched
             HoeUnit an = hx.units[d.y];
-            kloop([&](int ks, float *ph) {
+            kloop([&](int ks, float *ph, float *ps) {
                 const int q0 = ks * 8;
                 const HoeUnit a = an;
                 const int r0 = a.run;
@@ -612,6 +637,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (
                         c2x2f = (float)(2.0 * hc[2]);
                         c3f = (float)hc[3];
                         c3x3f = (float)(3.0 * hc[3]);
+                        if constexpr (DT) {
+                            double hd[4];
+                            run_setup<L>(hx.dtrec + (size_t)row * sk, hx.beta + (size_t)r0 * L * 4, hd);
+                            p0f = (float)(hx.dtw * hd[0]);
+                            p1f = (float)(hx.dtw * hd[1]);
+                            p2f = (float)(hx.dtw * hd[2]);
+                            p2x2f = (float)(2.0 * hx.dtw * hd[2]);
+                            p3f = (float)(hx.dtw * hd[3]);
+                            p3x3f = (float)(3.0 * hx.dtw * hd[3]);
+                        }
                         cur = r0;
                     }
                     const double xa = (double)a.xa;
@@ -623,6 +658,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (
                     for (int j = 0; j < 8; ++j) {
                         const float u = uv.ud[j][0], dfm = uv.ud[j][1];
                         ph[j] = fmaf(tf, dfm, fmaf(fmaf(fmaf(c3f, u, d2), u, d1), u, base));
+                    }
+                    if constexpr (DT) {
+                        const float qa = fmaf(fmaf(fmaf(p3f, a.xa, p2f), a.xa, p1f), a.xa, p0f);
+                        const float q2 = fmaf(p3x3f, a.xa, p2f);
+                        const float q1 = fmaf(fmaf(p3x3f, a.xa, p2x2f), a.xa, p1f);
+#pragma unroll
+                        for (int j = 0; j < 8; ++j) {
+                            const float u = uv.ud[j][0];
+                            ps[j] = fmaf(fmaf(fmaf(p3f, u, q2), u, q1), u, qa);
+                        }
                     }
                 } else {
                     // a run boundary inside the K step (or anchored phases off for this
@@ -638,6 +683,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (
 #pragma unroll
                         for (int i = 1; i <= L; ++i) p = fma(a[i], sm[i - 1], p);
                         ph[j] = (float)__builtin_amdgcn_fract(p);
+                    }
+                    if constexpr (DT) {
+                        const double *dp = hx.dtrec + (size_t)row * sk;
+                        double b[L + 1];
+#pragma unroll
+                        for (int i = 0; i <= L; ++i) b[i] = dp[i];
+#pragma unroll
+                        for (int j = 0; j < 8; ++j) {
+                            const double *sm = basis + (size_t)(q0 + j) * sb;
+                            double q = b[0];
+#pragma unroll
+                            for (int i = 1; i <= L; ++i) q = fma(b[i], sm[i - 1], q);
+                            ps[j] = (float)(hx.dtw * q);
+                        }
                     }
                 }
                 an = hx.units[min(ks + 1, n_ks - 1)];
@@ -749,7 +808,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (
             load_a(d.y, anx);
             if (d.y < d.z) setup(anx, cl);
             load_a(d.y + 2, anx);
-            kloop([&](int ks, float *ph) {
+            kloop([&](int ks, float *ph, float *) {
                 const int rel = ks - d.y, k2 = rel & 1;
                 const float *cur = cl + ((rel >> 1) & 1) * SLOT;
                 if (k2 == 1 && ks + 1 < d.z) {
@@ -1959,8 +2018,6 @@ __global__ void k_copy2(cplx_t<R> *__restrict__ d, const cplx_t<R> *__restrict__
 
 static inline unsigned *slot(Handle *h, int i) { return reinterpret_cast<unsigned *>(h->scal.p) + 64 + i; }
 
-// Backend of the forward product: the derivative product (kspha_dt set) runs on the FP32 kernels.
-static inline int fwd_backend(const Handle *h) { return h->dt_on ? HOE_BACKEND_DIRECT : h->backend; }
 static inline double *cgs(Handle *h) { return reinterpret_cast<double *>(h->scal.p); }
 
 static inline int blocks_for(int64_t n, int b = 256) { return (int)((n + b - 1) / b); }
@@ -2027,23 +2084,23 @@ static void visit_direct(int L, int cb, F &&f) {
 }
 // shape: coil groups per wave (1, 2) for 32-coil groups, or 3 / 4 for the packed layouts
 // of up to 16 / 8 coils (see k_mixed)
-template <typename T16, bool FWD, bool HX, int L, class F>
+template <typename T16, bool FWD, bool HX, int L, bool DT, class F>
 static void visit_mixed_nb(int shape, F &&f) {
     switch (shape) {
-        case 1: f(k_mixed<T16, FWD, HX, L, 1>); break;
-        case 3: f(k_mixed<T16, FWD, HX, L, 1, 1>); break;
-        case 4: f(k_mixed<T16, FWD, HX, L, 1, 2>); break;
-        default: f(k_mixed<T16, FWD, HX, L, 2>); break;
+        case 1: f(k_mixed<T16, FWD, HX, L, 1, 0, DT>); break;
+        case 3: f(k_mixed<T16, FWD, HX, L, 1, 1, DT>); break;
+        case 4: f(k_mixed<T16, FWD, HX, L, 1, 2, DT>); break;
+        default: f(k_mixed<T16, FWD, HX, L, 2, 0, DT>); break;
     }
 }
-template <typename T16, bool FWD, bool HX, class F>
+template <typename T16, bool FWD, bool HX, bool DT = false, class F>
 static void visit_mixed(int L, int shape, F &&f) {
     switch (L) {
-        case 6: visit_mixed_nb<T16, FWD, HX, 6>(shape, f); break;
-        case 10: visit_mixed_nb<T16, FWD, HX, 10>(shape, f); break;
-        case 12: visit_mixed_nb<T16, FWD, HX, 12>(shape, f); break;
-        case 16: visit_mixed_nb<T16, FWD, HX, 16>(shape, f); break;
-        default: visit_mixed_nb<T16, FWD, HX, 20>(shape, f); break;
+        case 6: visit_mixed_nb<T16, FWD, HX, 6, DT>(shape, f); break;
+        case 10: visit_mixed_nb<T16, FWD, HX, 10, DT>(shape, f); break;
+        case 12: visit_mixed_nb<T16, FWD, HX, 12, DT>(shape, f); break;
+        case 16: visit_mixed_nb<T16, FWD, HX, 16, DT>(shape, f); break;
+        default: visit_mixed_nb<T16, FWD, HX, 20, DT>(shape, f); break;
     }
 }
 template <int L, class F>
@@ -2124,15 +2181,30 @@ static inline int mixed_shape(const Handle *h) {
     const int pk = mixed_pack(h);
     return pk ? 2 + pk : mixed_nb(h);
 }
+// dt: the derivative forward (the schedules are built for the plain forward, whose
+// instance has the same occupancy, so that setting kspha_dt keeps them)
 template <bool FWD, class F>
-static void visit_contract_mixed(Handle *h, int backend, F &&f) {
+static void visit_contract_mixed(Handle *h, int backend, F &&f, bool dt = false) {
     if constexpr (FWD) {
+        const int sh = mixed_shape(h);
         if (backend == HOE_BACKEND_MIXED_FP16) {
-            if (h->horner) visit_mixed<_Float16, true, true>(h->LK, mixed_shape(h), f);
-            else visit_mixed<_Float16, true, false>(h->LK, mixed_shape(h), f);
+            if (dt) {
+                if (h->horner) visit_mixed<_Float16, true, true, true>(h->LK, sh, f);
+                else visit_mixed<_Float16, true, false, true>(h->LK, sh, f);
+            } else if (h->horner) {
+                visit_mixed<_Float16, true, true>(h->LK, sh, f);
+            } else {
+                visit_mixed<_Float16, true, false>(h->LK, sh, f);
+            }
         } else {
-            if (h->horner) visit_mixed<__bf16, true, true>(h->LK, mixed_shape(h), f);
-            else visit_mixed<__bf16, true, false>(h->LK, mixed_shape(h), f);
+            if (dt) {
+                if (h->horner) visit_mixed<__bf16, true, true, true>(h->LK, sh, f);
+                else visit_mixed<__bf16, true, false, true>(h->LK, sh, f);
+            } else if (h->horner) {
+                visit_mixed<__bf16, true, true>(h->LK, sh, f);
+            } else {
+                visit_mixed<__bf16, true, false>(h->LK, sh, f);
+            }
         }
     } else {
         if (backend == HOE_BACKEND_MIXED_FP16) {
@@ -2447,7 +2519,7 @@ hipError_t prepare_backend(Handle *h, int backend) {
     const int64_t seg_bytes = (int64_t)256 * h->ncs * h->cbytes, budget = part_budget();
     const int64_t seg_bytes_a = (int64_t)256 * groups_a * h->cbytes;
     // the Horner forward (MFMA and FP32) weighs its run-crossing K units
-    const bool fweighted = h->horner && !h->dt_on &&
+    const bool fweighted = h->horner && !(h->dt_on && !is_mixed) &&
                            std::any_of(h->fwd_k_w.begin(), h->fwd_k_w.end(), [](double w) { return w != 1.0; });
     if ((e = build_passes(sf, (int)(h->ns_pad / 256), (int)(h->nv_pad / 8), slots_f, 0, nullptr,
                           fweighted ? &h->fwd_k_w : nullptr, seg_bytes, budget)) != hipSuccess)
@@ -2544,13 +2616,13 @@ static void launch_direct(void (*k)(const double *, int, const double *, int, co
 
 template <bool FWD>
 static void launch_contract(Handle *h, const Sched &sc, const void *operand, void *part) {
-    const int backend = FWD ? fwd_backend(h) : h->backend;
+    const int backend = h->backend;
     const double *coef = (const double *)h->coef.p, *basis = (const double *)h->basis.p;
     const int4 *segs = (const int4 *)sc.seg.p;
     const int *coff = (const int *)sc.chunk_off.p;
     HornerArgs hx{(const double2 *)h->hrec.p, (const int *)h->vrun.p, (const double *)h->beta.p, h->t_index,
                   (const HoeUnit *)h->units.p, (const HoeUnitVox *)h->uvox.p, (const HoeAdjAnchor *)h->aanc.p,
-                  (const float2 *)h->aud.p, h->anchored ? 1 : 0};
+                  (const float2 *)h->aud.p, h->anchored ? 1 : 0, (const double *)h->dtrec.p, h->dt_w};
     if (backend == HOE_BACKEND_DIRECT || backend == HOE_BACKEND_FP64) {
         dim3 grid(sc.n_chunks, direct_batches(h, FWD));
         const int n_stream = (int)(FWD ? h->n_vox : h->n_sam);
@@ -2563,7 +2635,7 @@ static void launch_contract(Handle *h, const Sched &sc, const void *operand, voi
         visit_contract_mixed<FWD>(h, backend, [&](auto k) {
             k<<<grid, 256, 0, h->stream>>>(coef, h->sk, basis, h->sb, (const uint4 *)operand, sc.n_ks,
                                            (float2 *)part, h->ncs, segs, coff, sc.cpx, hx, (const float2 *)h->csm.p);
-        });
+        }, FWD && h->dt_on);
     }
 }
 
@@ -2578,7 +2650,7 @@ static inline bool direct_path(int backend) {
 // the last in between.
 template <bool FWD, class Red>
 static hipError_t contract_passes(Handle *h, const void *operand, void *part, bool time_it, int ev0, Red red) {
-    const std::vector<Sched> &passes = FWD ? h->sched_f[fwd_backend(h)] : h->sched_a[h->backend];
+    const std::vector<Sched> &passes = FWD ? h->sched_f[h->backend] : h->sched_a[h->backend];
     if (time_it) (void)hipEventRecord(h->ev[ev0], h->stream);
     for (size_t p = 0; p < passes.size(); ++p) {
         launch_contract<FWD>(h, passes[p], operand, part);
@@ -2594,7 +2666,7 @@ static hipError_t contract_passes(Handle *h, const void *operand, void *part, bo
 // already holds max|xm| (gathered).
 template <class Red>
 static hipError_t forward_from_xm(Handle *h, const void *xmv, bool time_it, bool have_max, Red red) {
-    if (direct_path(fwd_backend(h))) {
+    if (direct_path(h->backend)) {
         const int64_t nz = h->nv_pad * h->ncs;
         by_prec(h, [&](auto r) {
             using R = decltype(r);
@@ -2645,7 +2717,9 @@ static int fwd_reduce_slices(const Handle *h, const Sched &sc) {
 // Reduction of one forward pass (its 32-row blocks of samples) into the user signal
 // layout (complex [c][s]) with scale.
 static hipError_t launch_reduce_fwd_user(Handle *h, const Sched &sc, void *y_out, double scale) {
-    const bool unscale = !direct_path(fwd_backend(h));
+    const bool unscale = !direct_path(h->backend);
+    // the MFMA derivative forward's A operand carries the power of two dt_sigma
+    if (unscale && h->dt_on) scale /= h->dt_sigma;
     dim3 grid((unsigned)((sc.t_hi - sc.t_lo) * 8), (unsigned)(h->ncs / 32));
     const int thr = 256 * fwd_reduce_slices(h, sc);
     by_prec(h, [&](auto r) {
@@ -2792,7 +2866,7 @@ static hipError_t run_grad_mfma(Handle *h, bool time_it) {
     }
     HornerArgs hx{(const double2 *)h->hrec.p, (const int *)h->vrun.p, (const double *)h->beta.p, h->t_index,
                   (const HoeUnit *)h->units.p, (const HoeUnitVox *)h->uvox.p, (const HoeAdjAnchor *)h->aanc.p,
-                  (const float2 *)h->aud.p, h->anchored ? 1 : 0};
+                  (const float2 *)h->aud.p, h->anchored ? 1 : 0, (const double *)h->dtrec.p, h->dt_w};
     if (time_it) (void)hipEventRecord(h->ev[0], h->stream);
     visit_gmixed(h, [&](auto k) {
         k<<<sc.n_chunks, 256, kGradLds, h->stream>>>(

@@ -241,7 +241,8 @@ int hoe_set_kspha(hoe_handle *h, const float *kspha, const float *k_nominal);
  * layout, not processed by recon_terms). While set, hoe_forward computes the
  * derivative product B x = 1/sqrt(nVox) sum_v 2 pi i (sum_l kdt_l(s) b_l(v))
  * exp(2 pi i phi) csm x (explicit_array_derivative, array_operator.jl:187-235;
- * delay estimation, FindDelay.jl:139-214) on the FP32 kernels, hoe_adjoint
+ * delay estimation, FindDelay.jl:139-214) on the handle's backend (the
+ * mixed backends at their own accuracy: 3e-5 FP16, 1e-4 BF16), hoe_adjoint
  * stays E^H (the reference's ctprod, array_operator.jl:167-171), and
  * hoe_normal / hoe_cg / hoe_grad return HOE_EINVAL. A model with
  * concomitant terms returns HOE_EINVAL: the reference builds the derivative

@@ -5,7 +5,11 @@ and the reference's operator semantics around it:
   - concomitant terms are ignored when kspha_dt is given (array_operator.jl:139,
     test_Concomitant.jl:124-129);
   - kspha_dt is used as given (recon_terms processes kspha only).
-Every backend is checked; the mixed backends run this product on the FP32 kernels.
+Every backend is checked, each at the bar of its other products (3e-5; 1e-4 for BF16):
+the mixed backends run this product on the matrix cores, their A operand carrying the
+factor 2 pi psi scaled by a power of two. Both phase paths are covered: the 12x10 grid
+takes the generic phase (runs too short for Horner), the 192x8 and 200x200 ones the
+anchored Horner forward.
 """
 import numpy as np
 import pytest
@@ -17,6 +21,7 @@ import hoe  # noqa: E402
 from hoe import synth  # noqa: E402
 
 BACKENDS = ["direct", "mixed_fp16", "mixed_bf16"]
+TOLS = {"direct": 3e-5, "mixed_fp16": 3e-5, "mixed_bf16": 1e-4}
 
 
 def _problem(seed=31, nz=1):
@@ -43,21 +48,25 @@ def test_derivative_product_vs_fp64(backend, gpu_lib):
         grid, kspha, kdt, times, fm, csm, x = _problem(nz=nz)
         m = O.make_model(grid, kspha, times, fm, csm, None)
         ref = O.forward_dt_f64(m, kdt, x)
+        tol = TOLS[backend]
         with hoe.HighOrderExplicitOp(grid, kspha, times, fieldmap=fm, csm=csm, kspha_dt=kdt,
                                      backend=backend) as op:
             bx = op * x
             e = O.relerr(bx, ref)
             print(f"B x {backend} nz={nz}: vs FP64 {e:.2e}")
-            assert e < 3e-5
+            assert e < tol
             # the adjoint is the plain E^H (the reference's ctprod)
             y = (np.random.default_rng(2).normal(size=op.nrow) + 0j).astype(np.complex64)
             xa = op.H * y
-            assert O.relerr(xa, O.adjoint_f64(m, y)) < 3e-5
+            assert O.relerr(xa, O.adjoint_f64(m, y)) < tol
             # clearing kspha_dt restores E
             op.set_kspha_dt(None)
-            assert O.relerr(op * x, O.forward_f64(m, x)) < 3e-5
+            assert O.relerr(op * x, O.forward_f64(m, x)) < tol
             op.set_kspha_dt(kdt)
-            assert O.relerr(op * x, ref) < 3e-5
+            assert O.relerr(op * x, ref) < tol
+            # B x is linear in kspha_dt: the scale of the A operand follows it
+            op.set_kspha_dt(kdt * np.float32(2.0 ** -20))
+            assert O.relerr(op * x, ref * 2.0 ** -20) < tol
 
 
 def test_derivative_semantics(gpu_lib):
@@ -84,20 +93,72 @@ def test_derivative_semantics(gpu_lib):
         assert O.relerr(op * x, O.forward_dt_f64(m, kdt, x)) < 3e-5
 
 
-@pytest.mark.parametrize("backend", ["direct", "mixed_fp16"])
+def _stencil_dt(kspha, dwell):
+    """kspha_dt from the 5-tap stencil of FindDelay.jl:179-180 (edge samples repeated)."""
+    k = np.asarray(kspha, np.float64)
+    pad = np.pad(k, ((0, 0), (2, 2)), mode="edge")
+    taps = np.array([1 / 8, 1 / 4, 0.0, -1 / 4, -1 / 8])
+    kdt = np.stack([np.convolve(pad[r], taps, mode="valid") for r in range(k.shape[0])]) / dwell
+    return kdt.astype(np.float32)
+
+
+@pytest.mark.parametrize("backend", BACKENDS)
 def test_derivative_product_workload1_subset(backend, gpu_lib):
     """Workload 1 geometry (200x200, 32 coils, Horner runs) on 256 spiral samples,
     kspha_dt from the 5-tap stencil of FindDelay.jl:179-180."""
     c = synth.make(1, n_sam=256)
-    k = c["kspha"].astype(np.float64)
-    pad = np.pad(k, ((0, 0), (2, 2)), mode="edge")
-    taps = np.array([1 / 8, 1 / 4, 0.0, -1 / 4, -1 / 8])
-    kdt = np.stack([np.convolve(pad[r], taps, mode="valid") for r in range(16)]) / c["dwell"]
-    kdt = kdt.astype(np.float32)
+    kdt = _stencil_dt(c["kspha"], c["dwell"])
     m = O.make_model(c["grid"], c["kspha"], c["times"], c["fieldmap"], c["csm"], None)
     ref = O.forward_dt_f64(m, kdt, c["image"])
     with hoe.HighOrderExplicitOp(c["grid"], c["kspha"], c["times"], fieldmap=c["fieldmap"], csm=c["csm"],
                                  kspha_dt=kdt, backend=backend) as op:
+        assert op.horner & 1, "runs long enough for the Horner forward"
         e = O.relerr(op * c["image"], ref)
     print(f"workload1 subset B x {backend}: vs FP64 {e:.2e}")
-    assert e < 3e-5
+    assert e < TOLS[backend]
+
+
+@pytest.mark.parametrize("case", ["192x8_6c", "disc40_12c", "96x12_40c", "192x8_6c_rough"])
+@pytest.mark.parametrize("backend", BACKENDS)
+def test_derivative_horner_layouts(case, backend, gpu_lib):
+    """The Horner forward's derivative paths and the coil layouts of the MFMA kernel:
+    - 192x8, 6 coils: every K unit inside one run (anchored psi), 8-coil packed layout;
+    - 40x40 with a disc mask, 12 coils: K units across run ends (psi from the full records
+      there), 16-coil packed layout;
+    - 96x12, 40 coils: two 32-coil groups per wave;
+    - 192x8 with a +-3 kHz uncorrelated fieldmap: anchored phases off (FP64 phase and psi
+      on every K step).
+    B x against the FP64 derivative operator (array_operator.jl:187-235 restated in the
+    oracle), then the plain forward after clearing kspha_dt."""
+    rng = np.random.default_rng(41)
+    nx, ny, nc = {"192x8_6c": (192, 8, 6), "disc40_12c": (40, 40, 12), "96x12_40c": (96, 12, 40),
+                  "192x8_6c_rough": (192, 8, 6)}[case]
+    fov = 0.22
+    grid = hoe.Grid(nx, ny, 1, np.float32(fov / nx), np.float32(fov / nx), np.float32(fov / nx))
+    mask = None
+    if case.startswith("disc"):
+        i, j = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+        mask = ((i - nx / 2 + 0.5) ** 2 + (j - ny / 2 + 0.5) ** 2 <= (0.45 * nx) ** 2).reshape(nx, ny, 1)
+    ns, dwell = 1024, 2e-6
+    kx, ky = synth.spiral(ns, nx / (2 * fov), 6)
+    kspha = synth.coefficient_rows(rng, np.stack([kx, ky, np.zeros(ns)]), fov / 2, ns).astype(np.float32)
+    times = (np.arange(ns) * dwell).astype(np.float32)
+    fm = synth.gaussian_b0(grid, rng, 120.0)
+    if case.endswith("rough"):
+        fm = rng.uniform(-3000.0, 3000.0, size=fm.shape).astype(np.float32)
+    csm = synth.gaussian_coils(grid, nc, rng, sigma=fov / 3, ring_radius=0.3 * fov, z_spread=0.0)
+    x = synth.phantom_image(grid, rng, (fov, fov * ny / nx, fov / nx))
+    kdt = _stencil_dt(kspha, dwell)
+    m = O.make_model(grid, kspha, times, fm, csm, mask)
+    ref = O.forward_dt_f64(m, kdt, x)
+    tol = TOLS[backend]
+    with hoe.HighOrderExplicitOp(grid, kspha, times, fieldmap=fm, csm=csm, mask=mask, kspha_dt=kdt,
+                                 backend=backend) as op:
+        assert op.horner & 1, "runs long enough for the Horner forward"
+        if backend != "direct":
+            assert bool(op.horner & 4) == (not case.endswith("rough"))
+        e = O.relerr(op * x, ref)
+        op.set_kspha_dt(None)
+        e0 = O.relerr(op * x, O.forward_f64(m, x))
+    print(f"{case} {backend} (horner {op.horner}): B x {e:.2e}, E x after clearing {e0:.2e}")
+    assert e < tol and e0 < tol
