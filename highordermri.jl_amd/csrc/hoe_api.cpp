@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <mutex>
 #include <new>
 #include <string>
@@ -236,6 +237,17 @@ bool need_direct_resched(const Handle *h) {
     return h->backend == HOE_BACKEND_DIRECT || !h->sched_f[HOE_BACKEND_DIRECT].empty();
 }
 
+// The MFMA kernels' decision for the current records: anchored phases need the fieldmap
+// bound (fixed at creation) and the device's Horner offset bound of the last record build.
+bool anchored_now(const Handle *h) {
+    if (!h->horner || !h->anchored || h->f64 || h->closed) return false;
+    float b = std::numeric_limits<float>::infinity();
+    if (hipStreamSynchronize(h->stream) != hipSuccess ||
+        hipMemcpy(&b, (const unsigned char *)h->abound.p + 256, sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+        return false;
+    return (double)b <= hoe::kAnchorMaxOffsetCycles;
+}
+
 // Without a mask the masked voxel order is the grid order (mask_idx is the identity).
 bool unmasked(const Handle *h) { return h->n_vox == h->n_grid; }
 
@@ -310,7 +322,8 @@ void release(Handle *h) {
                               &h->part_a,   &h->part_g, &h->dvec,   &h->io_x,    &h->io_x2,  &h->io_y,   &h->io_y2,
                               &h->scal,     &h->cg_vec, &h->red,    &h->gk,      &h->cg_log, &h->gzfrag, &h->grfrag,
                               &h->gbfrag,   &h->gbscale, &h->part_gm, &h->ghbuf, &h->growmap, &h->dtrec,
-                              &h->traces_d, &h->times_d,  &h->act_d,   &h->units,   &h->uvox,    &h->aanc,    &h->aud};
+                              &h->traces_d, &h->times_d,  &h->act_d,   &h->units,   &h->uvox,    &h->aanc,    &h->aud,
+                              &h->abound};
     for (auto *b : bufs) dfree(*b);
     for (int b = 0; b < 4; ++b) {
         hoe::drop_passes(h->sched_f[b]);
@@ -564,6 +577,7 @@ int create_impl(const Desc *d, hoe_handle **out) {
     std::vector<hoe::HoeUnitVox> uvox;
     std::vector<hoe::HoeAdjAnchor> aanc;
     std::vector<float> aud;
+    std::vector<double> abound;
     {
         int run = -1;
         T py = T(0), pz = T(0);
@@ -690,6 +704,34 @@ int create_impl(const Desc *d, hoe_handle **out) {
             h->anchor_fm_cycles = tmax * fmax_off;
             h->anchored = h->anchor_fm_cycles <= hoe::kAnchorMaxCycles;
         }
+        // D_r: the largest change of record entry r's run polynomial between a voxel and its
+        // anchor, over the forward's K units and the adjoint's groups. The device bounds the
+        // Horner offsets of the current records by max_s sum_r |A_s[r]| D_r (entry 0 and t: 0).
+        abound.assign(32, 0.0);
+        if (h->horner) {
+            auto poly = [&](int run, int r, double x) {
+                const double *p = &beta[((size_t)run * h->LK + r) * 4];
+                return p[0] + x * (p[1] + x * (p[2] + x * p[3]));
+            };
+            auto widen = [&](int run, double xa, double x) {
+                for (int r = 1; r <= na; ++r)
+                    abound[(size_t)r] = std::max(abound[(size_t)r], std::fabs(poly(run, r, x) - poly(run, r, xa)));
+            };
+            for (int64_t k = 0; k < h->nv_pad / 8; ++k) {
+                const hoe::HoeUnit &a = units[(size_t)k];
+                if (a.run < 0) continue;
+                for (int64_t v = 8 * k; v < std::min<int64_t>(8 * k + 8, nv); ++v)
+                    widen(a.run, (double)a.xa, hrec[2 * (size_t)v]);
+            }
+            for (int64_t w = 0; w < h->nv_pad / 64; ++w) {
+                const int r_lo = vrun[(size_t)(64 * w)];
+                if (vrun[(size_t)(64 * w + 63)] - r_lo > 1) continue;
+                for (int64_t v = 64 * w; v < std::min<int64_t>(64 * w + 64, nv); ++v) {
+                    const int c = 2 * (vrun[(size_t)v] - r_lo) + (int)((v % 64) / 32);
+                    widen(vrun[(size_t)v], (double)aanc[(size_t)w].xa[c], hrec[2 * (size_t)v]);
+                }
+            }
+        }
     }
 
     hipDeviceProp_t prop;
@@ -736,6 +778,7 @@ int create_impl(const Desc *d, hoe_handle **out) {
         {&h->uvox, sizeof(hoe::HoeUnitVox) * uvox.size()},
         {&h->aanc, sizeof(hoe::HoeAdjAnchor) * aanc.size()},
         {&h->aud, sizeof(float) * aud.size()},
+        {&h->abound, 512},
         {&h->xm, cb * (size_t)h->nv_pad},
         {&h->zbuf, cb * (size_t)h->nv_pad * h->ncs},
         {&h->zfrag, F64 ? 0 : (size_t)n_gb_pad * (size_t)(h->nv_pad / 8) * 64 * 64},
@@ -771,6 +814,14 @@ int create_impl(const Desc *d, hoe_handle **out) {
     if (e == hipSuccess)
         e = hipMemcpy(h->aanc.p, aanc.data(), sizeof(hoe::HoeAdjAnchor) * aanc.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(h->aud.p, aud.data(), sizeof(float) * aud.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+        // D_r, then the offset bound +inf (anchored phases off) until the first record build
+        std::vector<unsigned char> ab(512, 0);
+        std::memcpy(ab.data(), abound.data(), sizeof(double) * 32);
+        const float inf = std::numeric_limits<float>::infinity();
+        std::memcpy(ab.data() + 256, &inf, sizeof(float));
+        e = hipMemcpy(h->abound.p, ab.data(), ab.size(), hipMemcpyHostToDevice);
+    }
     if (e == hipSuccess && !F64)
         e = hipMemcpy(h->times_d.p, h->times.data(), sizeof(float) * (size_t)ns, hipMemcpyHostToDevice);
     if (e == hipSuccess && !h->active.empty())
@@ -816,7 +867,7 @@ int hoe_get_info(const hoe_handle *hh, hoe_info_t *info) {
     info->backend = h->backend;
     info->device = h->device;
     info->closed = h->closed ? 1 : 0;
-    info->horner = (h->horner ? 1 : 0) | (h->horner_adj ? 2 : 0) | (h->horner && h->anchored ? 4 : 0);
+    info->horner = (h->horner ? 1 : 0) | (h->horner_adj ? 2 : 0) | (anchored_now(h) ? 4 : 0);
     info->n_runs = h->n_runs;
     return HOE_OK;
 }

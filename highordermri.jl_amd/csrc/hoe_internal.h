@@ -39,6 +39,15 @@ struct alignas(16) HoeUnit {
 // MFMA Horner kernels evaluate anchored phases: its FP32 rounding (about 3 * 2^-24 of it,
 // 1.4e-6 cycles at the limit) stays far below the FP16 hi/lo split's 2^-22.
 constexpr double kAnchorMaxCycles = 8.0;
+// Largest Horner offset |P(x_a + u) - P(x_a)| over the samples and the anchor groups, in
+// cycles, for which they stay on: the device bounds it after each record build by
+// max_s sum_i |A_s[i]| D_i, D_i the largest change of record entry i's basis polynomial
+// within a K unit or adjoint group (about 12 cycles on trajectories up to the grid's
+// Nyquist radius; a trajectory reaching 4 times that doubles the FP32 rounding of the
+// offsets and stays within bounds). With both limits an anchored phase is at most
+// 1 + 32 + 8 cycles, inside the range where v_sin_f32 / v_cos_f32 take it without a
+// prior reduction (measured: the same 1.2e-7 error on +-64 cycles as on one).
+constexpr double kAnchorMaxOffsetCycles = 32.0;
 
 struct alignas(64) HoeUnitVox {
     float ud[8][2];   // (u_j, dfm_j)
@@ -122,6 +131,8 @@ struct Handle {
     DeviceBuf uvox;       // HoeUnitVox [nv_pad / 8]: voxel offsets from the anchors
     DeviceBuf aanc;       // HoeAdjAnchor [nv_pad / 64]: group anchors of the MFMA Horner adjoint
     DeviceBuf aud;        // float2 [nv_pad]: (x - x_c, fieldmap - fm_c) against the voxel's adjoint group
+    DeviceBuf abound;     // double [32] D_i per record entry, then (byte 256) the float bits of the
+                          // Horner offset bound of the current records (kAnchorMaxOffsetCycles)
     DeviceBuf vrun;       // int [nv_pad]: run of each voxel
     DeviceBuf beta;       // double [n_runs][LK][4]: x-polynomial coefficients of each record entry
 

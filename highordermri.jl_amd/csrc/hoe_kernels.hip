@@ -99,11 +99,14 @@ __device__ __forceinline__ void sincos_cycles(double ph, float &sn, float &cs) {
     cs = __builtin_amdgcn_cosf(r);
 }
 
-// The same from an FP32 phase in cycles (any magnitude below 2^23): v_fract_f32 is exact.
+// The same from an FP32 phase in cycles of magnitude at most 64, taken without a reduction:
+// v_sin_f32 / v_cos_f32 accept +-256 cycles and, measured on +-64, are as accurate there
+// (1.2e-7) as on one cycle. Every FP32 phase is either reduced already (fract of an FP64
+// phase) or an anchored one, whose size kAnchorMaxCycles and kAnchorMaxOffsetCycles bound
+// (at most 41 cycles).
 __device__ __forceinline__ void sincos_cycles(float ph, float &sn, float &cs) {
-    const float r = __builtin_amdgcn_fractf(ph);
-    sn = __builtin_amdgcn_sinf(r);
-    cs = __builtin_amdgcn_cosf(r);
+    sn = __builtin_amdgcn_sinf(ph);
+    cs = __builtin_amdgcn_cosf(ph);
 }
 
 // exp(2 pi i ph) in FP64 (the FP64 backend): ph - rint(ph) in [-1/2, 1/2] is exact, then
@@ -214,7 +217,15 @@ struct HornerArgs {
     int anchored;             // 0: the MFMA kernels take the FP64 generic phase on every step
     const double *dtrec;      // [ns_pad][sk] derivative records (MFMA derivative forward)
     double dtw;               // 2 pi sigma: psi's factor in the MFMA derivative forward's A operand
+    const float *obound;      // bound of the current records' Horner offsets (cycles), set on the device
 };
+
+// Anchored phases for this launch: the fieldmap offsets allow them (host, at creation) and
+// the Horner offsets of the current records stay within kAnchorMaxOffsetCycles (device, at
+// each record build). One scalar load per wave.
+__device__ __forceinline__ bool anchored_on(const HornerArgs &hx) {
+    return hx.anchored && *hx.obound <= (float)kAnchorMaxOffsetCycles;
+}
 
 template <typename T16>
 struct mfma_t;
@@ -624,11 +635,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (
             // the anchor of the next K unit is fetched during this step, so that the step's
             // branch does not wait on a scalar load before the unit's voxel offsets are fetched
             HoeUnit an = hx.units[d.y];
+            const bool anch = anchored_on(hx);
             kloop([&](int ks, float *ph, float *ps) {
                 const int q0 = ks * 8;
                 const HoeUnit a = an;
                 const int r0 = a.run;
-                if (r0 >= 0 && hx.anchored) {
+                if (r0 >= 0 && anch) {
                     const HoeUnitVox &uv = hx.uvox[ks];
                     if (r0 != cur) {
                         run_setup<L>(coef + (size_t)row * sk, hx.beta + (size_t)r0 * L * 4, hc);
@@ -707,7 +719,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (
             // takes the generic phase; the decision is uniform over the wave.
             const int wv0 = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
             const int base0 = d.x * 256 + wv0 * 64;
-            if (hx.vrun[base0 + 63] - hx.vrun[base0] > 1 || !hx.anchored) {
+            if (hx.vrun[base0 + 63] - hx.vrun[base0] > 1 || !anchored_on(hx)) {
                 generic_path();
             } else {
             // Horner adjoint. The wave's 64 voxels lie in at most two runs (r_lo, r_hi).
@@ -1089,6 +1101,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         double hc[4] = {0.0, 0.0, 0.0, 0.0};
         float c1f = 0.0f, c2f = 0.0f, c2x2f = 0.0f, c3f = 0.0f, c3x3f = 0.0f;
         int cur = -1;
+        const bool anch = HX && anchored_on(hx);
         if (HX) {
             st[0] = coef[(size_t)row * sk + hx.t_index];
         } else {
@@ -1143,7 +1156,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                     // fieldmap term at each unit's anchor, FP32 offsets of its 8 voxels)
                     const HoeUnit a0 = hx.units[q0 / 8], a1 = hx.units[q0 / 8 + 1];
                     const int r0 = a0.run;
-                    if (r0 >= 0 && a1.run == r0 && hx.anchored) {
+                    if (r0 >= 0 && a1.run == r0 && anch) {
                         if (r0 != cur) {
                             run_setup<L>(coef + (size_t)row * sk, hx.beta + (size_t)r0 * L * 4, hc);
                             c1f = (float)hc[1];
@@ -1846,6 +1859,27 @@ __global__ void k_build_records(const float *__restrict__ traces, int ns, int ns
     }
     coef[i] = c;
     dcoef[i] = dc;
+}
+
+// Bound of the anchored phases' Horner offsets for the records just built:
+// max_s sum_{r <= nr} |coef[s][r]| * D[r] into *out (float bits; cleared before the
+// launch, non-negative, so the unsigned order is the float order). 256 threads per block.
+__global__ __launch_bounds__(256) void k_anchor_bound(const double *__restrict__ coef, int sk, int ns,
+                                                      const double *__restrict__ D, int nr, unsigned *out) {
+    __shared__ float red[256];
+    float m = 0.0f;
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < ns; s += gridDim.x * blockDim.x) {
+        double b = 0.0;
+        for (int r = 1; r < nr; ++r) b = fma(fabs(coef[(size_t)s * sk + r]), D[r], b);
+        m = fmaxf(m, (float)b);
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicMax(out, __float_as_uint(red[0]));
 }
 
 // ---- gradient chain rule through the delay interpolation ----
@@ -2622,7 +2656,8 @@ static void launch_contract(Handle *h, const Sched &sc, const void *operand, voi
     const int *coff = (const int *)sc.chunk_off.p;
     HornerArgs hx{(const double2 *)h->hrec.p, (const int *)h->vrun.p, (const double *)h->beta.p, h->t_index,
                   (const HoeUnit *)h->units.p, (const HoeUnitVox *)h->uvox.p, (const HoeAdjAnchor *)h->aanc.p,
-                  (const float2 *)h->aud.p, h->anchored ? 1 : 0, (const double *)h->dtrec.p, h->dt_w};
+                  (const float2 *)h->aud.p, h->anchored ? 1 : 0, (const double *)h->dtrec.p, h->dt_w,
+                  (const float *)((const unsigned char *)h->abound.p + 256)};
     if (backend == HOE_BACKEND_DIRECT || backend == HOE_BACKEND_FP64) {
         dim3 grid(sc.n_chunks, direct_batches(h, FWD));
         const int n_stream = (int)(FWD ? h->n_vox : h->n_sam);
@@ -2866,7 +2901,8 @@ static hipError_t run_grad_mfma(Handle *h, bool time_it) {
     }
     HornerArgs hx{(const double2 *)h->hrec.p, (const int *)h->vrun.p, (const double *)h->beta.p, h->t_index,
                   (const HoeUnit *)h->units.p, (const HoeUnitVox *)h->uvox.p, (const HoeAdjAnchor *)h->aanc.p,
-                  (const float2 *)h->aud.p, h->anchored ? 1 : 0, (const double *)h->dtrec.p, h->dt_w};
+                  (const float2 *)h->aud.p, h->anchored ? 1 : 0, (const double *)h->dtrec.p, h->dt_w,
+                  (const float *)((const unsigned char *)h->abound.p + 256)};
     if (time_it) (void)hipEventRecord(h->ev[0], h->stream);
     visit_gmixed(h, [&](auto k) {
         k<<<sc.n_chunks, 256, kGradLds, h->stream>>>(
@@ -2921,6 +2957,11 @@ hipError_t launch_build_records(Handle *h, int shift, int m, const double w[4],
         (const float *)h->traces_d.p, (int)h->n_sam, (int)h->ns_pad, h->L, (const int *)h->act_d.p, (int)h->active.size(),
         (const float *)h->times_d.p, h->sk, shift, m, make_double4(w[0], w[1], w[2], w[3]),
         make_double4(dw[0], dw[1], dw[2], dw[3]), inv, (double *)h->coef.p, (double *)h->dcoef.p);
+    // the Horner offset bound of these records (entries 1..na: the active terms)
+    unsigned *ob = reinterpret_cast<unsigned *>((unsigned char *)h->abound.p + 256);
+    k_zero_u32<<<1, 64, 0, h->stream>>>(ob, 1);
+    k_anchor_bound<<<std::max(1, std::min(256, blocks_for(h->n_sam))), 256, 0, h->stream>>>(
+        (const double *)h->coef.p, h->sk, (int)h->n_sam, (const double *)h->abound.p, (int)h->active.size() + 1, ob);
     return hipGetLastError();
 }
 

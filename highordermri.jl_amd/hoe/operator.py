@@ -248,8 +248,7 @@ class HighOrderExplicitOp:
         info = _lib.Info()
         check(lib.hoe_get_info(h, C.byref(info)), h)
         self.nVox = int(info.n_vox)
-        self.horner = int(info.horner)         # bit 0: Horner forward, bit 1: Horner MFMA adjoint,
-                                               # bit 2: anchored phases in the MFMA Horner kernels
+        self._horner_last = int(info.horner)
         self.nrow, self.ncol = n_sam * n_cha, n_grid
         self.device = gpus[0]
         self.backend = backend
@@ -258,6 +257,19 @@ class HighOrderExplicitOp:
         self.kspha_dt = None
         if kdt is not None:
             self.set_kspha_dt(kdt)
+
+    @property
+    def horner(self):
+        """Phase paths of the current model: bit 0 the Horner forward, bit 1 the Horner MFMA
+        adjoint, bit 2 anchored phases in the MFMA Horner kernels (their fieldmap offsets and,
+        for the coefficients last set, their Horner offsets within bounds). A closed operator
+        reports its last value."""
+        if self._h is None:
+            return self._horner_last
+        info = _lib.Info()
+        check(self._lib.hoe_get_info(self._h, C.byref(info)), self._h)
+        self._horner_last = int(info.horner)
+        return self._horner_last
 
     # ---- LinearOperator protocol ----
     @property

@@ -137,7 +137,9 @@ typedef struct {
     int32_t device;
     int32_t closed;
     int32_t horner;   /* phase evaluation: bit 0 = Horner runs in the forward kernels, bit 1 = in the MFMA adjoint,
-                         bit 2 = the MFMA kernels evaluate anchored phases (FP64 anchor, FP32 offsets) on them */
+                         bit 2 = the MFMA kernels evaluate anchored phases (FP64 anchor, FP32 offsets) on them
+                         for the coefficients last set (the fieldmap offsets and the Horner offsets within
+                         bounds; reading it waits for the handle's queued work) */
     int32_t n_runs;   /* runs of consecutive masked voxels sharing (y, z) */
 } hoe_info_t;
 

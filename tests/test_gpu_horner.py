@@ -156,3 +156,43 @@ def test_anchored_phase_bound(fm_kind, expect_anchored, backend, gpu_lib):
     print(f"{fm_kind} {backend} (horner {op.horner}): E {e_f:.2e}  E^H {e_a:.2e}  E^H E {e_n:.2e}  grad {e_g:.2e}")
     assert e_f < tol and e_a < tol and e_n < tol
     assert e_g < 1e-4
+
+
+@pytest.mark.parametrize("kscale,expect_anchored", [(1.0, True), (2.0, True), (8.0, False), (16.0, False)])
+@pytest.mark.parametrize("backend", ["mixed_fp16", "mixed_bf16"])
+def test_anchored_offset_bound(kscale, expect_anchored, backend, gpu_lib):
+    """Trajectories reaching kscale times the grid's Nyquist radius (every coefficient row
+    scaled): the Horner offsets P(x_a + u) - P(x_a) of the anchored phases, added in FP32,
+    grow with them (about 8 cycles per Nyquist radius in the adjoint's 32-voxel groups). After
+    each record build the device bounds them, and above 32 cycles the MFMA kernels take the
+    FP64 phase of every pair. On a 192x8 grid: E x, E^H y and E^H E x against the FP64
+    operator at 3e-5 (1e-4 for BF16) in both regimes; the decision follows hoe_set_kspha (off
+    at 8 times the trajectory, on again at the original)."""
+    grid, kspha0, times, fm, csm, x, _ = _problem(192, 8, 1)
+    kspha = (kspha0 * np.float32(kscale)).astype(np.float32)
+    tol = TOLS[backend]
+
+    def check(op, k):
+        m = O.make_model(grid, k, times, fm, csm, None)
+        y_ref = O.forward_f64(m, x)
+        probe = (np.random.default_rng(1).normal(size=y_ref.size)
+                 + 1j * np.random.default_rng(2).normal(size=y_ref.size)).astype(np.complex64)
+        e_f = O.relerr(op * x, y_ref)
+        e_a = O.relerr(op.H * probe, O.adjoint_f64(m, probe))
+        e_n = O.relerr(hoe.normalOperator(op) * x, O.normal_f64(m, x))
+        return e_f, e_a, e_n
+
+    with hoe.HighOrderExplicitOp(grid, kspha, times, fieldmap=fm, csm=csm, backend=backend) as op:
+        assert op.horner & 3 == 3, "runs long enough for the Horner kernels"
+        assert bool(op.horner & 4) == expect_anchored
+        e = check(op, kspha)
+        print(f"kscale {kscale} {backend} (horner {op.horner}): E {e[0]:.2e}  E^H {e[1]:.2e}  E^H E {e[2]:.2e}")
+        assert max(e) < tol
+        if kscale == 1.0:
+            k8 = (kspha0 * np.float32(8.0)).astype(np.float32)
+            op.set_kspha(k8)
+            assert not op.horner & 4
+            assert max(check(op, k8)) < tol
+            op.set_kspha(kspha0)
+            assert op.horner & 4
+            assert max(check(op, kspha0)) < tol
