@@ -42,11 +42,11 @@ constexpr double kAnchorMaxCycles = 8.0;
 // Largest Horner offset |P(x_a + u) - P(x_a)| over the samples and the anchor groups, in
 // cycles, for which they stay on: the device bounds it after each record build by
 // max_s sum_i |A_s[i]| D_i, D_i the largest change of record entry i's basis polynomial
-// within a K unit or adjoint group (about 12 cycles on trajectories up to the grid's
-// Nyquist radius; a trajectory reaching 4 times that doubles the FP32 rounding of the
-// offsets and stays within bounds). With both limits an anchored phase is at most
-// 1 + 32 + 8 cycles, inside the range where v_sin_f32 / v_cos_f32 take it without a
-// prior reduction (measured: the same 1.2e-7 error on +-64 cycles as on one).
+// within a K unit or adjoint group (2-12 cycles on trajectories that reach the grid's
+// Nyquist radius; the FP32 rounding of the offsets grows in proportion). With both limits
+// an anchored phase is at most 1 + 32 + 8 cycles, inside the range where v_sin_f32 /
+// v_cos_f32 take it without a prior reduction (measured: the same 1.2e-7 error on
+// +-64 cycles as on one).
 constexpr double kAnchorMaxOffsetCycles = 32.0;
 
 struct alignas(64) HoeUnitVox {
