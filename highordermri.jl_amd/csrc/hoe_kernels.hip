@@ -193,13 +193,14 @@ __device__ __forceinline__ void split2<__bf16>(float c, float s, unsigned &hi, u
 // Horner forward: voxels of one run share (y, z), so every record entry is a
 // cubic in x and phi(s,v) = c0 + x (c1 + x (c2 + x c3)) + t_s * fieldmap_v with
 // c_j(s) = sum_i A_s[i] * beta[run][i][j], set up once per run and lane.
-template <int L>
+// (S: stride between the record's entries, 1 in global memory, 64 in a wave's LDS copy)
+template <int L, int S = 1>
 __device__ __forceinline__ void run_setup(const double *__restrict__ arec, const double *__restrict__ beta_run,
                                           double c[4]) {
     c[0] = c[1] = c[2] = c[3] = 0.0;
 #pragma unroll
     for (int i = 0; i < L; ++i) {
-        const double a = arec[i];
+        const double a = arec[i * S];
 #pragma unroll
         for (int j = 0; j < 4; ++j) c[j] = fma(a, beta_run[4 * i + j], c[j]);
     }
@@ -626,6 +627,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (
             // 4 FP64 FMAs, a fract and a conversion.
             const double t = coef[(size_t)row * sk + hx.t_index];
             const float tf = (float)t;
+            // the lane's record entries 0..L, copied once per segment into the wave's LDS
+            // ([entry][lane]): the run setups and the run-crossing steps read it there. Read
+            // from global memory at every run change, the records of all the XCD's tiles
+            // (2 MB at 16 000 samples and L = 10, 3.9 MB at L = 20) competed with the K part's
+            // fragments for its L2.
+            __shared__ double rlds[HX && FWD ? 4 * (L + 1) * 64 : 1];
+            double *myr = rlds + (threadIdx.x >> 6) * (L + 1) * 64 + lane;
+            {
+                const double *ap = coef + (size_t)row * sk;
+#pragma unroll
+                for (int i = 0; i <= L; ++i) myr[i * 64] = ap[i];
+            }
             double hc[4] = {0.0, 0.0, 0.0, 0.0};
             float c1f = 0.0f, c2f = 0.0f, c2x2f = 0.0f, c3f = 0.0f, c3x3f = 0.0f;
             // DT: psi of the run, a cubic in x like P (scaled by 2 pi sigma), evaluated in FP32
@@ -643,7 +656,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (
                 if (r0 >= 0 && anch) {
                     const HoeUnitVox &uv = hx.uvox[ks];
                     if (r0 != cur) {
-                        run_setup<L>(coef + (size_t)row * sk, hx.beta + (size_t)r0 * L * 4, hc);
+                        run_setup<L, 64>(myr, hx.beta + (size_t)r0 * L * 4, hc);
                         c1f = (float)hc[1];
                         c2f = (float)hc[2];
                         c2x2f = (float)(2.0 * hc[2]);
@@ -684,10 +697,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (
                 } else {
                     // a run boundary inside the K step (or anchored phases off for this
                     // operator): generic phase from the full records
-                    const double *ap = coef + (size_t)row * sk;
                     double a[L + 1];
 #pragma unroll
-                    for (int i = 0; i <= L; ++i) a[i] = ap[i];
+                    for (int i = 0; i <= L; ++i) a[i] = myr[i * 64];
 #pragma unroll
                     for (int j = 0; j < 8; ++j) {
                         const double *sm = basis + (size_t)(q0 + j) * sb;
