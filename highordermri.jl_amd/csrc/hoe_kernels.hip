@@ -766,6 +766,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (
             const int v0 = base + (lane & 31), v1 = v0 + 32;
             const float2 o0 = hx.aud[v0], o1 = hx.aud[v1];
             const int c0 = 2 * (hx.vrun[v0] - r_lo), c1 = 2 * (hx.vrun[v1] - r_lo) + 1;
+            // a wave across two runs leaves one of its four groups empty (two when the run
+            // boundary is the wave's middle): (r_lo, upper half) or (r_hi, lower half)
+            const bool has1 = __any(c1 == 1), has2 = __any(c0 == 2);
             const int half = lane >> 5;
             const int kq = lane >> 4;
             // B entry (k, j) of group c and k block m: Taylor coefficient j at x_c of record
@@ -811,6 +814,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {
                     if (c == 2 && !two) break;
+                    if ((c == 1 && !has1) || (c == 2 && !has2)) continue;
                     double acc = 0.0;
 #pragma unroll
                     for (int m = 0; m < NM; ++m)
