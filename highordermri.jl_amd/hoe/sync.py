@@ -241,6 +241,51 @@ def _delay_step(y1, y2):
     return float((np.vdot(y2, y1) / np.vdot(y2, y2)).real)
 
 
+class _Signals:
+    """The measured signal ``yv`` (nSam*nCha, coil-major) and the update's products
+    (FindDelay.jl:194-196). With torch and a visible GPU they stay on the operator's
+    device: the reconstruction reads the signal there, ``y - A x`` and ``B x`` are formed
+    there, and only the scalar step comes back (no host round trip of the signals per
+    update). Otherwise they live on the host."""
+
+    def __init__(self, yv):
+        self.yv = yv
+        self.torch = None
+        self.y = None
+        try:
+            import torch
+            if torch.cuda.is_available():
+                self.torch = torch
+        except ImportError:
+            pass
+
+    def data(self, op):
+        """The signal for ``recon_HOOp`` on ``op``'s device (or the host array)."""
+        if self.torch is None:
+            return self.yv
+        if self.y is None:
+            dev = self.torch.device("cuda", op.device)
+            self.y = self.torch.from_numpy(np.ascontiguousarray(self.yv.astype(op._cdtype))).to(dev)
+        return self.y
+
+    def step(self, r, x, kdt_tau):
+        """``real((B x) \\ (y - A x))`` with ``A`` = ``r`` and ``B`` its derivative model."""
+        if self.torch is None:
+            y1 = self.yv - r * x
+            r.set_kspha_dt(kdt_tau)
+            y2 = r * x
+            r.set_kspha_dt(None)
+            return _delay_step(y1, y2)
+        t = self.torch
+        y = self.data(r)
+        xd = t.from_numpy(np.ascontiguousarray(np.asarray(x).astype(r._cdtype))).to(y.device)
+        y1 = (y - r * xd).to(t.complex128)
+        r.set_kspha_dt(kdt_tau)
+        y2 = (r * xd).to(t.complex128)
+        r.set_kspha_dt(None)
+        return float((t.vdot(y2, y1) / t.vdot(y2, y2)).real)
+
+
 class _Loop:
     """The update rule shared by both estimators (FindDelay.jl:196-210)."""
 
@@ -325,7 +370,7 @@ def FindDelay(gridding, data, kspha, *args, JumpFact=3, Δτ_min=0.005, intermod
     kspha = np.asarray(kspha)
     kspha_dt = kspha_derivative(kspha, dt)
     rec = _rec_params(gridding, reg, iter_max, solver)
-    yv = data.reshape(-1, order="F")
+    sig = _Signals(data.reshape(-1, order="F"))
     loop = _Loop(JumpFact, Δτ_min, dt, max_updates)
     if kspha.ndim != 2:
         raise DimensionMismatch("traj must be (nFieldSample, nTerm) or (nShot, nFieldSample, nTerm)")
@@ -342,12 +387,8 @@ def FindDelay(gridding, data, kspha, *args, JumpFact=3, Δτ_min=0.005, intermod
                                          recon_terms=recon_terms, backend=backend, verbose=verbose)
             else:
                 op.set_kspha(k_tau)
-            x = recon_HOOp(op, data, w, rec).reshape(-1, order="F")
-            y1 = yv - op * x                  # y - A x
-            op.set_kspha_dt(kdt_tau)
-            y2 = op * x                       # B x
-            op.set_kspha_dt(None)
-            loop.update(_delay_step(y1, y2))
+            x = recon_HOOp(op, sig.data(op), w, rec).reshape(-1, order="F")
+            loop.update(sig.step(op, x, kdt_tau))   # real((B x) \ (y - A x))
     finally:
         if op is not None:
             op.close()
@@ -406,7 +447,7 @@ def FindDelay_multishot(gridding, data, kspha, *args, JumpFact=3, Δτ_min=0.005
     t_recon = np.ascontiguousarray(datatime.reshape(-1))            # ADC fastest, then shots
     t_first = np.tile(datatime[0], n_shot)
     same_times = np.array_equal(t_recon, t_first)
-    yv = data.reshape(-1, order="F")
+    sig = _Signals(data.reshape(-1, order="F"))
     loop = _Loop(JumpFact, Δτ_min, dt, max_updates)
     if StartTime.size != n_shot:
         raise AssertionError("Number of shots in trajectory must match length of delTime vector")
@@ -431,13 +472,8 @@ def FindDelay_multishot(gridding, data, kspha, *args, JumpFact=3, Δτ_min=0.005
                 op.set_kspha(k_r)
                 if op_res is not None:
                     op_res.set_kspha(k_r)
-            x = recon_HOOp(op, data, w, rec).reshape(-1, order="F")
-            r = op if op_res is None else op_res
-            y1 = yv - r * x
-            r.set_kspha_dt(kdt_r)
-            y2 = r * x
-            r.set_kspha_dt(None)
-            loop.update(_delay_step(y1, y2))
+            x = recon_HOOp(op, sig.data(op), w, rec).reshape(-1, order="F")
+            loop.update(sig.step(op if op_res is None else op_res, x, kdt_r))
     finally:
         for o in (op, op_res):
             if o is not None:
