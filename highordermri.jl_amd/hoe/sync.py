@@ -160,10 +160,52 @@ class _Resampler:
             raise ArgumentError("interpolation needs at least 2 field samples")
         self.x0 = float(dt) * np.arange(n)
         self.t = akima_tangents(self.x0, self.y) if self.mode == "akima" else None
+        self.torch = None
+
+    def to_device(self, torch):
+        """Evaluate on the current GPU with torch from now on (the same FP64 formulas; the
+        result still comes back as a numpy array): the delay loops' 2 x nTerm rows per update
+        then cost a few small kernels instead of host passes over every sample."""
+        dev = torch.device("cuda", torch.cuda.current_device())
+        self.torch, self.dev = torch, dev
+        self.x0_d = torch.from_numpy(self.x0).to(dev)
+        self.y_d = torch.from_numpy(np.ascontiguousarray(self.y)).to(dev)
+        self.t_d = torch.from_numpy(np.ascontiguousarray(self.t)).to(dev) if self.t is not None else None
+        self.xq_key, self.xq_d = None, None
+        return self
 
     def __call__(self, delTime, datatime):
         xq = np.asarray(datatime, dtype=np.float64).reshape(-1)
+        if self.torch is not None:
+            return self._eval_device(float(delTime), xq).astype(self.dtype)
         return _eval_columns(self.x0 - float(delTime), self.y, self.t, xq, self.mode).astype(self.dtype)
+
+    def _eval_device(self, delTime, xq):
+        """``_eval_columns`` on the device (float64 result)."""
+        tc = self.torch
+        if self.xq_key is None or self.xq_key.shape != xq.shape or not np.array_equal(self.xq_key, xq):
+            self.xq_key, self.xq_d = xq.copy(), tc.from_numpy(np.ascontiguousarray(xq)).to(self.dev)
+        x = self.x0_d - delTime
+        q = self.xq_d
+        n = x.numel()
+        inside = (q >= x[0]) & (q <= x[-1])
+        i = tc.clamp(tc.searchsorted(x, q, right=True) - 1, 0, n - 2)
+        h = x[i + 1] - x[i]
+        s = (q - x[i]) / h
+        if self.mode == "linear":
+            w0, w1 = 1.0 - s, s
+            out = self.y_d[i] * w0[:, None] + self.y_d[i + 1] * w1[:, None]
+        else:
+            s2 = s * s
+            s3 = s2 * s
+            h00 = 2 * s3 - 3 * s2 + 1
+            h10 = (s3 - 2 * s2 + s) * h
+            h01 = -2 * s3 + 3 * s2
+            h11 = (s3 - s2) * h
+            out = (self.y_d[i] * h00[:, None] + self.t_d[i] * h10[:, None]) + \
+                  (self.y_d[i + 1] * h01[:, None] + self.t_d[i + 1] * h11[:, None])
+        out = tc.where(inside[:, None], out, tc.zeros_like(out))
+        return out.cpu().numpy()
 
 
 def InterpTrajTime(traj, dt, delTime, datatime=None, *, intermode=None):
@@ -376,6 +418,9 @@ def FindDelay(gridding, data, kspha, *args, JumpFact=3, Δτ_min=0.005, intermod
         raise DimensionMismatch("traj must be (nFieldSample, nTerm) or (nShot, nFieldSample, nTerm)")
     # InterpTrajTime of the same samples at every update: node slopes formed once
     res_k, res_dt = _Resampler(kspha, dt, intermode), _Resampler(kspha_dt, dt, intermode)
+    if sig.torch is not None:
+        res_k.to_device(sig.torch)
+        res_dt.to_device(sig.torch)
     op = None
     try:
         while loop.running():
@@ -454,6 +499,9 @@ def FindDelay_multishot(gridding, data, kspha, *args, JumpFact=3, Δτ_min=0.005
     # InterpTrajTime of each shot's samples at every update: node slopes formed once per shot
     res_k = [_Resampler(kspha[i], dt, intermode) for i in range(n_shot)]
     res_dt = [_Resampler(kspha_dt[i], dt, intermode) for i in range(n_shot)]
+    if sig.torch is not None:
+        for r in res_k + res_dt:
+            r.to_device(sig.torch)
     op = op_res = None
     try:
         while loop.running():

@@ -148,3 +148,30 @@ def test_find_delay_at_200x200_32_coils(gpu_lib):
                                 np.float32(dt), max_updates=12, **kw)
     print(f"  with the right StartTime: τ/dt {np.round(np.array(hist0) / dt, 4)}")
     assert abs(tau0) < 0.05 * dt
+
+
+@pytest.mark.parametrize("intermode", [None, "LinearMonotonicInterpolation"])
+def test_device_resampling_matches_the_host(intermode, gpu_lib):
+    """The delay loops resample the field samples on the GPU (torch, FP64) with the host
+    resampler's formulas: InterpTrajTime's Akima and linear modes (InterpTraj.jl:18-44,
+    restated in hoe/sync.py and pinned there against scipy's Akima) at delays inside and
+    beyond the first and last samples (zero outside), to 1e-12 of the largest value."""
+    import torch
+    from hoe import sync
+    rng = np.random.default_rng(12)
+    n_field, dt = 900, 1e-6
+    s = np.arange(n_field) * dt
+    traj = np.stack([np.cos(2e4 * s + r) * (r + 1) + rng.normal(size=n_field) * 1e-2 for r in range(16)],
+                    axis=1).astype(np.float32)
+    datatime = (dt * np.arange(700) + 40 * dt).astype(np.float32)
+    host = sync._Resampler(traj, dt, intermode)
+    dev = sync._Resampler(traj, dt, intermode).to_device(torch)
+    for tau in (0.0, 3.3 * dt, -41.7 * dt, 250.2 * dt):
+        a, b = host(tau, datatime), dev(tau, datatime)
+        assert a.dtype == b.dtype == np.float32 and a.shape == b.shape == (700, 16)
+        ah = sync._eval_columns(host.x0 - tau, host.y, host.t, datatime.astype(np.float64), host.mode)
+        bd = dev._eval_device(tau, datatime.astype(np.float64))
+        scale = np.max(np.abs(ah))
+        assert bd.dtype == np.float64
+        assert np.max(np.abs(ah - bd)) <= 1e-12 * scale, (tau, np.max(np.abs(ah - bd)))
+        assert np.max(np.abs(a.astype(np.float64) - b.astype(np.float64))) <= 2 ** -22 * scale
