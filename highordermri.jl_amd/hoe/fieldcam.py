@@ -75,11 +75,25 @@ def _kb_kernel(d, half_width, beta):
     return np.where(np.abs(d) < half_width, np.i0(beta * np.sqrt(u)), 0.0)
 
 
+def _torch_gpu():
+    """torch when it sees a GPU, else None."""
+    try:
+        import torch
+    except ImportError:
+        return None
+    return torch if torch.cuda.is_available() else None
+
+
 def _pipe_menon(k, gshape, iters, hw=2.0, sigma=2.0):
     """Density estimate w (per sample) of normalised coordinates k ((dim, nSam) in
-    [-0.5, 0.5]) after ``iters`` iterations w <- w / (C * w) on the periodic grid."""
+    [-0.5, 0.5]) after ``iters`` iterations w <- w / (C * w) on the periodic grid.
+    Trajectories of 4096 samples or more run on the GPU when torch sees one
+    (``_pipe_menon_device``, the same FP64 arithmetic)."""
     beta = np.pi * np.sqrt((2 * hw / sigma) ** 2 * (sigma - 0.5) ** 2 - 0.8)
     ndim, ns = k.shape
+    tc = _torch_gpu() if ns >= 4096 else None
+    if tc is not None:
+        return _pipe_menon_device(tc, k, gshape, iters, hw, beta)
     # per dimension: the 5 grid cells around each sample and their kernel values
     idx, val = [], []
     for a in range(ndim):
@@ -102,6 +116,42 @@ def _pipe_menon(k, gshape, iters, hw=2.0, sigma=2.0):
         cw = (wt * grid[flat]).sum(axis=1)
         w = w / np.where(cw > 0, cw, 1.0)
     return w
+
+
+def _pipe_menon_device(tc, k, gshape, iters, hw, beta):
+    """``_pipe_menon`` with torch on the current GPU. The gridding sums run as a
+    segmented reduction over the (sample, tap) pairs sorted by grid cell (stable, so each
+    cell adds its pairs in the host's order) instead of atomics: the result does not vary
+    from run to run."""
+    dev = tc.device("cuda", tc.cuda.current_device())
+    kd = tc.from_numpy(np.ascontiguousarray(k, dtype=np.float64)).to(dev)
+    ndim, ns = kd.shape
+    taps = tc.arange(5, device=dev)
+    flat = tc.zeros((ns, 1), dtype=tc.int64, device=dev)
+    wt = tc.ones((ns, 1), dtype=tc.float64, device=dev)
+    stride = 1
+    for a in range(ndim):
+        u = (kd[a] + 0.5) * gshape[a]
+        cells = tc.floor(u).to(tc.int64)[:, None] - 2 + taps[None, :]
+        d = cells.to(tc.float64) - u[:, None]
+        arg = tc.clamp(1.0 - (d / hw) ** 2, min=0.0)
+        val = tc.where(tc.abs(d) < hw, tc.special.i0(beta * tc.sqrt(arg)), tc.zeros_like(d))
+        idx = tc.remainder(cells, gshape[a])
+        flat = (flat[:, :, None] + stride * idx[:, None, :]).reshape(ns, -1)
+        wt = (wt[:, :, None] * val[:, None, :]).reshape(ns, -1)
+        stride *= int(gshape[a])
+    keys = flat.reshape(-1)
+    perm = tc.argsort(keys, stable=True)
+    cells_u, counts = tc.unique_consecutive(keys[perm], return_counts=True)
+    wt_p = wt.reshape(-1)[perm]
+    sample = perm // flat.shape[1]
+    w = tc.ones(ns, dtype=tc.float64, device=dev)
+    grid = tc.zeros(stride, dtype=tc.float64, device=dev)
+    for _ in range(int(iters)):
+        grid[cells_u] = tc.segment_reduce(wt_p * w[sample], "sum", lengths=counts)
+        cw = (wt * grid[flat]).sum(dim=1)
+        w = w / tc.where(cw > 0, cw, tc.ones_like(cw))
+    return w.cpu().numpy()
 
 
 def samplingDensity(tr, shape, *, iters: int = 10, normalize: bool = False):

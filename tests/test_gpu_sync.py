@@ -175,3 +175,19 @@ def test_device_resampling_matches_the_host(intermode, gpu_lib):
         assert bd.dtype == np.float64
         assert np.max(np.abs(ah - bd)) <= 1e-12 * scale, (tau, np.max(np.abs(ah - bd)))
         assert np.max(np.abs(a.astype(np.float64) - b.astype(np.float64))) <= 2 ** -22 * scale
+
+
+def test_device_density_matches_the_host(gpu_lib):
+    """samplingDensity's Pipe-Menon iterations on the GPU (segmented sums over the
+    (sample, tap) pairs sorted by grid cell) against the host's bincount form, on a
+    16 000-sample spiral for a 200x200 grid: to 1e-12, and the same on a second call."""
+    from hoe import fieldcam
+    kx, ky = synth.spiral(16000, 200 / (2 * 0.2), 12)
+    k = np.stack([kx, ky]) / (2.0 * np.max(np.abs(np.stack([kx, ky]))))
+    dev = fieldcam._pipe_menon(k, (400, 400), 10)
+    import unittest.mock as um
+    with um.patch.object(fieldcam, "_torch_gpu", lambda: None):
+        host = fieldcam._pipe_menon(k, (400, 400), 10)
+    assert dev.shape == host.shape == (16000,)
+    assert np.max(np.abs(dev - host)) <= 1e-12 * np.max(np.abs(host))
+    assert np.array_equal(dev, fieldcam._pipe_menon(k, (400, 400), 10))
