@@ -33,7 +33,7 @@
 
 The y data of the full-size cases comes from the GPU forward with perturbed coefficients
 (1 % order 1, 5 % orders 2-3) and a 3 us delay; it is data only, the gradient formula
-holds for any y. Bars: 1e-4 relative (the north-star gradient bar) for every backend.
+holds for any y. Bars: 1e-4 relative (the gradient bar of the FP32 operator products) for every backend.
 """
 import numpy as np
 import pytest
