@@ -413,7 +413,7 @@ __device__ __forceinline__ float reduce_scatter16(float (&x)[16], int lane) {
 // [re hi | im hi | re lo | im lo] of coils 0..7, so Ahi*B + Alo*B holds all three
 // products (and the lo*lo term) in 4 MFMAs and one B fragment; column c + 16 is added
 // to column c in the epilogue.
-template <typename T16, bool FWD, bool HX, int L, int NB, int PK = 0, bool DT = false>
+template <typename T16, bool FWD, bool HX, int L, int NB, int PK = 0, bool DT = false, bool PF = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (FWD ? HOE_WAVES_FWD : HOE_WAVES_ADJ) : 2))) void k_mixed(
     const double *__restrict__ coef, int sk, const double *__restrict__ basis, int sb,
     const uint4 *__restrict__ frag, int n_ks, float2 *__restrict__ part, int ncs,
@@ -422,6 +422,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (
     typedef typename mfma_t<T16>::vec vec_t;
     static_assert(PK == 0 || NB == 1, "packed coil layouts hold one group");
     static_assert(FWD || !DT, "the derivative product is a forward product");
+    static_assert(!PF || (FWD && NB == 1 && PK == 0), "fragment prefetch: forward, one 32-coil group");
     constexpr int NQ = PK == 0 ? 4 : (PK == 1 ? 2 : 1);   // B fragments (uint4) per lane and K step
     constexpr int NA = PK == 0 ? 2 : 1;                   // accumulator tiles per row tile
     const int chunk = chunk_id(cpx);
@@ -562,6 +563,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (
                     __builtin_amdgcn_s_waitcnt(0);
                     __syncthreads();
                     bsel ^= 1;
+                }
+            } else if constexpr (PF) {
+                // PF (a forward whose K part overflows an XCD's L2): the next step's B
+                // fragments are loaded a step ahead. Where the part fits it costs 3 % instead,
+                // and as a runtime branch in one kernel it raised every instance's registers.
+                uint4 bfr[NB][NQ], nxt[NB][NQ];
+                auto fetch = [&](int ks, uint4 (&o)[NB][NQ]) {
+                    const uint4 *p = frag + (((size_t)grp * n_ks + min(ks, n_ks - 1)) * 64 + lane) * NQ;
+#pragma unroll
+                    for (int q = 0; q < NQ; ++q) o[0][q] = p[q];
+                };
+                fetch(d.y, nxt);
+                for (int ks = d.y; ks < d.z; ++ks) {
+#pragma unroll
+                    for (int q = 0; q < NQ; ++q) bfr[0][q] = nxt[0][q];
+                    fetch(ks + 1, nxt);
+                    step(ks, bfr);
                 }
             } else {
                 for (int ks = d.y; ks < d.z; ++ks) {
@@ -2135,7 +2153,13 @@ static void visit_direct(int L, int cb, F &&f) {
 // shape: coil groups per wave (1, 2) for 32-coil groups, or 3 / 4 for the packed layouts
 // of up to 16 / 8 coils (see k_mixed)
 template <typename T16, bool FWD, bool HX, int L, bool DT, class F>
-static void visit_mixed_nb(int shape, F &&f) {
+static void visit_mixed_nb(int shape, F &&f, bool pf) {
+    if constexpr (FWD && HX && !DT) {
+        if (pf && shape == 1) {
+            f(k_mixed<T16, FWD, HX, L, 1, 0, DT, true>);
+            return;
+        }
+    }
     switch (shape) {
         case 1: f(k_mixed<T16, FWD, HX, L, 1, 0, DT>); break;
         case 3: f(k_mixed<T16, FWD, HX, L, 1, 1, DT>); break;
@@ -2144,13 +2168,13 @@ static void visit_mixed_nb(int shape, F &&f) {
     }
 }
 template <typename T16, bool FWD, bool HX, bool DT = false, class F>
-static void visit_mixed(int L, int shape, F &&f) {
+static void visit_mixed(int L, int shape, F &&f, bool pf = false) {
     switch (L) {
-        case 6: visit_mixed_nb<T16, FWD, HX, 6, DT>(shape, f); break;
-        case 10: visit_mixed_nb<T16, FWD, HX, 10, DT>(shape, f); break;
-        case 12: visit_mixed_nb<T16, FWD, HX, 12, DT>(shape, f); break;
-        case 16: visit_mixed_nb<T16, FWD, HX, 16, DT>(shape, f); break;
-        default: visit_mixed_nb<T16, FWD, HX, 20, DT>(shape, f); break;
+        case 6: visit_mixed_nb<T16, FWD, HX, 6, DT>(shape, f, pf); break;
+        case 10: visit_mixed_nb<T16, FWD, HX, 10, DT>(shape, f, pf); break;
+        case 12: visit_mixed_nb<T16, FWD, HX, 12, DT>(shape, f, pf); break;
+        case 16: visit_mixed_nb<T16, FWD, HX, 16, DT>(shape, f, pf); break;
+        default: visit_mixed_nb<T16, FWD, HX, 20, DT>(shape, f, pf); break;
     }
 }
 template <int L, class F>
@@ -2237,12 +2261,14 @@ template <bool FWD, class F>
 static void visit_contract_mixed(Handle *h, int backend, F &&f, bool dt = false) {
     if constexpr (FWD) {
         const int sh = mixed_shape(h);
+        // K parts (one per XCD) longer than HOE_DP_MIN_KX units overflow an XCD's L2
+        const bool pf = (h->nv_pad / 8) / 8 > HOE_DP_MIN_KX;
         if (backend == HOE_BACKEND_MIXED_FP16) {
             if (dt) {
                 if (h->horner) visit_mixed<_Float16, true, true, true>(h->LK, sh, f);
                 else visit_mixed<_Float16, true, false, true>(h->LK, sh, f);
             } else if (h->horner) {
-                visit_mixed<_Float16, true, true>(h->LK, sh, f);
+                visit_mixed<_Float16, true, true>(h->LK, sh, f, pf);
             } else {
                 visit_mixed<_Float16, true, false>(h->LK, sh, f);
             }
@@ -2251,7 +2277,7 @@ static void visit_contract_mixed(Handle *h, int backend, F &&f, bool dt = false)
                 if (h->horner) visit_mixed<__bf16, true, true, true>(h->LK, sh, f);
                 else visit_mixed<__bf16, true, false, true>(h->LK, sh, f);
             } else if (h->horner) {
-                visit_mixed<__bf16, true, true>(h->LK, sh, f);
+                visit_mixed<__bf16, true, true>(h->LK, sh, f, pf);
             } else {
                 visit_mixed<__bf16, true, false>(h->LK, sh, f);
             }
