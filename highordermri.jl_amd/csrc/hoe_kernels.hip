@@ -1130,14 +1130,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                 my[((kc * 2 + t) * 2 + 1) * 64] = src[1];
             }
         f16x_t g[2] = {f16x_t{}, f16x_t{}};
-        // Horner: t of the lane's sample; generic: its full record
-        double st[HX ? 1 : L + 1];
+        // Horner: t of the lane's sample, and its L coefficients for the run setups (in
+        // registers: re-read at every run change, the records of all sample tiles competed
+        // with the Z fragments for L2); generic: its full record
+        double st[HX ? 1 : L + 1], rec[HX ? L : 1];
         double hc[4] = {0.0, 0.0, 0.0, 0.0};
         float c1f = 0.0f, c2f = 0.0f, c2x2f = 0.0f, c3f = 0.0f, c3x3f = 0.0f;
         int cur = -1;
         const bool anch = HX && anchored_on(hx);
         if (HX) {
             st[0] = coef[(size_t)row * sk + hx.t_index];
+#pragma unroll
+            for (int i = 0; i < L; ++i) rec[i] = coef[(size_t)row * sk + i];
         } else {
 #pragma unroll
             for (int i = 0; i <= L; ++i) st[i] = coef[(size_t)row * sk + i];
@@ -1192,7 +1196,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                     const int r0 = a0.run;
                     if (r0 >= 0 && a1.run == r0 && anch) {
                         if (r0 != cur) {
-                            run_setup<L>(coef + (size_t)row * sk, hx.beta + (size_t)r0 * L * 4, hc);
+                            run_setup<L>(rec, hx.beta + (size_t)r0 * L * 4, hc);
                             c1f = (float)hc[1];
                             c2f = (float)hc[2];
                             c2x2f = (float)(2.0 * hc[2]);
@@ -1221,10 +1225,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
                     } else {
                         // a run boundary inside the half block (or anchored phases off):
                         // full records (generic phase)
-                        const double *ap = coef + (size_t)row * sk;
                         double a[L + 1];
 #pragma unroll
-                        for (int i = 0; i <= L; ++i) a[i] = ap[i];
+                        for (int i = 0; i < L; ++i) a[i] = rec[i];
+                        a[L] = coef[(size_t)row * sk + L];
 #pragma unroll
                         for (int j = 0; j < 16; ++j) {
                             const double *sm = basis + (size_t)(q0 + j) * sb;
