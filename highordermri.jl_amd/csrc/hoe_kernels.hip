@@ -431,12 +431,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NB == 1 ? (
     const int wave = threadIdx.x >> 6;
     const int sg0 = chunk_off[chunk], sg1 = chunk_off[chunk + 1];
     constexpr int NST = FWD ? L + 1 : L;
-    // Adjoint with one coil group and 16 record terms: B fragments staged in LDS, S K
+    // Adjoint with one coil group and 16 or 20 record terms: B fragments staged in LDS, S K
     // steps per chunk. Staging removes the L1 misses of the fragment loads (the 4 waves
     // read the same fragments) but its barrier drains the Horner setup's prefetch each
-    // chunk; measured, that nets -5 % on a 16-term 3-D workload and +4.5 % on a 10-term
-    // 2-D one, so the other record lengths keep per-wave loads.
-    constexpr bool STAGE = !FWD && NB == 1 && HOE_ADJ_LDS && L == 16;
+    // chunk; measured, that takes 5 % off a 16-term 3-D adjoint and 1.2 % off a 20-term
+    // 2-D one, and adds 4.5 % to a 10-term 2-D one, so the shorter records keep per-wave
+    // loads. (At 20 terms the workgroup's 52.5 KiB of LDS still admits 3 per CU.)
+    constexpr bool STAGE = !FWD && NB == 1 && HOE_ADJ_LDS && (L == 16 || L == 20);
     constexpr int S = 4;
     __shared__ uint4 fbuf[STAGE ? 2 : 1][STAGE ? S * 256 : 1];
     for (int sgi = sg0; sgi < sg1; ++sgi) {
